@@ -1,0 +1,301 @@
+"""icm_amd -- MI355X-native ICM equity engine (ctypes front end of libicm_hip.so).
+
+Mirrors the reference Python package (python/icm/__init__.py): ``init``,
+``equity`` and ``equity_subset`` with the same argument meaning, validation
+and error messages, plus the reusable plan of the reference GPU header
+(src/gpu/icm_gpu.h:79-111) as :class:`Plan`.  Every computation runs in the
+HIP library; there is no CPU fallback: if the library is missing, loading
+raises :class:`ICMError`.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["ICMError", "init", "equity", "equity_subset", "node_inner", "Plan",
+           "plan_geometry", "make_nodes", "library_path", "load_library"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libicm_hip.so"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class ICMError(Exception):
+    """Error raised by the ICM library (mirrors python/icm/__init__.py)."""
+
+
+class Options(ctypes.Structure):
+    """Layout of IcmHipOptions (= IcmGpuOptions, src/gpu/icm_gpu.h:12-34)."""
+    _fields_ = [("device_id", ctypes.c_int), ("use_cufftdx", ctypes.c_int),
+                ("enable_graphs", ctypes.c_int), ("enable_q_pipeline", ctypes.c_int),
+                ("memory_strategy", ctypes.c_int),
+                ("force_uncached_fused_levels", ctypes.c_int),
+                ("force_uncached_cufft_levels", ctypes.c_int),
+                ("fast_mode", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class RunStats(ctypes.Structure):
+    """Layout of IcmHipRunStats (= IcmGpuRunStats, src/gpu/icm_gpu.h:36-50)."""
+    _fields_ = [("total_ns", ctypes.c_double), ("sort_ns", ctypes.c_double),
+                ("quadrature_overhead_ns", ctypes.c_double),
+                ("block_build_ns", ctypes.c_double), ("tree_build_ns", ctypes.c_double),
+                ("tree_propagate_cached_ns", ctypes.c_double),
+                ("tree_propagate_recomputed_ns", ctypes.c_double),
+                ("leaf_extract_ns", ctypes.c_double), ("peak_vram_bytes", ctypes.c_size_t),
+                ("engine", ctypes.c_int), ("B", ctypes.c_int),
+                ("uncached_fused_levels", ctypes.c_int),
+                ("uncached_cufft_levels", ctypes.c_int)]
+
+
+class PlanSummary(ctypes.Structure):
+    """Layout of IcmHipPlanSummary (= IcmGpuPlanSummary, src/gpu/icm_gpu.h:52-63)."""
+    _fields_ = [("n", ctypes.c_int), ("k", ctypes.c_int), ("B", ctypes.c_int),
+                ("engine", ctypes.c_int), ("n_levels", ctypes.c_int),
+                ("n_tier1", ctypes.c_int), ("n_tier2", ctypes.c_int),
+                ("n_tier3", ctypes.c_int), ("q_batch", ctypes.c_int),
+                ("planned_peak_vram_bytes", ctypes.c_size_t)]
+
+
+# (name, restype, argtypes) of every symbol declared in include/icm_hip.h
+_SIGS = [
+    ("icm_hip_init", ctypes.c_int, [ctypes.c_int]),
+    ("icm_hip_shutdown", None, []),
+    ("icm_hip_last_error", ctypes.c_char_p, []),
+    ("icm_hip_release_pooled_memory", ctypes.c_int, []),
+    ("icm_hip_plan_create", ctypes.c_void_p, [ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(Options)]),
+    ("icm_hip_plan_destroy", None, [ctypes.c_void_p]),
+    ("icm_hip_plan_summary", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PlanSummary)]),
+    ("icm_hip_equity_with_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.POINTER(RunStats)]),
+    ("icm_hip_equity_with_plan_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RunStats)]),
+    ("icm_hip_equity", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
+    ("icm_hip_equity_subset", ctypes.c_int,
+     [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int]),
+    ("icm_hip_node_inner", ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_int, _D]),
+    ("icm_gpu_init", ctypes.c_int, [ctypes.c_int]),
+    ("icm_gpu_shutdown", None, []),
+    ("icm_gpu_last_error", ctypes.c_char_p, []),
+    ("icm_gpu_release_pooled_memory", ctypes.c_int, []),
+    ("icm_gpu_plan_create", ctypes.c_void_p, [ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(Options)]),
+    ("icm_gpu_plan_destroy", None, [ctypes.c_void_p]),
+    ("icm_gpu_plan_summary", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PlanSummary)]),
+    ("icm_gpu_equity_with_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.POINTER(RunStats)]),
+    ("icm_gpu_equity", ctypes.c_int,
+     [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, ctypes.POINTER(Options), ctypes.POINTER(RunStats)]),
+    ("icm_gpu_equity_subset", ctypes.c_int,
+     [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int,
+      ctypes.POINTER(Options), ctypes.POINTER(RunStats)]),
+    ("icm_init", None, [ctypes.c_char_p]),
+    ("icm_equity", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
+    ("icm_equity_subset", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int]),
+    ("icm_hip_plan_geometry", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I]),
+    ("icm_hip_make_nodes", None, [ctypes.c_int, ctypes.c_double, _D, _D]),
+]
+
+_lib = None
+
+
+def library_path():
+    """Path of the in-tree library ($ICM_HIP_LIB overrides)."""
+    return os.environ.get("ICM_HIP_LIB", os.path.join(HERE, _LIB_NAME))
+
+
+def load_library():
+    """Load libicm_hip.so and bind every C-ABI symbol (raises ICMError)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.isfile(path):
+        raise ICMError(
+            f"Cannot find {_LIB_NAME} at {path}. Build it with 'python -m icm_amd.build'.")
+    try:
+        lib = ctypes.CDLL(path)
+    except OSError as e:
+        raise ICMError(f"Failed to load ICM library from {path}: {e}") from e
+    for name, res, args in _SIGS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _err(lib, what):
+    msg = lib.icm_hip_last_error()
+    return ICMError(f"{what} failed: {msg.decode() if msg else 'unknown error'}")
+
+
+def _to_double_array(data, name):
+    arr = np.ascontiguousarray(data, dtype=np.float64)
+    if arr.ndim != 1:
+        raise ValueError(f"{name} must be a 1-D array, got shape {arr.shape}")
+    if arr.size == 0:
+        raise ValueError(f"{name} must not be empty")
+    return arr
+
+
+def _to_int_array(data, name):
+    arr = np.ascontiguousarray(data, dtype=np.int32)
+    if arr.ndim != 1:
+        raise ValueError(f"{name} must be a 1-D array, got shape {arr.shape}")
+    if arr.size == 0:
+        raise ValueError(f"{name} must not be empty")
+    return arr
+
+
+def _ptr(arr, ctype=ctypes.c_double):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+_initialized = False
+
+
+def init(wisdom_path=None, device_id=0):
+    """Select the GPU (python/icm/__init__.py:155-178; wisdom_path is accepted
+    for signature compatibility and ignored: no FFT plans are loaded)."""
+    global _initialized
+    lib = load_library()
+    if lib.icm_hip_init(int(device_id)) != 1:
+        raise _err(lib, "icm_hip_init")
+    _initialized = True
+
+
+def _auto_init():
+    if not _initialized:
+        init()
+
+
+def _validate(n, k, Q):
+    if k < 1 or k > n:
+        raise ValueError(f"payouts length k={k} must satisfy 1 <= k <= n={n}")
+    if Q < 1:
+        raise ValueError(f"Q must be positive, got {Q}")
+
+
+def equity(stacks, payouts, Q=256):
+    """ICM equities for all players (python/icm/__init__.py:191-235)."""
+    _auto_init()
+    s = _to_double_array(stacks, "stacks")
+    p = _to_double_array(payouts, "payouts")
+    n, k = len(s), len(p)
+    _validate(n, k, Q)
+    out = np.zeros(n, dtype=np.float64)
+    lib = load_library()
+    if lib.icm_hip_equity(n, _ptr(s), int(Q), _ptr(p), k, _ptr(out)) != 0:
+        raise _err(lib, "icm_hip_equity")
+    return out
+
+
+def equity_subset(stacks, payouts, targets, Q=256):
+    """Equities for target players only; others are 0 (python/icm/__init__.py:238-292)."""
+    _auto_init()
+    s = _to_double_array(stacks, "stacks")
+    p = _to_double_array(payouts, "payouts")
+    t = _to_int_array(targets, "targets")
+    n, k = len(s), len(p)
+    _validate(n, k, Q)
+    if len(t) < 1:
+        raise ValueError("targets must not be empty")
+    if np.any(t < 0) or np.any(t >= n):
+        raise ValueError(f"target indices must be in [0, {n-1}]")
+    out = np.zeros(n, dtype=np.float64)
+    lib = load_library()
+    if lib.icm_hip_equity_subset(n, _ptr(s), int(Q), _ptr(p), k, _ptr(out),
+                                 _ptr(t, ctypes.c_int), len(t)) != 0:
+        raise _err(lib, "icm_hip_equity_subset")
+    return out
+
+
+def node_inner(a, payouts):
+    """Psi_i of one quadrature node for given a_i (GPU tree, given order)."""
+    _auto_init()
+    a = _to_double_array(a, "a")
+    p = _to_double_array(payouts, "payouts")
+    n, k = len(a), len(p)
+    _validate(n, k, 1)
+    out = np.zeros(n)
+    lib = load_library()
+    if lib.icm_hip_node_inner(n, _ptr(a), _ptr(p), k, _ptr(out)) != 0:
+        raise _err(lib, "icm_hip_node_inner")
+    return out
+
+
+class Plan:
+    """Reusable plan (IcmGpuPlan, src/gpu/icm_gpu.h:79-111): sort, geometry and
+    device memory are set up once for (stacks, k); ``run`` takes Q and payout."""
+
+    def __init__(self, stacks, k, block_size=0, q_batch=0, verbose=False):
+        _auto_init()
+        self._lib = load_library()
+        s = _to_double_array(stacks, "stacks")
+        self.n, self.k = len(s), int(k)
+        _validate(self.n, self.k, 1)
+        opts = Options()
+        opts.force_uncached_fused_levels = int(block_size)
+        opts.force_uncached_cufft_levels = int(q_batch)
+        opts.verbose = int(bool(verbose))
+        self._h = self._lib.icm_hip_plan_create(self.n, _ptr(s), self.k, ctypes.byref(opts))
+        if not self._h:
+            raise _err(self._lib, "icm_hip_plan_create")
+        self.stats = RunStats()
+
+    def summary(self):
+        sm = PlanSummary()
+        if self._lib.icm_hip_plan_summary(self._h, ctypes.byref(sm)) != 0:
+            raise _err(self._lib, "icm_hip_plan_summary")
+        return {f: getattr(sm, f) for f, _ in PlanSummary._fields_}
+
+    def run(self, payouts, Q=256):
+        p = _to_double_array(payouts, "payouts")
+        if len(p) != self.k:
+            raise ValueError(f"payouts length {len(p)} != plan k={self.k}")
+        if Q < 1:
+            raise ValueError(f"Q must be positive, got {Q}")
+        out = np.zeros(self.n)
+        if self._lib.icm_hip_equity_with_plan(self._h, int(Q), _ptr(p), _ptr(out),
+                                              ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_equity_with_plan")
+        return out
+
+    def run_device(self, d_payout_ptr, d_equity_ptr, Q=256):
+        """Device-resident run: raw device pointers (e.g. torch .data_ptr())."""
+        if self._lib.icm_hip_equity_with_plan_device(self._h, int(Q), ctypes.c_void_p(d_payout_ptr),
+                                                     ctypes.c_void_p(d_equity_ptr),
+                                                     ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_equity_with_plan_device")
+        return self.stats.total_ns
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.icm_hip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_geometry(n, k, B=32, max_levels=64):
+    """Host-only planner query: per-level (n_real, c, G, F)."""
+    lib = load_library()
+    arrs = [np.zeros(max_levels, dtype=np.int32) for _ in range(4)]
+    L = lib.icm_hip_plan_geometry(int(n), int(k), int(B), max_levels,
+                                  *[_ptr(a, ctypes.c_int) for a in arrs])
+    if L < 0:
+        raise ValueError("invalid geometry arguments")
+    return {name: a[:L].copy() for name, a in zip(("n_real", "c", "G", "F"), arrs)}
+
+
+def make_nodes(Q, Smax):
+    """(logv, w) of the erfc-trap rule computed by the library (host code)."""
+    lib = load_library()
+    logv = np.zeros(Q)
+    w = np.zeros(Q)
+    lib.icm_hip_make_nodes(int(Q), float(Smax), _ptr(logv), _ptr(w))
+    return logv, w

@@ -1,0 +1,173 @@
+// fft_lds.hpp -- workgroup-cooperative FP64 complex FFTs on an LDS tile.
+//
+// One workgroup of 256 threads (four 64-lane wavefronts) owns a tile of 4096
+// complex doubles (64 KiB of LDS).  The tile holds 4096/FS independent
+// transforms of size FS, each contiguous.  Every Stockham pass gives each
+// thread 16 points: one radix-16 butterfly, or 16/r butterflies of radix r
+// for the one mixed pass when log2(FS) is not a multiple of 4.  Data goes
+// LDS -> registers -> butterfly -> LDS, two barriers per pass.
+//
+// Twiddles come from a table tw[m] = exp(-2 pi i m / 2^twShift) computed on
+// the host from long-double cos/sin (one rounding per entry), so every
+// twiddle is the correctly rounded FP64 value; no on-device sin/cos.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace icm {
+
+constexpr int kTile = 4096;   // complex points per workgroup tile
+constexpr int kThreads = 256; // threads per workgroup
+constexpr int kPer = kTile / kThreads;  // 16 points per thread
+
+__device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ double2 c_mulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 c_conj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 c_scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+// Multiply by the forward (-i) or inverse (+i) quarter turn.
+template <bool INV> __device__ __forceinline__ double2 mul_q(double2 a) {
+    return INV ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+// Multiply by the constant (cr, ci) for the forward direction; conjugate for inverse.
+template <bool INV> __device__ __forceinline__ double2 mul_k(double2 a, double cr, double ci) {
+    const double c_i = INV ? -ci : ci;
+    return make_double2(a.x * cr - a.y * c_i, a.x * c_i + a.y * cr);
+}
+
+// cos(pi/8), sin(pi/8), sqrt(2)/2 rounded to FP64.
+constexpr double kC8 = 0.92387953251128675613;
+constexpr double kS8 = 0.38268343236508977173;
+constexpr double kR2 = 0.70710678118654752440;
+
+template <bool INV> __device__ __forceinline__ void dft2(double2 &a0, double2 &a1) {
+    double2 t = a0;
+    a0 = c_add(t, a1);
+    a1 = c_sub(t, a1);
+}
+
+template <bool INV> __device__ __forceinline__ void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3) {
+    double2 t0 = c_add(a0, a2), t1 = c_sub(a0, a2);
+    double2 t2 = c_add(a1, a3), t3 = mul_q<INV>(c_sub(a1, a3));
+    a0 = c_add(t0, t2);
+    a2 = c_sub(t0, t2);
+    a1 = c_add(t1, t3);
+    a3 = c_sub(t1, t3);
+}
+
+template <bool INV> __device__ __forceinline__ void dft8(double2 *a) {
+    dft4<INV>(a[0], a[2], a[4], a[6]);
+    dft4<INV>(a[1], a[3], a[5], a[7]);
+    a[3] = mul_k<INV>(a[3], kR2, -kR2);
+    a[5] = mul_q<INV>(a[5]);
+    a[7] = mul_k<INV>(a[7], -kR2, -kR2);
+    double2 x0 = c_add(a[0], a[1]), x4 = c_sub(a[0], a[1]);
+    double2 x1 = c_add(a[2], a[3]), x5 = c_sub(a[2], a[3]);
+    double2 x2 = c_add(a[4], a[5]), x6 = c_sub(a[4], a[5]);
+    double2 x3 = c_add(a[6], a[7]), x7 = c_sub(a[6], a[7]);
+    a[0] = x0; a[1] = x1; a[2] = x2; a[3] = x3;
+    a[4] = x4; a[5] = x5; a[6] = x6; a[7] = x7;
+}
+
+// 16-point DFT as 4 x 4: X[k1 + 4 k2] = sum_s w4^(s k2) w16^(s k1) DFT4_u(a[s + 4u])[k1].
+template <bool INV> __device__ __forceinline__ void dft16(double2 *a) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) dft4<INV>(a[s], a[s + 4], a[s + 8], a[s + 12]);
+    // a[s + 4 k1] *= w16^(s k1)
+    a[5]  = mul_k<INV>(a[5],  kC8, -kS8);    // s=1,k1=1: w1
+    a[9]  = mul_k<INV>(a[9],  kR2, -kR2);    // s=1,k1=2: w2
+    a[13] = mul_k<INV>(a[13], kS8, -kC8);    // s=1,k1=3: w3
+    a[6]  = mul_k<INV>(a[6],  kR2, -kR2);    // s=2,k1=1: w2
+    a[10] = mul_q<INV>(a[10]);               // s=2,k1=2: w4
+    a[14] = mul_k<INV>(a[14], -kR2, -kR2);   // s=2,k1=3: w6
+    a[7]  = mul_k<INV>(a[7],  kS8, -kC8);    // s=3,k1=1: w3
+    a[11] = mul_k<INV>(a[11], -kR2, -kR2);   // s=3,k1=2: w6
+    a[15] = mul_k<INV>(a[15], -kC8, kS8);    // s=3,k1=3: w9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
+    // X[k1 + 4 k2] currently sits at a[4 k1 + k2]: transpose 4x4.
+    double2 t[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = a[4 * k1 + k2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a[i] = t[i];
+}
+
+template <int R, bool INV> __device__ __forceinline__ void dft(double2 *a) {
+    if constexpr (R == 2) dft2<INV>(a[0], a[1]);
+    else if constexpr (R == 4) dft4<INV>(a[0], a[1], a[2], a[3]);
+    else if constexpr (R == 8) dft8<INV>(a);
+    else dft16<INV>(a);
+}
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+// One Stockham pass of radix R over all transforms of size FS in the tile.
+// Ns = product of the radices of earlier passes.
+template <int FS, int R, bool INV>
+__device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 *__restrict__ tw, int twShift) {
+    constexpr int NBI = FS / R;               // butterflies per transform
+    constexpr int LOG_NBI = ilog2c(NBI);
+    constexpr int PER = kPer / R;             // butterflies per thread
+    const int t = threadIdx.x;
+    const int Ns = 1 << logNs;
+    double2 v[PER][R];
+    int item[PER], jj[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int bj = t + u * kThreads;
+        item[u] = bj >> LOG_NBI;
+        jj[u] = bj & (NBI - 1);
+        const double2 *src = lds + item[u] * FS + jj[u];
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[u][s] = src[s * NBI];
+    }
+    __syncthreads();
+    const int twSh = twShift - (logNs + ilog2c(R));
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int k = jj[u] & (Ns - 1);
+        if (logNs > 0) {
+#pragma unroll
+            for (int s = 1; s < R; ++s) {
+                double2 w = tw[(k * s) << twSh];
+                if (INV) w.y = -w.y;
+                v[u][s] = c_mul(v[u][s], w);
+            }
+        }
+        dft<R, INV>(v[u]);
+        double2 *dst = lds + item[u] * FS + (jj[u] - k) * R + k;
+#pragma unroll
+        for (int s = 0; s < R; ++s) dst[s * Ns] = v[u][s];
+    }
+    __syncthreads();
+}
+
+// In-place transform of every size-FS block in the 4096-point tile.
+// Starts with a barrier so callers may write the tile and call directly;
+// ends with a barrier (the last pass's).
+template <int FS, bool INV>
+__device__ __forceinline__ void fft_tile(double2 *lds, const double2 *__restrict__ tw, int twShift) {
+    constexpr int LOG = ilog2c(FS);
+    constexpr int REM = LOG % 4;
+    __syncthreads();
+    int logNs = 0;
+    if constexpr (REM == 1) { fft_pass<FS, 2, INV>(lds, logNs, tw, twShift); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass<FS, 4, INV>(lds, logNs, tw, twShift); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass<FS, 8, INV>(lds, logNs, tw, twShift); logNs += 3; }
+#pragma unroll
+    for (int p = 0; p < LOG / 4; ++p) {
+        fft_pass<FS, 16, INV>(lds, logNs, tw, twShift);
+        logNs += 4;
+    }
+}
+
+}  // namespace icm

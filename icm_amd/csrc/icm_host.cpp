@@ -1,0 +1,613 @@
+// icm_host.cpp -- planner, plan lifecycle and the C ABI of libicm_hip.so.
+//
+// Tree geometry (this engine's own; the reference's is src/cpu/icm.c:1205-1263
+// and src/gpu/gpu_cost_model.cu:494-572):
+//   nb = ceil(n / B) blocks, N = next_pow2(nb), L = log2(N) + 1 levels,
+//   n_real[0] = nb, n_real[l] = ceil(n_real[l-1] / 2)   (ragged: no padding nodes)
+//   D_l = B 2^l   (largest degree of a level-l node)
+//   c_l = min(D_l + 1, k)   coefficients kept per node (product mod x^k)
+//   G_l = min(D_l, k)       g entries per node (pairs with leave-one-out
+//                            products of degree <= D_l - 1, truncated at k)
+//   F_l = next_pow2(max(2 c_{l-1} - 2, G_{l-1} + c_{l-1} - 1, 16))
+// F_l serves both passes of level l: the build product of two c_{l-1}-term
+// polynomials aliases at most coefficient 0 (fixed exactly from the children),
+// and the propagate correlation g_child[m] = sum_t P_sib[t] g_par[m + t],
+// m < G_{l-1}, never wraps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/icm_hip.h"
+#include "icm_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+int g_device = -1;
+std::mutex g_mu;
+
+void set_err(const std::string &s) { g_err = s; }
+
+bool hip_ok(hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    set_err(buf);
+    return false;
+}
+#define HIP_OK(x) hip_ok((x), #x)
+
+int next_pow2_ll(long long v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+int ilog2i(long long v) {
+    int l = 0;
+    while ((1LL << l) < v) ++l;
+    return l;
+}
+
+// erfc-trap nodes: src/cpu/icm.c:117-132 (h = 0 for Q = 1 as src/gpu/gpu_exec.cu:17).
+double log_Phi(double y) {
+    if (y >= 0) return std::log1p(-std::erfc(y / std::sqrt(2.0)) / 2.0);
+    return std::log(std::erfc(-y / std::sqrt(2.0)) / 2.0);
+}
+void make_nodes(int Q, double Smax, double *logv, double *w) {
+    const double y_lo = -7.7;
+    double y_hi = std::sqrt(2.0) * std::sqrt(std::log(Smax) + 25.0);
+    if (y_hi < 6.5) y_hi = 6.5;
+    const double h = (Q > 1) ? (y_hi - y_lo) / (Q - 1) : 0.0;
+    for (int q = 0; q < Q; ++q) {
+        const double y = y_lo + q * h;
+        logv[q] = log_Phi(y);
+        w[q] = h * std::exp(-0.5 * y * y) / std::sqrt(2.0 * M_PI);
+    }
+}
+
+struct Geometry {
+    int n = 0, k = 0, B = 0, nb = 0, L = 0;
+    std::vector<int> n_real, c, G, F;
+};
+
+bool make_geometry(int n, int k, int B, Geometry &g) {
+    if (n < 1 || k < 1 || k > n || (B != 16 && B != 32 && B != 64)) return false;
+    g.n = n; g.k = k; g.B = B;
+    g.nb = (n + B - 1) / B;
+    const int N = next_pow2_ll(g.nb);
+    g.L = ilog2i(N) + 1;
+    g.n_real.assign(g.L, 0); g.c.assign(g.L, 0); g.G.assign(g.L, 0); g.F.assign(g.L, 0);
+    g.n_real[0] = g.nb;
+    for (int l = 1; l < g.L; ++l) g.n_real[l] = (g.n_real[l - 1] + 1) / 2;
+    for (int l = 0; l < g.L; ++l) {
+        const long long D = (long long)B << l;
+        g.c[l] = (int)std::min<long long>(D + 1, k);
+        g.G[l] = (int)std::min<long long>(D, k);
+    }
+    for (int l = 1; l < g.L; ++l) {
+        const long long need = std::max<long long>(
+            {2LL * g.c[l - 1] - 2, (long long)g.G[l - 1] + g.c[l - 1] - 1, 16LL});
+        g.F[l] = next_pow2_ll(need);
+    }
+    return true;
+}
+
+int pick_B(int n, int k, int forced) {
+    if (forced == 16 || forced == 32 || forced == 64) return forced;
+    (void)k;
+    if (n <= 16) return 16;
+    return 32;
+}
+
+}  // namespace
+
+struct IcmHipPlan {
+    Geometry geo;
+    int qb = 0;                         // q-planes per batch
+    int nchunk = 1;                     // q-chunks of the leaf extract
+    int twShift = 0;
+    std::vector<int> perm;              // sorted position -> original index
+    std::vector<double> S_sorted;
+    double Smax = 0.0;
+    bool verbose = false;
+    // device memory
+    double2 *d_tw = nullptr;
+    double *d_S = nullptr;
+    int *d_perm = nullptr;
+    double *d_payout = nullptr;          // k (copied per call)
+    double *d_nodes = nullptr;           // [2][Qcap]: logv, w
+    int q_cap = 0;
+    double *d_blocks = nullptr;          // [qb][nb][B+1]
+    std::vector<double *> d_coef;        // level l (1..L-2): [qb][n_real][c]
+    double *d_g[2] = {nullptr, nullptr};
+    long long g_elems = 0;               // per q-plane
+    double2 *d_bufA = nullptr, *d_bufB = nullptr;
+    long long big_elems = 0;             // complex per q-plane per buffer
+    double *d_partial = nullptr;         // [nchunk][n]
+    double *d_acc = nullptr;             // [n] sorted order
+    double *d_equity = nullptr;          // [n] original order
+    unsigned char *d_active = nullptr;   // [n] sorted order or null
+    const double *d_a_in = nullptr;      // optional a values [qb][n]
+    bool raw_node = false;               // single node with logv = 0, w = 1 (per-node entry)
+    double *d_inner = nullptr;           // optional Psi output [qb][n]
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t bytes = 0;
+};
+
+namespace {
+
+template <class T>
+bool dalloc(IcmHipPlan *p, T **ptr, size_t count) {
+    if (count == 0) count = 1;
+    size_t b = count * sizeof(T);
+    if (!hip_ok(hipMalloc((void **)ptr, b), "hipMalloc")) return false;
+    p->bytes += b;
+    return true;
+}
+
+void plan_free(IcmHipPlan *p) {
+    if (!p) return;
+    auto f = [](void *x) { if (x) (void)hipFree(x); };
+    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_blocks);
+    for (double *c : p->d_coef) f(c);
+    f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_partial); f(p->d_acc);
+    f(p->d_equity); f(p->d_active); f(p->d_inner);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+bool ensure_device() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_device >= 0) return HIP_OK(hipSetDevice(g_device));
+    int cnt = 0;
+    if (!HIP_OK(hipGetDeviceCount(&cnt))) return false;
+    if (cnt < 1) { set_err("no HIP device"); return false; }
+    if (!HIP_OK(hipSetDevice(0))) return false;
+    g_device = 0;
+    return true;
+}
+
+// Per-q-plane device bytes of the tree for this geometry.
+long long plane_bytes(const Geometry &g, long long *g_elems, long long *big_elems) {
+    long long bytes = (long long)g.nb * (g.B + 1) * 8;
+    long long ge = 0, be = 0;
+    for (int l = 1; l <= g.L - 2; ++l) bytes += (long long)g.n_real[l] * g.c[l] * 8;
+    for (int l = 0; l < g.L; ++l) ge = std::max(ge, (long long)g.n_real[l] * g.G[l]);
+    for (int l = 1; l < g.L; ++l) {
+        if (g.F[l] > icm::kMaxLdsF) {
+            const long long nrp = g.n_real[l], np = (nrp + 1) / 2;
+            be = std::max(be, (nrp + np) * (long long)g.F[l]);
+        }
+    }
+    bytes += 2 * ge * 8 + 2 * be * 16;
+    *g_elems = ge;
+    *big_elems = be;
+    return bytes;
+}
+
+IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts, bool sort_players,
+                        int max_q = 256) {
+    if (!S || n < 1 || k < 1 || k > n) { set_err("invalid arguments: need 1 <= k <= n"); return nullptr; }
+    for (int i = 0; i < n; ++i) {
+        if (!(S[i] > 0.0) || !std::isfinite(S[i])) { set_err("stacks must be positive and finite"); return nullptr; }
+    }
+    if (!ensure_device()) return nullptr;
+    IcmHipPlan *p = new IcmHipPlan();
+    const int forcedB = opts ? opts->force_uncached_fused_levels : 0;
+    const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
+    p->verbose = opts && opts->verbose;
+    const int B = pick_B(n, k, forcedB);
+    if (!make_geometry(n, k, B, p->geo)) { set_err("invalid geometry"); delete p; return nullptr; }
+    const Geometry &g = p->geo;
+
+    // Stack-descending order, index tie-break (src/gpu/gpu_api.cu:492-502).
+    p->perm.resize(n);
+    for (int i = 0; i < n; ++i) p->perm[i] = i;
+    if (sort_players)
+        std::sort(p->perm.begin(), p->perm.end(), [S](int a, int b) {
+            return S[a] > S[b] || (S[a] == S[b] && a < b);
+        });
+    p->S_sorted.resize(n);
+    for (int i = 0; i < n; ++i) p->S_sorted[i] = S[p->perm[i]];
+    p->Smax = *std::max_element(S, S + n);
+
+    long long ge = 0, be = 0;
+    const long long pb = plane_bytes(g, &ge, &be);
+    p->g_elems = ge;
+    p->big_elems = be;
+    size_t freeb = 0, totb = 0;
+    if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { plan_free(p); return nullptr; }
+    long long budget = (long long)(0.6 * (double)freeb);
+    int qb = (int)std::max<long long>(1, budget / std::max<long long>(pb, 1));
+    qb = std::min(qb, std::max(1, std::min(max_q, 256)));
+    if (forcedQ > 0) qb = std::min(qb, forcedQ);
+    // launch-shape limits: row passes put (q-plane x array) on grid.y
+    for (int l = 1; l < g.L; ++l) {
+        if (g.F[l] > icm::kMaxLdsF) {
+            const long long arrays = (long long)g.n_real[l] + (g.n_real[l] + 1) / 2;
+            while (qb > 1 && arrays * qb > 65535) --qb;
+            if (arrays > 65535) { set_err("level too wide for the row-pass grid"); delete p; return nullptr; }
+        }
+    }
+    p->qb = qb;
+
+    int maxF = 16;
+    for (int l = 1; l < g.L; ++l) maxF = std::max(maxF, g.F[l]);
+    maxF = std::max(maxF, icm::kMaxLdsF);
+    p->twShift = ilog2i(maxF);
+
+    // leaf-extract q-chunks: enough workgroups to fill the chip
+    const int bpw = 256 / B;
+    const int wg_blocks = (g.nb + bpw - 1) / bpw;
+    int nchunk = std::max(1, 4096 / std::max(1, wg_blocks));
+    p->nchunk = std::min(nchunk, qb);
+
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
+              HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1));
+    ok = ok && dalloc(p, &p->d_tw, (size_t)maxF) && dalloc(p, &p->d_S, n) && dalloc(p, &p->d_perm, n) &&
+         dalloc(p, &p->d_payout, k) && dalloc(p, &p->d_blocks, (size_t)qb * g.nb * (B + 1)) &&
+         dalloc(p, &p->d_g[0], (size_t)qb * ge) && dalloc(p, &p->d_g[1], (size_t)qb * ge) &&
+         dalloc(p, &p->d_partial, (size_t)p->nchunk * n) && dalloc(p, &p->d_acc, n) &&
+         dalloc(p, &p->d_equity, n);
+    if (ok && be > 0) ok = dalloc(p, &p->d_bufA, (size_t)qb * be) && dalloc(p, &p->d_bufB, (size_t)qb * be);
+    p->d_coef.assign(g.L, nullptr);
+    for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
+    if (!ok) { plan_free(p); return nullptr; }
+
+    // twiddles exp(-2 pi i m / maxF), rounded once from long double
+    std::vector<double2> tw(maxF);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < maxF; ++m) {
+        long double ang = -2.0L * PI * (long double)m / (long double)maxF;
+        tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * maxF, hipMemcpyHostToDevice)) &&
+         HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice)) &&
+         HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (!ok) { plan_free(p); return nullptr; }
+
+    if (p->verbose) {
+        fprintf(stderr, "[icm_hip] n=%d k=%d B=%d nb=%d L=%d qb=%d nchunk=%d bytes=%.3f GB\n", n, k, B, g.nb,
+                g.L, qb, p->nchunk, p->bytes / 1e9);
+        for (int l = 0; l < g.L; ++l)
+            fprintf(stderr, "  l=%2d n_real=%8d c=%8d G=%8d F=%8d\n", l, g.n_real[l], g.c[l], g.G[l], g.F[l]);
+    }
+    return p;
+}
+
+bool ensure_nodes(IcmHipPlan *p, int Qa) {
+    if (Qa <= p->q_cap) return true;
+    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
+    if (!dalloc(p, &p->d_nodes, (size_t)2 * Qa)) return false;
+    p->q_cap = Qa;
+    return true;
+}
+
+// Runs the whole quadrature for the active nodes with device payout; leaves
+// the result in d_acc (sorted order) and d_equity (original order).
+int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats) {
+    const Geometry &g = p->geo;
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    // nodes (host), compacted to the active ones (w != 0, src/cpu/icm.c:2558)
+    std::vector<double> logv(Q), w(Q);
+    make_nodes(Q, p->Smax, logv.data(), w.data());
+    if (p->raw_node) { logv.assign(1, 0.0); w.assign(1, 1.0); Q = 1; }
+    std::vector<double> nodes;
+    std::vector<double> lv_a, w_a;
+    for (int q = 0; q < Q; ++q)
+        if (w[q] != 0.0) { lv_a.push_back(logv[q]); w_a.push_back(w[q]); }
+    const int Qa = (int)lv_a.size();
+    if (!ensure_nodes(p, std::max(Qa, 1))) return -1;
+    nodes.resize(2 * (size_t)std::max(Qa, 1));
+    std::copy(lv_a.begin(), lv_a.end(), nodes.begin());
+    std::copy(w_a.begin(), w_a.end(), nodes.begin() + std::max(Qa, 1));
+    hipStream_t s = p->stream;
+    if (!HIP_OK(hipMemcpyAsync(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice, s)))
+        return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    const double *d_lv = p->d_nodes;
+    const double *d_w = p->d_nodes + std::max(Qa, 1);
+
+    if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
+    if (!HIP_OK(hipMemsetAsync(p->d_acc, 0, sizeof(double) * g.n, s))) return -1;
+    const int B = g.B;
+    const int L = g.L;
+    int n_lds = 0, n_big = 0;
+    for (int l = 1; l < L; ++l) (g.F[l] > icm::kMaxLdsF ? n_big : n_lds)++;
+
+    for (int q0 = 0; q0 < Qa; q0 += p->qb) {
+        const int nq = std::min(p->qb, Qa - q0);
+        icm::LeafJob lj{};
+        lj.S = p->d_S;
+        lj.logv = d_lv + q0;
+        lj.wq = d_w + q0;
+        lj.a_in = p->d_a_in ? p->d_a_in + (long long)q0 * g.n : nullptr;
+        lj.blocks = p->d_blocks;
+        lj.n = g.n; lj.nb = g.nb; lj.B = B; lj.k = g.k; lj.nq = nq;
+        lj.G0 = g.G[0];
+        if (!hip_ok(icm::launch_leaf_build(lj, s), "leaf_build")) return -1;
+
+        // views of level l's coefficients
+        auto coef_view = [&](int l) {
+            icm::CoefView v{};
+            if (l == 0) {
+                v.base = p->d_blocks; v.plane = (long long)g.nb * (B + 1); v.stride = B + 1; v.count = g.c[0];
+            } else {
+                v.base = p->d_coef[l]; v.plane = (long long)g.n_real[l] * g.c[l]; v.stride = g.c[l];
+                v.count = g.c[l];
+            }
+            return v;
+        };
+        for (int l = 1; l <= L - 2; ++l) {
+            icm::LevelJob J{};
+            J.child = coef_view(l - 1);
+            J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            hipError_t e = (J.F <= icm::kMaxLdsF)
+                               ? icm::launch_build_lds(J, p->d_tw, p->twShift, s)
+                               : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s);
+            if (!hip_ok(e, "build level")) return -1;
+        }
+        // propagate: root g = payout (same for every q-plane)
+        int cur = 0;
+        for (int l = L - 1; l >= 1; --l) {
+            icm::LevelJob J{};
+            J.child = coef_view(l - 1);
+            if (l == L - 1) {
+                J.gpar = icm::CoefView{d_payout, 0, g.k, g.k};
+            } else {
+                J.gpar = icm::CoefView{p->d_g[cur], (long long)g.n_real[l] * g.G[l], g.G[l], g.G[l]};
+                cur ^= 1;
+            }
+            J.gchild = icm::CoefOut{p->d_g[cur], (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            hipError_t e = (J.F <= icm::kMaxLdsF)
+                               ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s)
+                               : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s);
+            if (!hip_ok(e, "propagate level")) return -1;
+        }
+        if (L == 1) {
+            lj.g = d_payout; lj.g_plane = 0; lj.g_stride = g.k;
+        } else {
+            lj.g = p->d_g[cur]; lj.g_plane = (long long)g.n_real[0] * g.G[0]; lj.g_stride = g.G[0];
+        }
+        const int nchunk = std::min(p->nchunk, nq);
+        lj.q_per_chunk = (nq + nchunk - 1) / nchunk;
+        const int nch = (nq + lj.q_per_chunk - 1) / lj.q_per_chunk;
+        lj.partial = p->d_partial;
+        lj.inner_out = p->d_inner ? p->d_inner + (long long)q0 * g.n : nullptr;
+        lj.active = p->d_active;
+        if (!hip_ok(icm::launch_leaf_extract(lj, nch, s), "leaf_extract")) return -1;
+        if (!hip_ok(icm::launch_finalize(p->d_partial, nch, g.n, p->d_perm, p->d_active, p->d_acc, 1, s),
+                    "finalize"))
+            return -1;
+    }
+    if (!hip_ok(icm::launch_scatter(p->d_acc, g.n, p->d_perm, p->d_equity, s), "scatter")) return -1;
+    if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+        memset(stats, 0, sizeof(*stats));
+        stats->total_ns = (double)ms * 1e6;
+        stats->peak_vram_bytes = p->bytes;
+        stats->engine = 1;
+        stats->B = B;
+        stats->uncached_fused_levels = n_lds;
+        stats->uncached_cufft_levels = n_big;
+    }
+    return 0;
+}
+
+int equity_host(IcmHipPlan *p, int Q, const double *payout, double *equity, IcmHipRunStats *stats) {
+    if (!p || !payout || !equity) { set_err("null argument"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!HIP_OK(hipMemcpyAsync(p->d_payout, payout, sizeof(double) * p->geo.k, hipMemcpyHostToDevice,
+                               p->stream)))
+        return -1;
+    if (run_plan(p, Q, p->d_payout, stats) != 0) return -1;
+    if (!HIP_OK(hipMemcpy(equity, p->d_equity, sizeof(double) * p->geo.n, hipMemcpyDeviceToHost))) return -1;
+    return 0;
+}
+
+int set_targets(IcmHipPlan *p, const int *targets, int n_targets) {
+    const int n = p->geo.n;
+    std::vector<int> inv(n);
+    for (int i = 0; i < n; ++i) inv[p->perm[i]] = i;
+    std::vector<unsigned char> act(n, 0);
+    for (int t = 0; t < n_targets; ++t)
+        if (targets[t] >= 0 && targets[t] < n) act[inv[targets[t]]] = 1;  // out-of-range ignored (gpu_api.cu:636-641)
+    if (!p->d_active && !dalloc(p, &p->d_active, n)) return -1;
+    return HIP_OK(hipMemcpy(p->d_active, act.data(), n, hipMemcpyHostToDevice)) ? 0 : -1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int icm_hip_init(int device_id) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int cnt = 0;
+    if (!HIP_OK(hipGetDeviceCount(&cnt))) return 0;
+    if (device_id < 0 || device_id >= cnt) { set_err("device id out of range"); return 0; }
+    if (!HIP_OK(hipSetDevice(device_id))) return 0;
+    g_device = device_id;
+    return 1;
+}
+
+void icm_hip_shutdown(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_device = -1;
+}
+
+const char *icm_hip_last_error(void) { return g_err.c_str(); }
+
+int icm_hip_release_pooled_memory(void) { return 0; }
+
+IcmHipPlan *icm_hip_plan_create(int n, const double *S, int k, const IcmHipOptions *opts) {
+    return plan_create(n, S, k, opts, true);
+}
+
+void icm_hip_plan_destroy(IcmHipPlan *plan) { plan_free(plan); }
+
+int icm_hip_plan_summary(const IcmHipPlan *p, IcmHipPlanSummary *s) {
+    if (!p || !s) { set_err("null argument"); return -1; }
+    memset(s, 0, sizeof(*s));
+    s->n = p->geo.n; s->k = p->geo.k; s->B = p->geo.B; s->engine = 1; s->n_levels = p->geo.L;
+    for (int l = 1; l < p->geo.L; ++l) {
+        const int F = p->geo.F[l];
+        if (F <= 64) s->n_tier1++;
+        else if (F <= icm::kMaxLdsF) s->n_tier2++;
+        else s->n_tier3++;
+    }
+    s->q_batch = p->qb;
+    s->planned_peak_vram_bytes = p->bytes;
+    return 0;
+}
+
+int icm_hip_equity_with_plan(IcmHipPlan *plan, int Q, const double *payout, double *equity,
+                             IcmHipRunStats *stats) {
+    return equity_host(plan, Q, payout, equity, stats);
+}
+
+int icm_hip_equity_with_plan_device(IcmHipPlan *plan, int Q, const double *d_payout, double *d_equity,
+                                    IcmHipRunStats *stats) {
+    if (!plan || !d_payout || !d_equity) { set_err("null argument"); return -1; }
+    if (run_plan(plan, Q, d_payout, stats) != 0) return -1;
+    return HIP_OK(hipMemcpy(d_equity, plan->d_equity, sizeof(double) * plan->geo.n, hipMemcpyDeviceToDevice)) ? 0
+                                                                                                          : -1;
+}
+
+int icm_hip_equity(int n, const double *S, int Q, const double *payout, int k, double *equity) {
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    IcmHipPlan *p = plan_create(n, S, k, nullptr, true, Q);
+    if (!p) return -1;
+    int rc = equity_host(p, Q, payout, equity, nullptr);
+    plan_free(p);
+    return rc;
+}
+
+int icm_hip_equity_subset(int n, const double *S, int Q, const double *payout, int k, double *equity,
+                          const int *targets, int n_targets) {
+    if (Q < 1 || !targets || n_targets < 1) { set_err("invalid arguments"); return -1; }
+    IcmHipPlan *p = plan_create(n, S, k, nullptr, true, Q);
+    if (!p) return -1;
+    int rc = set_targets(p, targets, n_targets);
+    if (rc == 0) rc = equity_host(p, Q, payout, equity, nullptr);
+    plan_free(p);
+    return rc;
+}
+
+int icm_hip_node_inner(int n, const double *a, const double *payout, int k, double *inner) {
+    if (!a || !payout || !inner || n < 1 || k < 1 || k > n) { set_err("invalid arguments"); return -1; }
+    std::vector<double> ones(n, 1.0);
+    IcmHipPlan *p = plan_create(n, ones.data(), k, nullptr, false, 1);
+    if (!p) return -1;
+    double *d_a = nullptr, *d_in = nullptr;
+    int rc = -1;
+    if (dalloc(p, &d_a, n) && dalloc(p, &d_in, n) &&
+        HIP_OK(hipMemcpy(d_a, a, sizeof(double) * n, hipMemcpyHostToDevice)) &&
+        HIP_OK(hipMemcpy(p->d_payout, payout, sizeof(double) * k, hipMemcpyHostToDevice))) {
+        p->d_a_in = d_a;
+        p->d_inner = d_in;
+        p->raw_node = true;
+        p->qb = 1;
+        if (run_plan(p, 1, p->d_payout, nullptr) == 0 &&
+            HIP_OK(hipMemcpy(inner, d_in, sizeof(double) * n, hipMemcpyDeviceToHost)))
+            rc = 0;
+    }
+    if (d_a) (void)hipFree(d_a);
+    p->d_inner = nullptr;
+    if (d_in) (void)hipFree(d_in);
+    plan_free(p);
+    return rc;
+}
+
+// reference GPU names
+int icm_gpu_init(int device_id) { return icm_hip_init(device_id); }
+void icm_gpu_shutdown(void) { icm_hip_shutdown(); }
+const char *icm_gpu_last_error(void) { return icm_hip_last_error(); }
+int icm_gpu_release_pooled_memory(void) { return icm_hip_release_pooled_memory(); }
+IcmHipPlan *icm_gpu_plan_create(int n, const double *S, int k, const IcmHipOptions *opts) {
+    return icm_hip_plan_create(n, S, k, opts);
+}
+void icm_gpu_plan_destroy(IcmHipPlan *plan) { icm_hip_plan_destroy(plan); }
+int icm_gpu_plan_summary(const IcmHipPlan *plan, IcmHipPlanSummary *s) { return icm_hip_plan_summary(plan, s); }
+int icm_gpu_equity_with_plan(IcmHipPlan *plan, int Q, const double *payout, double *equity,
+                             IcmHipRunStats *stats) {
+    return icm_hip_equity_with_plan(plan, Q, payout, equity, stats);
+}
+int icm_gpu_equity(int n, const double *S, int Q, const double *payout, int k, double *equity,
+                   const IcmHipOptions *opts, IcmHipRunStats *stats) {
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    IcmHipPlan *p = plan_create(n, S, k, opts, true, Q);
+    if (!p) return -1;
+    int rc = equity_host(p, Q, payout, equity, stats);
+    plan_free(p);
+    return rc;
+}
+int icm_gpu_equity_subset(int n, const double *S, int Q, const double *payout, int k, double *equity,
+                          const int *targets, int n_targets, const IcmHipOptions *opts, IcmHipRunStats *stats) {
+    if (Q < 1 || !targets || n_targets < 1) { set_err("invalid arguments"); return -1; }
+    IcmHipPlan *p = plan_create(n, S, k, opts, true, Q);
+    if (!p) return -1;
+    int rc = set_targets(p, targets, n_targets);
+    if (rc == 0) rc = equity_host(p, Q, payout, equity, stats);
+    plan_free(p);
+    return rc;
+}
+
+// CPU-named aliases (python/icm/__init__.py:92-119 binds these three)
+void icm_init(const char *wisdom_path) {
+    (void)wisdom_path;
+    (void)icm_hip_init(0);
+}
+void icm_equity(int n, const double *S, int Q, const double *payout, int k, double *equity) {
+    if (icm_hip_equity(n, S, Q, payout, k, equity) != 0 && equity && n > 0)
+        memset(equity, 0, sizeof(double) * n);
+}
+void icm_equity_subset(int n, const double *S, int Q, const double *payout, int k, double *equity,
+                       const int *targets, int n_targets) {
+    if (icm_hip_equity_subset(n, S, Q, payout, k, equity, targets, n_targets) != 0 && equity && n > 0)
+        memset(equity, 0, sizeof(double) * n);
+}
+
+int icm_hip_plan_geometry(int n, int k, int B, int max_levels, int *n_real, int *c, int *G, int *F) {
+    Geometry g;
+    if (!make_geometry(n, k, B, g)) return -1;
+    if (g.L > max_levels) return -1;
+    for (int l = 0; l < g.L; ++l) {
+        if (n_real) n_real[l] = g.n_real[l];
+        if (c) c[l] = g.c[l];
+        if (G) G[l] = g.G[l];
+        if (F) F[l] = g.F[l];
+    }
+    return g.L;
+}
+
+void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w) { make_nodes(Q, Smax, logv, w); }
+
+}  // extern "C"

@@ -1,0 +1,723 @@
+// icm_kernels.hip -- CDNA4 (gfx950) kernels of the ICM equity engine, FP64.
+//
+// Per quadrature node q the engine evaluates
+//   Psi_i = sum_{r<k} payout[r] [x^r] prod_{j!=i} (a_j + (1 - a_j) x),
+//   a_j = exp(S_j logv_q),
+// through a product tree over blocks of B players, its transpose (the
+// "propagate" pass, a middle product per level), and a per-block
+// leave-one-out division.  Reference semantics: src/cpu/icm.c:2186-2303
+// (block build, bidirectional divide), src/cpu/icm.c:1605-1819 (tree build
+// and propagate), src/gpu/gpu_kernels.cu:1472-1545 (leaf extract and the
+// quadrature-weighted accumulation).  All q-planes of a launch run together:
+// grid.y (or grid.z) is the q-plane index.
+#include "fft_lds.hpp"
+#include "icm_kernels.hpp"
+
+namespace icm {
+
+// ---------------------------------------------------------------------------
+// Leaf build: one lane owns one block and multiplies its B linear factors in
+// registers (triangular recurrence, src/cpu/icm.c:2196-2213).  Players past n
+// are the identity factor (a = 1).  A 64 x (B+1) LDS transpose makes the
+// stores of block_prods[q][b][0..B] contiguous.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double a_of(double S, double lv) {
+    const double arg = S * lv;
+    return (arg < -700.0) ? 0.0 : exp(arg);
+}
+
+template <int B>
+__global__ __launch_bounds__(64) void k_leaf_build(LeafJob J) {
+    __shared__ double sh[64 * (B + 1)];
+    const int lane = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int b0 = blockIdx.x * 64;
+    const int b = b0 + lane;
+    double P[B + 1];
+    P[0] = 1.0;
+#pragma unroll
+    for (int m = 1; m <= B; ++m) P[m] = 0.0;
+    if (b < J.nb) {
+        const double lv = J.logv[qi];
+        const long long arow = (long long)qi * J.n;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int j = b * B + i;
+            double a = 1.0;
+            if (j < J.n) a = J.a_in ? J.a_in[arow + j] : a_of(J.S[j], lv);
+            const double bb = 1.0 - a;
+#pragma unroll
+            for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
+            P[0] *= a;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
+    __syncthreads();
+    const int nblk = min(64, J.nb - b0);
+    double *out = J.blocks + (long long)qi * J.nb * (B + 1) + (long long)b0 * (B + 1);
+    for (int e = lane; e < nblk * (B + 1); e += 64) out[e] = sh[e];
+}
+
+// ---------------------------------------------------------------------------
+// Leaf extract: Psi_j = sum_{m<pk} g_b[m] (P_b / (a_j + (1-a_j)x))[m] with the
+// bidirectional division of src/cpu/icm.c:2244-2299 (forward if a > 0.5,
+// backward if 1e-15 < a <= 0.5, shift otherwise), then the quadrature weight
+// w_q S_j a_j exp(-logv_q) (non-finite -> 0, src/cpu/icm.c:2584-2589).
+// One lane per player; the lane loops over its chunk of q-planes and keeps the
+// sum in a register, so the q-sum order is fixed (no atomics).
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
+    constexpr int BPW = 256 / B;
+    __shared__ double shP[BPW * (B + 1)];
+    __shared__ double shG[BPW * B];
+    const int t = threadIdx.x;
+    const int lb = t / B, li = t % B;
+    const int bfirst = blockIdx.x * BPW;
+    const int b = bfirst + lb;
+    const int j = b * B + li;
+    const int chunk = blockIdx.y;
+    const int q0 = chunk * J.q_per_chunk;
+    const int q1 = min(q0 + J.q_per_chunk, J.nq);
+    const bool valid = (b < J.nb) && (j < J.n);
+    const int bsize = (b < J.nb) ? min(B, J.n - b * B) : 0;
+    const int pk = min(J.G0, bsize);
+    const double Sj = valid ? J.S[j] : 0.0;
+    const bool act = valid && (!J.active || J.active[j]);
+    const long long bplane = (long long)J.nb * (B + 1);
+    double acc = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        for (int e = t; e < BPW * (B + 1); e += 256) {
+            const int bb = bfirst + e / (B + 1);
+            shP[e] = (bb < J.nb) ? J.blocks[q * bplane + (long long)bfirst * (B + 1) + e] : 0.0;
+        }
+        for (int e = t; e < BPW * B; e += 256) {
+            const int lbk = e / B, m = e % B;
+            const int bb = bfirst + lbk;
+            shG[e] = (bb < J.nb && m < J.G0)
+                         ? J.g[q * J.g_plane + (long long)bb * J.g_stride + m] : 0.0;
+        }
+        __syncthreads();
+        if (act) {
+            const double lv = J.logv[q];
+            const double a = J.a_in ? J.a_in[(long long)q * J.n + j] : a_of(Sj, lv);
+            const double bm = 1.0 - a;
+            const double *P = shP + lb * (B + 1);
+            const double *g = shG + lb * B;
+            double eq = 0.0;
+            if (a > 0.5) {
+                const double ia = 1.0 / a, c = -bm * ia;
+                double qv = P[0] * ia;
+                eq = g[0] * qv;
+                for (int m = 1; m < pk; ++m) {
+                    qv = c * qv + P[m] * ia;
+                    eq += g[m] * qv;
+                }
+            } else if (a > 1e-15) {
+                const double ib = 1.0 / bm, c = -a * ib;
+                double qv = P[bsize] * ib;
+                if (bsize - 1 < pk) eq += g[bsize - 1] * qv;
+                for (int m = bsize - 2; m >= 0; --m) {
+                    qv = c * qv + P[m + 1] * ib;
+                    if (m < pk) eq += g[m] * qv;
+                }
+            } else {
+                for (int m = 0; m < pk; ++m) eq += g[m] * P[m + 1];
+            }
+            if (J.inner_out) J.inner_out[(long long)q * J.n + j] = eq;
+            double pw = J.wq[q] * Sj * a * exp(-lv);
+            if (!isfinite(pw)) pw = 0.0;
+            acc += pw * eq;
+        }
+        __syncthreads();
+    }
+    if (valid && J.partial) J.partial[(long long)chunk * J.n + j] = act ? acc : 0.0;
+}
+
+__global__ void k_finalize(const double *__restrict__ partial, int nchunk, int n,
+                           const unsigned char *__restrict__ active, double *__restrict__ acc,
+                           int accumulate) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double s = accumulate ? acc[j] : 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[(long long)c * n + j];
+    if (active && !active[j]) s = 0.0;
+    acc[j] = s;
+}
+
+__global__ void k_scatter(const double *__restrict__ acc, int n, const int *__restrict__ perm,
+                          double *__restrict__ equity) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    equity[perm[j]] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum helpers.  A complex transform of z = x + i y (x, y real) gives
+// X[K] = (Z[K] + conj(Z[-K]))/2 and Y[K] = (Z[K] - conj(Z[-K]))/(2i).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 split_x(double2 zk, double2 zm) {
+    return make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+}
+__device__ __forceinline__ double2 split_y(double2 zk, double2 zm) {
+    return make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+}
+// A + i B for complex A, B.
+__device__ __forceinline__ double2 pack_ai(double2 A, double2 Bv) {
+    return make_double2(A.x - Bv.y, A.y + Bv.x);
+}
+
+// Child pair of parent p packed as L + i R; a missing right child is the
+// identity polynomial.
+__device__ __forceinline__ double2 load_child_pair(const LevelJob &J, const double *cb, int p, int m) {
+    double2 z = make_double2(0.0, 0.0);
+    if (p < J.nr_parent) {
+        const int cs = J.child.stride, cc = J.child.count;
+        const long long off = (long long)(2 * p) * cs;
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        if (m < cc) {
+            z.x = cb[off + m];
+            if (has_r) z.y = cb[off + cs + m];
+        }
+        if (!has_r && m == 0) z.y = 1.0;
+    }
+    return z;
+}
+
+// Exact constant term L0 R0 and top term L[c-1] R[c-1] of parent p.
+__device__ __forceinline__ double child_prod_at(const LevelJob &J, const double *cb, int p, int m) {
+    const int cs = J.child.stride;
+    const long long off = (long long)(2 * p) * cs;
+    const double l = cb[off + m];
+    const double r = ((2 * p + 1) < J.nr_child) ? cb[off + cs + m] : (m == 0 ? 1.0 : 0.0);
+    return l * r;
+}
+
+// ---------------------------------------------------------------------------
+// Build, LDS-resident (F <= 4096).  Item = a pair of parents (p, p+1); the
+// tile holds 4096/F items.  Z_p = L + iR -> FFT -> P_p = X Y (kept in
+// registers at this thread's bin pairs), same for p+1, then U = P_p + i P_p1
+// -> inverse FFT -> Re = parent p, Im = parent p+1.  The transform size F may
+// be one short of the linear product length (F = 2c-2): only coefficient 0
+// aliases, and it is replaced by the exact L0 R0; coefficient F (if stored) is
+// the exact product of the leading terms.
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(256) void k_build_lds(LevelJob J, const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 lds[kTile];
+    constexpr int ITEMS = kTile / F;
+    constexpr int HALF = F / 2;
+    constexpr int LOGF = ilog2c(F);
+    constexpr int LOGH = ilog2c(HALF);
+    const int t = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int pair0 = blockIdx.x * ITEMS;
+    const double *cb = J.child.base + (long long)qi * J.child.plane;
+
+    double2 P1[8];
+#pragma unroll
+    for (int which = 0; which < 2; ++which) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kThreads;
+            const int item = e >> LOGF, m = e & (F - 1);
+            lds[e] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
+        }
+        fft_tile<F, false>(lds, tw, twShift);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int ps = t + u * kThreads;
+            const int item = ps >> LOGH, jb = ps & (HALF - 1);
+            double2 *Zb = lds + item * F;
+            double2 Pv;
+            if (jb == 0) {
+                const double2 z0 = Zb[0], zh = Zb[HALF];
+                Pv = make_double2(z0.x * z0.y, zh.x * zh.y);
+            } else {
+                const double2 zj = Zb[jb], zm = Zb[F - jb];
+                Pv = c_mul(split_x(zj, zm), split_y(zj, zm));
+            }
+            if (which == 0) {
+                P1[u] = Pv;
+            } else if (jb == 0) {
+                Zb[0] = make_double2(P1[u].x, Pv.x);
+                Zb[HALF] = make_double2(P1[u].y, Pv.y);
+            } else {
+                Zb[jb] = pack_ai(P1[u], Pv);
+                Zb[F - jb] = pack_ai(c_conj(P1[u]), c_conj(Pv));
+            }
+        }
+        if (which == 0) __syncthreads();
+    }
+    fft_tile<F, true>(lds, tw, twShift);
+    const double invF = 1.0 / (double)F;
+    double *pb = J.parent.base + (long long)qi * J.parent.plane;
+    const int ps_ = J.parent.stride, pc = J.parent.count;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = 2 * (pair0 + item);
+        if (m < pc) {
+            const double2 v = lds[e];
+            if (p < J.nr_parent)
+                pb[(long long)p * ps_ + m] = (m == 0) ? child_prod_at(J, cb, p, 0) : v.x * invF;
+            if (p + 1 < J.nr_parent)
+                pb[(long long)(p + 1) * ps_ + m] = (m == 0) ? child_prod_at(J, cb, p + 1, 0) : v.y * invF;
+        }
+        if (m == 0 && pc > F) {
+            const int top = J.child.count - 1;
+            if (p < J.nr_parent) pb[(long long)p * ps_ + F] = child_prod_at(J, cb, p, top);
+            if (p + 1 < J.nr_parent) pb[(long long)(p + 1) * ps_ + F] = child_prod_at(J, cb, p + 1, top);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Propagate, LDS-resident (F <= 4096): for parent p with children L, R,
+//   gL[m] = sum_t R[t] g[m+t],  gR[m] = sum_t L[t] g[m+t],  m < G_child,
+// as cyclic correlations of size F (F >= G_child + c_child - 1, so no index
+// wraps).  Item = a pair of parents: their g vectors share one transform;
+// each parent's children share another; the two outputs of a parent share
+// the inverse (Re = gL, Im = gR).
+// ---------------------------------------------------------------------------
+template <int F>
+__global__ __launch_bounds__(256) void k_prop_lds(LevelJob J, const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 lds[kTile];
+    constexpr int ITEMS = kTile / F;
+    constexpr int HALF = F / 2;
+    constexpr int LOGF = ilog2c(F);
+    constexpr int LOGH = ilog2c(HALF);
+    const int t = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int pair0 = blockIdx.x * ITEMS;
+    const double *cb = J.child.base + (long long)qi * J.child.plane;
+    const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+    const int gs = J.gpar.stride, gcnt = J.gpar.count;
+
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = 2 * (pair0 + item);
+        double2 z = make_double2(0.0, 0.0);
+        if (m < gcnt) {
+            if (p < J.nr_parent) z.x = gb[(long long)p * gs + m];
+            if (p + 1 < J.nr_parent) z.y = gb[(long long)(p + 1) * gs + m];
+        }
+        lds[e] = z;
+    }
+    fft_tile<F, false>(lds, tw, twShift);
+    double2 G1[8], G2[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int ps = t + u * kThreads;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const double2 *Zb = lds + item * F;
+        if (jb == 0) {
+            const double2 z0 = Zb[0], zh = Zb[HALF];
+            G1[u] = make_double2(z0.x, zh.x);
+            G2[u] = make_double2(z0.y, zh.y);
+        } else {
+            const double2 zj = Zb[jb], zm = Zb[F - jb];
+            G1[u] = split_x(zj, zm);
+            G2[u] = split_y(zj, zm);
+        }
+    }
+    __syncthreads();
+    const double invF = 1.0 / (double)F;
+    double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
+    const int gcs = J.gchild.stride, gcc = J.gchild.count;
+#pragma unroll
+    for (int which = 0; which < 2; ++which) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kThreads;
+            const int item = e >> LOGF, m = e & (F - 1);
+            lds[e] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
+        }
+        fft_tile<F, false>(lds, tw, twShift);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int ps = t + u * kThreads;
+            const int item = ps >> LOGH, jb = ps & (HALF - 1);
+            double2 *Zb = lds + item * F;
+            const double2 g = which == 0 ? G1[u] : G2[u];
+            if (jb == 0) {
+                const double2 z0 = Zb[0], zh = Zb[HALF];
+                Zb[0] = make_double2(g.x * z0.y, g.x * z0.x);
+                Zb[HALF] = make_double2(g.y * zh.y, g.y * zh.x);
+            } else {
+                const double2 zj = Zb[jb], zm = Zb[F - jb];
+                const double2 Lh = split_x(zj, zm), Rh = split_y(zj, zm);
+                const double2 A = c_mulc(g, Rh), Bv = c_mulc(g, Lh);
+                Zb[jb] = pack_ai(A, Bv);
+                Zb[F - jb] = pack_ai(c_conj(A), c_conj(Bv));
+            }
+        }
+        fft_tile<F, true>(lds, tw, twShift);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kThreads;
+            const int item = e >> LOGF, m = e & (F - 1);
+            const int p = 2 * (pair0 + item) + which;
+            if (p < J.nr_parent && m < gcc) {
+                const double2 v = lds[e];
+                gcb[(long long)(2 * p) * gcs + m] = v.x * invF;
+                if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * gcs + m] = v.y * invF;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Large levels (F >= 8192): four-step transform with F = F1 x F2, F2 = 4096.
+//   forward: column FFTs (size F1, stride F2) + twiddle w_F^(n2 k1) -> [k1][n2]
+//            row FFTs (size F2) -> spectrum in [k1][k2] order (K = k1 + F1 k2)
+//   inverse: row IFFTs of [k1][k2] + twiddle w_F^(-n2 k1) -> [k1][n2]
+//            column IFFTs -> natural order.
+// Spectra stay in [k1][k2] order; the pointwise kernels find -K there.
+// ---------------------------------------------------------------------------
+constexpr int kF2 = kTile;
+
+struct LdChildPair {      // array a = parent p
+    LevelJob J;
+    __device__ double2 operator()(int a, int qi, int n) const {
+        const double *cb = J.child.base + (long long)qi * J.child.plane;
+        return load_child_pair(J, cb, a, n);
+    }
+};
+struct LdGPair {          // array a = pair of parents
+    LevelJob J;
+    __device__ double2 operator()(int a, int qi, int n) const {
+        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+        const int p = 2 * a;
+        double2 z = make_double2(0.0, 0.0);
+        if (n < J.gpar.count) {
+            z.x = gb[(long long)p * J.gpar.stride + n];
+            if (p + 1 < J.nr_parent) z.y = gb[(long long)(p + 1) * J.gpar.stride + n];
+        }
+        return z;
+    }
+};
+struct StParentPair {     // array a = pair of parents (build)
+    LevelJob J;
+    __device__ void operator()(int a, int qi, int n, double2 v) const {
+        const double invF = 1.0 / (double)J.F;
+        const double *cb = J.child.base + (long long)qi * J.child.plane;
+        double *pb = J.parent.base + (long long)qi * J.parent.plane;
+        const int p = 2 * a;
+        const int ps = J.parent.stride, pc = J.parent.count;
+        if (n < pc) {
+            if (p < J.nr_parent)
+                pb[(long long)p * ps + n] = (n == 0) ? child_prod_at(J, cb, p, 0) : v.x * invF;
+            if (p + 1 < J.nr_parent)
+                pb[(long long)(p + 1) * ps + n] = (n == 0) ? child_prod_at(J, cb, p + 1, 0) : v.y * invF;
+        }
+        if (n == 0 && pc > J.F) {
+            const int top = J.child.count - 1;
+            if (p < J.nr_parent) pb[(long long)p * ps + J.F] = child_prod_at(J, cb, p, top);
+            if (p + 1 < J.nr_parent) pb[(long long)(p + 1) * ps + J.F] = child_prod_at(J, cb, p + 1, top);
+        }
+    }
+};
+struct StGChildren {      // array a = parent p (propagate)
+    LevelJob J;
+    __device__ void operator()(int a, int qi, int n, double2 v) const {
+        if (n >= J.gchild.count) return;
+        const double invF = 1.0 / (double)J.F;
+        double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
+        const int p = a;
+        gcb[(long long)(2 * p) * J.gchild.stride + n] = v.x * invF;
+        if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * J.gchild.stride + n] = v.y * invF;
+    }
+};
+
+template <int F1, class Loader>
+__global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
+                                                 const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 lds[kTile];
+    constexpr int TC = kTile / F1;
+    constexpr int LOGTC = ilog2c(TC);
+    const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    const int F = 1 << logF;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int r = e >> LOGTC, c = e & (TC - 1);
+        const int n = r * kF2 + tile * TC + c;
+        lds[c * F1 + r] = ld(a, qi, n);
+    }
+    fft_tile<F1, false>(lds, tw, twShift);
+    double2 *o = out + ((long long)qi * nArr + a) * F;
+    const int sh = twShift - logF;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int r = e >> LOGTC, c = e & (TC - 1);
+        const int n2 = tile * TC + c;
+        const double2 w = tw[((n2 * r) & (F - 1)) << sh];
+        o[r * kF2 + n2] = c_mul(lds[c * F1 + r], w);
+    }
+}
+
+template <int F1, class Storer>
+__global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in, Storer st, int nArr, int logF,
+                                                 const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 lds[kTile];
+    constexpr int TC = kTile / F1;
+    constexpr int LOGTC = ilog2c(TC);
+    const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    const int F = 1 << logF;
+    const double2 *src = in + ((long long)qi * nArr + a) * F;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int r = e >> LOGTC, c = e & (TC - 1);
+        lds[c * F1 + r] = src[r * kF2 + tile * TC + c];
+    }
+    fft_tile<F1, true>(lds, tw, twShift);
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        const int r = e >> LOGTC, c = e & (TC - 1);
+        st(a, qi, r * kF2 + tile * TC + c, lds[c * F1 + r]);
+    }
+}
+
+// Row pass over arrays of F = F1 * 4096 complex; grid (F1 rows, arrays).
+template <bool INV>
+__global__ __launch_bounds__(256) void k_row(const double2 *__restrict__ in, double2 *__restrict__ out, int logF,
+                                             const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 lds[kTile];
+    const int k1 = blockIdx.x;
+    const long long base = (long long)blockIdx.y * (1LL << logF) + (long long)k1 * kF2;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        lds[e] = in[base + e];
+    }
+    fft_tile<kF2, INV>(lds, tw, twShift);
+    const int F = 1 << logF;
+    const int sh = twShift - logF;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int e = t + u * kThreads;
+        double2 v = lds[e];
+        if (INV) v = c_mulc(v, tw[((e * k1) & (F - 1)) << sh]);
+        out[base + e] = v;
+    }
+}
+
+__device__ __forceinline__ int mirror_pos(int pos, int F1) {
+    const int k1 = pos >> 12, k2 = pos & (kF2 - 1);
+    return (k1 == 0) ? ((kF2 - k2) & (kF2 - 1)) : ((F1 - k1) * kF2 + (kF2 - 1 - k2));
+}
+
+// U[pair] = P_p + i P_{p+1}, P = X Y from the packed child spectra.
+__global__ __launch_bounds__(256) void k_pw_build(const double2 *__restrict__ X, double2 *__restrict__ U, int nrp,
+                                                  int npairs, int logF, int nq) {
+    const int F = 1 << logF;
+    const int F1 = F >> 12;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)nq * npairs * F;
+    if (idx >= total) return;
+    const int pos = (int)(idx & (F - 1));
+    const long long pq = idx >> logF;
+    const int pair = (int)(pq % npairs), qi = (int)(pq / npairs);
+    const int mp = mirror_pos(pos, F1);
+    const int p = 2 * pair;
+    const double2 *Zp = X + ((long long)qi * nrp + p) * F;
+    double2 zk = Zp[pos], zm = Zp[mp];
+    const double2 P1 = c_mul(split_x(zk, zm), split_y(zk, zm));
+    double2 P2 = make_double2(0.0, 0.0);
+    if (p + 1 < nrp) {
+        zk = Zp[F + pos];
+        zm = Zp[F + mp];
+        P2 = c_mul(split_x(zk, zm), split_y(zk, zm));
+    }
+    U[((long long)qi * npairs + pair) * F + pos] = pack_ai(P1, P2);
+}
+
+// U1[p] = g_p conj(R_p) + i g_p conj(L_p).
+__global__ __launch_bounds__(256) void k_pw_prop(const double2 *__restrict__ X1, const double2 *__restrict__ X3,
+                                                 double2 *__restrict__ U1, int nrp, int npairs, int logF, int nq) {
+    const int F = 1 << logF;
+    const int F1 = F >> 12;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long total = (long long)nq * nrp * F;
+    if (idx >= total) return;
+    const int pos = (int)(idx & (F - 1));
+    const long long pq = idx >> logF;
+    const int p = (int)(pq % nrp), qi = (int)(pq / nrp);
+    const int mp = mirror_pos(pos, F1);
+    const double2 *Z3 = X3 + ((long long)qi * npairs + (p >> 1)) * F;
+    const double2 z3k = Z3[pos], z3m = Z3[mp];
+    const double2 g = (p & 1) ? split_y(z3k, z3m) : split_x(z3k, z3m);
+    const double2 *Z1 = X1 + ((long long)qi * nrp + p) * F;
+    const double2 zk = Z1[pos], zm = Z1[mp];
+    const double2 Lh = split_x(zk, zm), Rh = split_y(zk, zm);
+    U1[((long long)qi * nrp + p) * F + pos] = pack_ai(c_mulc(g, Rh), c_mulc(g, Lh));
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+#define ICM_TRY(x)                              \
+    do {                                        \
+        hipError_t e_ = (x);                    \
+        if (e_ != hipSuccess) return e_;        \
+    } while (0)
+
+hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s) {
+    dim3 grid((J.nb + 63) / 64, J.nq);
+    switch (J.B) {
+    case 16: k_leaf_build<16><<<grid, 64, 0, s>>>(J); break;
+    case 32: k_leaf_build<32><<<grid, 64, 0, s>>>(J); break;
+    case 64: k_leaf_build<64><<<grid, 64, 0, s>>>(J); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s) {
+    const int bpw = 256 / J.B;
+    dim3 grid((J.nb + bpw - 1) / bpw, nchunk);
+    switch (J.B) {
+    case 16: k_leaf_extract<16><<<grid, 256, 0, s>>>(J); break;
+    case 32: k_leaf_extract<32><<<grid, 256, 0, s>>>(J); break;
+    case 64: k_leaf_extract<64><<<grid, 256, 0, s>>>(J); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double *partial, int nchunk, int n, const int *perm,
+                           const unsigned char *active, double *acc, int accumulate, hipStream_t s) {
+    (void)perm;
+    k_finalize<<<(n + 255) / 256, 256, 0, s>>>(partial, nchunk, n, active, acc, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s) {
+    k_scatter<<<(n + 255) / 256, 256, 0, s>>>(acc, n, perm, equity);
+    return hipGetLastError();
+}
+
+template <int F>
+static hipError_t build_lds_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    constexpr int ITEMS = kTile / F;
+    const int npairs = (J.nr_parent + 1) / 2;
+    dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
+    k_build_lds<F><<<grid, kThreads, 0, s>>>(J, tw, twShift);
+    return hipGetLastError();
+}
+template <int F>
+static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    constexpr int ITEMS = kTile / F;
+    const int npairs = (J.nr_parent + 1) / 2;
+    dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
+    k_prop_lds<F><<<grid, kThreads, 0, s>>>(J, tw, twShift);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    switch (J.F) {
+    case 16: return build_lds_f<16>(J, tw, twShift, s);
+    case 32: return build_lds_f<32>(J, tw, twShift, s);
+    case 64: return build_lds_f<64>(J, tw, twShift, s);
+    case 128: return build_lds_f<128>(J, tw, twShift, s);
+    case 256: return build_lds_f<256>(J, tw, twShift, s);
+    case 512: return build_lds_f<512>(J, tw, twShift, s);
+    case 1024: return build_lds_f<1024>(J, tw, twShift, s);
+    case 2048: return build_lds_f<2048>(J, tw, twShift, s);
+    case 4096: return build_lds_f<4096>(J, tw, twShift, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    switch (J.F) {
+    case 16: return prop_lds_f<16>(J, tw, twShift, s);
+    case 32: return prop_lds_f<32>(J, tw, twShift, s);
+    case 64: return prop_lds_f<64>(J, tw, twShift, s);
+    case 128: return prop_lds_f<128>(J, tw, twShift, s);
+    case 256: return prop_lds_f<256>(J, tw, twShift, s);
+    case 512: return prop_lds_f<512>(J, tw, twShift, s);
+    case 1024: return prop_lds_f<1024>(J, tw, twShift, s);
+    case 2048: return prop_lds_f<2048>(J, tw, twShift, s);
+    case 4096: return prop_lds_f<4096>(J, tw, twShift, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class Loader>
+static hipError_t col_fwd(int F1, Loader ld, double2 *out, int nArr, int nq, int logF, const double2 *tw,
+                          int twShift, hipStream_t s) {
+    dim3 grid(F1, nArr, nq);  // tiles = F2 / TC = F1
+#define ICM_CF(V) case V: k_col_fwd<V, Loader><<<grid, kThreads, 0, s>>>(ld, out, nArr, logF, tw, twShift); break;
+    switch (F1) {
+        ICM_CF(2) ICM_CF(4) ICM_CF(8) ICM_CF(16) ICM_CF(32) ICM_CF(64) ICM_CF(128) ICM_CF(256) ICM_CF(512)
+    default: return hipErrorInvalidValue;
+    }
+#undef ICM_CF
+    return hipGetLastError();
+}
+
+template <class Storer>
+static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq, int logF, const double2 *tw,
+                          int twShift, hipStream_t s) {
+    dim3 grid(F1, nArr, nq);
+#define ICM_CI(V) case V: k_col_inv<V, Storer><<<grid, kThreads, 0, s>>>(in, st, nArr, logF, tw, twShift); break;
+    switch (F1) {
+        ICM_CI(2) ICM_CI(4) ICM_CI(8) ICM_CI(16) ICM_CI(32) ICM_CI(64) ICM_CI(128) ICM_CI(256) ICM_CI(512)
+    default: return hipErrorInvalidValue;
+    }
+#undef ICM_CI
+    return hipGetLastError();
+}
+
+static hipError_t row_pass(bool inv, const double2 *in, double2 *out, int F1, long long nArrTotal, int logF,
+                           const double2 *tw, int twShift, hipStream_t s) {
+    dim3 grid(F1, (unsigned)nArrTotal);
+    if (inv) k_row<true><<<grid, kThreads, 0, s>>>(in, out, logF, tw, twShift);
+    else k_row<false><<<grid, kThreads, 0, s>>>(in, out, logF, tw, twShift);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
+                              hipStream_t s) {
+    const int F1 = J.F >> 12;
+    const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
+    ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+    ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
+    const long long tot = (long long)J.nq * npairs * J.F;
+    k_pw_build<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufA, nrp, npairs, J.logF, J.nq);
+    ICM_TRY(hipGetLastError());
+    ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * npairs, J.logF, tw, twShift, s));
+    ICM_TRY(col_inv(F1, bufB, StParentPair{J}, npairs, J.nq, J.logF, tw, twShift, s));
+    return hipSuccess;
+}
+
+hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
+                             hipStream_t s) {
+    const int F1 = J.F >> 12;
+    const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
+    const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
+    ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+    ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
+    ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * (nrp + npairs), J.logF, tw, twShift, s));
+    const long long tot = (long long)J.nq * nrp * J.F;
+    k_pw_prop<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufB + r1, bufA, nrp, npairs, J.logF, J.nq);
+    ICM_TRY(hipGetLastError());
+    ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
+    ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
+    return hipSuccess;
+}
+
+}  // namespace icm

@@ -1,0 +1,78 @@
+// icm_kernels.hpp -- launch interface of the HIP kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace icm {
+
+// Device view of one tree level for the build / propagate kernels.
+// Coefficients of node j of q-plane qi live at
+//   base + qi * plane + j * stride + m,  m < count.
+struct CoefView {
+    const double *base;
+    long long plane;
+    int stride;
+    int count;
+};
+struct CoefOut {
+    double *base;
+    long long plane;
+    int stride;
+    int count;
+};
+
+struct LevelJob {
+    CoefView child;     // level l-1 products
+    CoefOut parent;     // level l products (build)
+    CoefView gpar;      // level l g-vectors (propagate); plane 0 at the root
+    CoefOut gchild;     // level l-1 g-vectors (propagate)
+    int nr_child;       // real nodes at level l-1
+    int nr_parent;      // real nodes at level l
+    int F;              // transform size of this level
+    int logF;
+    int nq;             // q-planes in the launch
+};
+
+struct LeafJob {
+    const double *S;        // sorted stacks [n]
+    const double *logv;     // [nq]
+    const double *wq;       // [nq] quadrature weights
+    const double *a_in;     // optional [nq][n] a values (replaces exp(S logv))
+    double *blocks;         // [nq][nb][B+1]
+    const double *g;        // g at the leaves: base, plane, stride
+    long long g_plane;
+    int g_stride;
+    int G0;                 // g entries per block
+    int n, nb, B, k;
+    int nq;
+    double *partial;        // [nchunk][n] equity partials (sorted order)
+    double *inner_out;      // optional [nq][n] Psi values (sorted order)
+    int q_per_chunk;
+    const unsigned char *active;  // optional [n] (sorted order)
+};
+
+// Leaf (block) build: block products of B players per (block, q).
+hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s);
+// Leaf divide + dot + quadrature-weighted accumulation.
+hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s);
+// equity[perm[j]] = sum_c partial[c][j] (fixed order), masked for subsets.
+hipError_t launch_finalize(const double *partial, int nchunk, int n, const int *perm,
+                           const unsigned char *active, double *acc, int accumulate,
+                           hipStream_t s);
+hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s);
+
+// One tree level, LDS-resident transforms (F <= 4096).
+hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s);
+hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s);
+
+// One tree level through HBM with a three-pass four-step transform (F >= 8192).
+// bufA / bufB: scratch of at least large_scratch_elems(...) complex each.
+hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
+                              double2 *bufB, hipStream_t s);
+hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
+                             double2 *bufB, hipStream_t s);
+
+constexpr int kMaxLdsF = 4096;
+
+}  // namespace icm

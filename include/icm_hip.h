@@ -1,0 +1,139 @@
+/*
+ * icm_hip.h -- C ABI of the MI355X ICM equity engine (libicm_hip.so).
+ *
+ * Drop-in boundary for the reference's two public headers:
+ *   - src/cpu/icm.h:26-41        icm_init / icm_equity / icm_equity_subset
+ *   - src/gpu/icm_gpu.h:10-116   IcmGpuPlan, options, stats, plan API
+ * Plain pointers and sizes only.  All host pointers belong to the caller;
+ * the library copies what it keeps (S at plan creation, payout per call).
+ *
+ * Return conventions follow the reference: the icm_hip_* / icm_gpu_* compute
+ * calls return 0 on success and -1 on failure (message in
+ * icm_hip_last_error()); icm_hip_init / icm_gpu_init return 1 on success and
+ * 0 on failure (src/gpu/gpu_api.cu:61-81); the CPU-named aliases return void
+ * like src/cpu/icm.h and leave the output zeroed on failure.
+ */
+#ifndef ICM_HIP_H
+#define ICM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct IcmHipPlan IcmHipPlan;
+
+/* Field order and types mirror IcmGpuOptions (src/gpu/icm_gpu.h:12-34) so a
+ * caller built against the reference header passes a compatible struct.
+ * Fields the reference uses for cuFFT/cuFFTDx tiers have MI355X meanings or
+ * are ignored, as noted. */
+typedef struct {
+    int device_id;                    /* informational; set by the library */
+    int use_cufftdx;                  /* ignored (no vendor FFT library is used) */
+    int enable_graphs;                /* ignored by the single-instance path */
+    int enable_q_pipeline;            /* ignored: all q-planes of a batch run in one launch per level */
+    int memory_strategy;              /* 0 = auto; others accepted and treated as auto */
+    int force_uncached_fused_levels;  /* reinterpreted: block size B (16, 32 or 64); <= 0 = auto */
+    int force_uncached_cufft_levels;  /* reinterpreted: q-planes per batch; <= 0 = auto */
+    int fast_mode;                    /* ignored */
+    int verbose;                      /* print the plan to stderr */
+} IcmHipOptions;
+
+/* Mirrors IcmGpuRunStats (src/gpu/icm_gpu.h:36-50). */
+typedef struct {
+    double total_ns;                  /* device time of the run (events on the plan's stream) */
+    double sort_ns;
+    double quadrature_overhead_ns;
+    double block_build_ns;
+    double tree_build_ns;
+    double tree_propagate_cached_ns;
+    double tree_propagate_recomputed_ns;
+    double leaf_extract_ns;
+    size_t peak_vram_bytes;
+    int engine;                       /* 1 = hybrid tree */
+    int B;
+    int uncached_fused_levels;        /* number of LDS-resident levels */
+    int uncached_cufft_levels;        /* number of four-step (HBM) levels */
+} IcmHipRunStats;
+
+/* Mirrors IcmGpuPlanSummary (src/gpu/icm_gpu.h:52-63). */
+typedef struct {
+    int n;
+    int k;
+    int B;
+    int engine;
+    int n_levels;
+    int n_tier1;                      /* levels with transform size <= 64 */
+    int n_tier2;                      /* other LDS-resident levels (<= 4096) */
+    int n_tier3;                      /* four-step levels (>= 8192) */
+    int q_batch;
+    size_t planned_peak_vram_bytes;
+} IcmHipPlanSummary;
+
+/* ---- lifecycle (src/gpu/icm_gpu.h:66-76) ---- */
+int icm_hip_init(int device_id);               /* 1 = ok, 0 = failure */
+void icm_hip_shutdown(void);
+const char *icm_hip_last_error(void);
+int icm_hip_release_pooled_memory(void);       /* 0 = ok (no pool is kept) */
+
+/* ---- plan (src/gpu/icm_gpu.h:79-81) ---- */
+IcmHipPlan *icm_hip_plan_create(int n, const double *S, int k, const IcmHipOptions *opts);
+void icm_hip_plan_destroy(IcmHipPlan *plan);
+int icm_hip_plan_summary(const IcmHipPlan *plan, IcmHipPlanSummary *summary);
+
+/* ---- compute (src/gpu/icm_gpu.h:94-111) ---- */
+int icm_hip_equity_with_plan(IcmHipPlan *plan, int Q, const double *payout, double *equity,
+                             IcmHipRunStats *stats);
+int icm_hip_equity(int n, const double *S, int Q, const double *payout, int k, double *equity);
+int icm_hip_equity_subset(int n, const double *S, int Q, const double *payout, int k,
+                          double *equity, const int *targets, int n_targets);
+
+/* Device-resident variant of icm_hip_equity_with_plan: d_payout[k] and
+ * d_equity[n] are device pointers; nothing crosses PCIe.  Runs on the plan's
+ * stream and returns after the stream has finished.  No reference
+ * counterpart (the reference copies payout in and equity out per call,
+ * src/gpu/gpu_api.cu:270,455). */
+int icm_hip_equity_with_plan_device(IcmHipPlan *plan, int Q, const double *d_payout,
+                                    double *d_equity, IcmHipRunStats *stats);
+
+/* One quadrature node through the GPU tree: inner[i] = Psi_i for the given
+ * a[i] (players in the given order), mirroring the IcmEngine hook
+ * (src/cpu/icm.h:45-47, engine_hybrid_ctx src/cpu/icm.c:2307-2318). */
+int icm_hip_node_inner(int n, const double *a, const double *payout, int k, double *inner);
+
+/* ---- reference GPU names (src/gpu/icm_gpu.h:66-111), same signatures ---- */
+int icm_gpu_init(int device_id);
+void icm_gpu_shutdown(void);
+const char *icm_gpu_last_error(void);
+int icm_gpu_release_pooled_memory(void);
+IcmHipPlan *icm_gpu_plan_create(int n, const double *S, int k, const IcmHipOptions *opts);
+void icm_gpu_plan_destroy(IcmHipPlan *plan);
+int icm_gpu_plan_summary(const IcmHipPlan *plan, IcmHipPlanSummary *summary);
+int icm_gpu_equity_with_plan(IcmHipPlan *plan, int Q, const double *payout, double *equity,
+                             IcmHipRunStats *stats);
+int icm_gpu_equity(int n, const double *S, int Q, const double *payout, int k, double *equity,
+                   const IcmHipOptions *opts, IcmHipRunStats *stats);
+int icm_gpu_equity_subset(int n, const double *S, int Q, const double *payout, int k,
+                          double *equity, const int *targets, int n_targets,
+                          const IcmHipOptions *opts, IcmHipRunStats *stats);
+
+/* ---- CPU-named aliases bound by python/icm/__init__.py:92-119 ---- */
+void icm_init(const char *wisdom_path);        /* wisdom_path ignored; selects device 0 */
+void icm_equity(int n, const double *S, int Q, const double *payout, int k, double *equity);
+void icm_equity_subset(int n, const double *S, int Q, const double *payout, int k,
+                       double *equity, const int *targets, int n_targets);
+
+/* ---- host-only planner query (no GPU needed) ----
+ * Fills per-level geometry for (n, k, B): n_real, coefficient count c,
+ * g length G, and transform size F (F[0] = 0).  Arrays need max_levels
+ * entries.  Returns the level count L, or -1 on invalid input. */
+int icm_hip_plan_geometry(int n, int k, int B, int max_levels, int *n_real, int *c, int *G, int *F);
+
+/* erfc-trap nodes (src/cpu/icm.c:117-132), host-only. */
+void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICM_HIP_H */

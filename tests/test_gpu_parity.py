@@ -1,0 +1,177 @@
+"""GPU parity: the HIP engine (through the C ABI) against the pinned oracle.
+
+Bars:
+  * per player |gpu - cpu| / |cpu| < 1e-8, denominator 1 when |cpu| < 1e-14
+    (bench/bench_gpu.cu:18-23,157);
+  * golden vectors at 5e-12 x max|E| (web/verify/verify_node.mjs:10);
+  * per-node closed forms at k = n (exact identities, any n).
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import icm_amd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_cases.json")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(gpu, cpu):
+    den = np.abs(cpu).copy()
+    den[den < 1e-14] = 1.0
+    return float(np.max(np.abs(gpu - cpu) / den))
+
+
+def bench_stacks(n, dist):
+    """Stack distributions of bench/bench_gpu.cu:25-47 (own generator)."""
+    rng = np.random.default_rng(42)
+    if dist == 0:
+        return 1.0 + 99.0 * rng.uniform(0, 1, n)
+    if dist == 1:
+        S = np.ones(n)
+        S[0] = 10000.0
+        return S
+    if dist == 2:
+        return 2.0 ** (np.arange(n) * 10.0 / n)
+    return np.full(n, 100.0)
+
+
+# ---------------------------------------------------------------- per node
+@pytest.mark.parametrize("n,k", [(5, 3), (16, 16), (33, 7), (100, 100), (1000, 150), (4096, 4096),
+                                 (5000, 1500), (20000, 20000), (16384, 1000)])
+def test_node_inner_vs_oracle(n, k):
+    say(f"node_inner n={n} k={k}")
+    rng = np.random.default_rng(n * 7 + k)
+    a = rng.uniform(0, 1, n)
+    a[: min(n, 6)] = [0.0, 1.0, 0.5, 1e-16, 1 - 1e-12, 0.25][: min(n, 6)]
+    p = np.sort(rng.uniform(0, 1, k))[::-1]
+    gpu = icm_amd.node_inner(a, p)
+    cpu = O.inner(a, p, engine=O.ENGINE_HYBRID, B=32, fft_min=64)
+    scale = np.max(np.abs(cpu))
+    err = np.max(np.abs(gpu - cpu)) / scale
+    say(f"  max err / max|psi| = {err:.3e}")
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize("n", [1000, 65536, 262144])
+def test_node_closed_forms_k_eq_n(n):
+    say(f"node closed forms n={n}")
+    rng = np.random.default_rng(n)
+    a = rng.uniform(0, 1, n)
+    A = a.sum() - a
+    B2 = (a * a).sum() - a * a
+    r = np.arange(n, dtype=float)
+    psi1 = icm_amd.node_inner(a, n - r)
+    e1 = np.max(np.abs(psi1 - (1.0 + A)) / (1.0 + A))
+    psi2 = icm_amd.node_inner(a, (n - r) ** 2)
+    quad = 1.0 + 3.0 * A + A * A - B2
+    e2 = np.max(np.abs(psi2 - quad) / quad)
+    say(f"  linear {e1:.3e} quadratic {e2:.3e}")
+    assert e1 < 1e-10 and e2 < 1e-10
+
+
+# ---------------------------------------------------------------- equities
+@pytest.mark.parametrize("case", json.load(open(GOLDEN)), ids=lambda c: c["name"])
+def test_golden_vectors_gpu(case):
+    say(f"golden {case['name']}")
+    S = np.array(case["stacks"])
+    p = np.array(case["payouts"])
+    ref = np.array(case["equity"])
+    eq = icm_amd.equity(S, p, Q=case["Q"])
+    err = np.max(np.abs(eq - ref)) / np.max(np.abs(ref))
+    say(f"  err {err:.3e}")
+    assert err <= 5e-12
+
+
+def test_config1_small_k():
+    """n=1024, k=10, Q=256, integer stacks U{1..1e6}, payout ~ 1/(j+1)."""
+    rng = np.random.default_rng(1)
+    S = rng.integers(1, 10 ** 6 + 1, 1024).astype(float)
+    p = 1.0 / np.arange(1, 11)
+    p /= p.sum()
+    eq = icm_amd.equity(S, p, Q=256)
+    cpu = O.equity(S, p, Q=256, fft_min=0)
+    err = rel_err(eq, cpu)
+    say(f"config1 rel err {err:.3e}")
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("n", [64, 256, 1024, 4096, 16384])
+@pytest.mark.parametrize("dist", [0, 1, 2])
+def test_verify_grid(n, dist):
+    """bench/bench_gpu.cu:65-192 grid: k in {min(100, n), n}, payout n - m."""
+    S = bench_stacks(n, dist)
+    for k in sorted({min(100, n), n}):
+        say(f"verify n={n} dist={dist} k={k}")
+        p = np.arange(n, n - k, -1, dtype=float)
+        eq = icm_amd.equity(S, p, Q=256)
+        cpu = O.equity(S, p, Q=256, fft_min=64)
+        err = rel_err(eq, cpu)
+        say(f"  rel err {err:.3e}")
+        assert err < 1e-8
+
+
+def test_config2_full_field():
+    """n=65536, k=n, lognormal(0,1) scaled to 1e9 chips, payout ~ 0.8^j."""
+    rng = np.random.default_rng(2)
+    n = 65536
+    S = rng.lognormal(0.0, 1.0, n)
+    S *= 1e9 / S.sum()
+    p = 0.8 ** np.arange(n)
+    p /= p.sum()
+    eq = icm_amd.equity(S, p, Q=256)
+    cpu = O.equity(S, p, Q=256, fft_min=64)
+    err = rel_err(eq, cpu)
+    say(f"config2 rel err {err:.3e}")
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("n,k", [(65536, 1024), (65536, 2048), (131072, 1000)])
+def test_k_band(n, k):
+    """k in the 1024-2048 band at n >= 65536 (bench/bench_gpu.cu:115-119)."""
+    S = bench_stacks(n, 0)
+    p = np.arange(n, n - k, -1, dtype=float)
+    say(f"k band n={n} k={k}")
+    eq = icm_amd.equity(S, p, Q=256)
+    cpu = O.equity(S, p, Q=256, fft_min=64)
+    err = rel_err(eq, cpu)
+    say(f"  rel err {err:.3e}")
+    assert err < 1e-8
+
+
+def test_subset_matches_full():
+    rng = np.random.default_rng(9)
+    n, k = 1024, 100
+    S = rng.uniform(1, 100, n)
+    p = np.arange(n, n - k, -1, dtype=float)
+    targets = rng.choice(n, 10, replace=False)
+    full = icm_amd.equity(S, p, Q=256)
+    sub = icm_amd.equity_subset(S, p, targets, Q=256)
+    np.testing.assert_allclose(sub[targets], full[targets], rtol=1e-10)
+    mask = np.ones(n, bool)
+    mask[targets] = False
+    assert np.all(sub[mask] == 0.0)
+
+
+def test_reference_python_semantics():
+    """python/tests/test_icm.py behaviours through the drop-in."""
+    eq = icm_amd.equity([1000, 1000, 1000], [0.5, 0.3, 0.2])
+    np.testing.assert_allclose(eq, eq[0], rtol=1e-6)
+    eq = icm_amd.equity([1000, 2000, 3000], [0.5, 0.3, 0.2])
+    assert eq[2] > eq[1] > eq[0]
+    np.testing.assert_allclose(eq.sum(), 1.0, rtol=1e-8)
+    S = [100 * (i + 1) for i in range(10)]
+    np.testing.assert_allclose(icm_amd.equity(S, [1.0]), np.array(S) / sum(S), rtol=1e-10)
+    sub = icm_amd.equity_subset([1000, 2000, 3000, 4000], [0.5, 0.3, 0.2], [1, 3])
+    assert sub[0] == 0.0 and sub[2] == 0.0
