@@ -93,6 +93,10 @@ _SIGS = [
     ("icm_hip_plan_geometry", ctypes.c_int,
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I]),
     ("icm_hip_make_nodes", None, [ctypes.c_int, ctypes.c_double, _D, _D]),
+    ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_kernel_stat", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
 ]
 
 _lib = None
@@ -268,6 +272,23 @@ class Plan:
                                                      ctypes.byref(self.stats)) != 0:
             raise _err(self._lib, "icm_hip_equity_with_plan_device")
         return self.stats.total_ns
+
+    def set_profiling(self, on=True):
+        """Per-kernel event timing for later runs (clears the aggregates)."""
+        if self._lib.icm_hip_plan_set_profiling(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_profiling")
+
+    def kernel_stats(self):
+        """{name: (launches, total_ns, alg_bytes, alg_flops)} since set_profiling."""
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        for i in range(self._lib.icm_hip_plan_kernel_count(self._h)):
+            ln = ctypes.c_int(0)
+            ns, by, fl = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+            self._lib.icm_hip_plan_kernel_stat(self._h, i, buf, 64, ctypes.byref(ln), ctypes.byref(ns),
+                                               ctypes.byref(by), ctypes.byref(fl))
+            out[buf.value.decode()] = (ln.value, ns.value, by.value, fl.value)
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -107,6 +107,55 @@ int pick_B(int n, int k, int forced) {
     return 32;
 }
 
+// Per-launch event timing, aggregated per kernel name (see icm::Prof).
+struct KernelProf : icm::Prof {
+    struct Rec { std::string name; hipEvent_t a, b; double bytes, flops; };
+    struct Agg { std::string name; int launches = 0; double ns = 0, bytes = 0, flops = 0; };
+    bool enabled = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<Rec> pending;
+    std::vector<Agg> agg;
+    hipEvent_t cur_a = nullptr;
+    ~KernelProf() override { for (hipEvent_t e : pool) (void)hipEventDestroy(e); }
+    bool on() const override { return enabled; }
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(hipStream_t s) override {
+        cur_a = take();
+        if (cur_a) (void)hipEventRecord(cur_a, s);
+    }
+    void end(hipStream_t s, const char *name, double bytes, double flops) override {
+        hipEvent_t b = take();
+        if (!cur_a || !b) return;
+        (void)hipEventRecord(b, s);
+        pending.push_back(Rec{name, cur_a, b, bytes, flops});
+        cur_a = nullptr;
+    }
+    // After the stream has finished: fold pending records into the aggregates.
+    void collect() {
+        for (const Rec &r : pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+            Agg *g = nullptr;
+            for (Agg &x : agg) if (x.name == r.name) { g = &x; break; }
+            if (!g) { agg.push_back(Agg{}); g = &agg.back(); g->name = r.name; }
+            g->launches += 1;
+            g->ns += (double)ms * 1e6;
+            g->bytes += r.bytes;
+            g->flops += r.flops;
+        }
+        pending.clear();
+        used = 0;
+    }
+};
+
 }  // namespace
 
 struct IcmHipPlan {
@@ -141,6 +190,7 @@ struct IcmHipPlan {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t bytes = 0;
+    KernelProf prof;
 };
 
 namespace {
@@ -337,7 +387,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.blocks = p->d_blocks;
         lj.n = g.n; lj.nb = g.nb; lj.B = B; lj.k = g.k; lj.nq = nq;
         lj.G0 = g.G[0];
-        if (!hip_ok(icm::launch_leaf_build(lj, s), "leaf_build")) return -1;
+        if (!hip_ok(icm::launch_leaf_build(lj, s, &p->prof), "leaf_build")) return -1;
 
         // views of level l's coefficients
         auto coef_view = [&](int l) {
@@ -360,8 +410,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
             hipError_t e = (J.F <= icm::kMaxLdsF)
-                               ? icm::launch_build_lds(J, p->d_tw, p->twShift, s)
-                               : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s);
+                               ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
+                               : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
             if (!hip_ok(e, "build level")) return -1;
         }
         // propagate: root g = payout (same for every q-plane)
@@ -382,8 +432,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
             hipError_t e = (J.F <= icm::kMaxLdsF)
-                               ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s)
-                               : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s);
+                               ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
+                               : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
             if (!hip_ok(e, "propagate level")) return -1;
         }
         if (L == 1) {
@@ -397,14 +447,15 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.partial = p->d_partial;
         lj.inner_out = p->d_inner ? p->d_inner + (long long)q0 * g.n : nullptr;
         lj.active = p->d_active;
-        if (!hip_ok(icm::launch_leaf_extract(lj, nch, s), "leaf_extract")) return -1;
-        if (!hip_ok(icm::launch_finalize(p->d_partial, nch, g.n, p->d_perm, p->d_active, p->d_acc, 1, s),
+        if (!hip_ok(icm::launch_leaf_extract(lj, nch, s, &p->prof), "leaf_extract")) return -1;
+        if (!hip_ok(icm::launch_finalize(p->d_partial, nch, g.n, p->d_active, p->d_acc, 1, s, &p->prof),
                     "finalize"))
             return -1;
     }
-    if (!hip_ok(icm::launch_scatter(p->d_acc, g.n, p->d_perm, p->d_equity, s), "scatter")) return -1;
+    if (!hip_ok(icm::launch_scatter(p->d_acc, g.n, p->d_perm, p->d_equity, s, &p->prof), "scatter")) return -1;
     if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
     if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (p->prof.enabled) p->prof.collect();
     if (stats) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
@@ -609,5 +660,33 @@ int icm_hip_plan_geometry(int n, int k, int B, int max_levels, int *n_real, int 
 }
 
 void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w) { make_nodes(Q, Smax, logv, w); }
+
+int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->prof.enabled = on != 0;
+    plan->prof.agg.clear();
+    plan->prof.pending.clear();
+    plan->prof.used = 0;
+    return 0;
+}
+
+int icm_hip_plan_kernel_count(const IcmHipPlan *plan) {
+    return plan ? (int)plan->prof.agg.size() : -1;
+}
+
+int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
+                             double *total_ns, double *alg_bytes, double *alg_flops) {
+    if (!plan || index < 0 || index >= (int)plan->prof.agg.size()) { set_err("bad kernel index"); return -1; }
+    const auto &a = plan->prof.agg[index];
+    if (name && name_len > 0) {
+        strncpy(name, a.name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (launches) *launches = a.launches;
+    if (total_ns) *total_ns = a.ns;
+    if (alg_bytes) *alg_bytes = a.bytes;
+    if (alg_flops) *alg_flops = a.flops;
+    return 0;
+}
 
 }  // extern "C"

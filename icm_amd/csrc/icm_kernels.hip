@@ -13,6 +13,9 @@
 #include "fft_lds.hpp"
 #include "icm_kernels.hpp"
 
+#include <cmath>
+#include <cstdio>
+
 namespace icm {
 
 // ---------------------------------------------------------------------------
@@ -574,38 +577,60 @@ __global__ __launch_bounds__(256) void k_pw_prop(const double2 *__restrict__ X1,
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
-hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s) {
+static inline double fft_flops(double N) { return N <= 1 ? 0.0 : 5.0 * N * std::log2(N); }
+
+// Profiler record names follow the kernel template instantiations.
+static const char *kname(const char *fmt, int v) {
+    static thread_local char buf[96];
+    snprintf(buf, sizeof buf, fmt, v);
+    return buf;
+}
+
+#define PF_BEGIN() do { if (pf && pf->on()) pf->begin(s); } while (0)
+#define PF_END(name, bytes, flops) do { if (pf && pf->on()) pf->end(s, name, (bytes), (flops)); } while (0)
+
+hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s, Prof *pf) {
     dim3 grid((J.nb + 63) / 64, J.nq);
+    PF_BEGIN();
     switch (J.B) {
     case 16: k_leaf_build<16><<<grid, 64, 0, s>>>(J); break;
     case 32: k_leaf_build<32><<<grid, 64, 0, s>>>(J); break;
     case 64: k_leaf_build<64><<<grid, 64, 0, s>>>(J); break;
     default: return hipErrorInvalidValue;
     }
+    const double nq = J.nq, B = J.B;
+    PF_END(kname("k_leaf_build<%d>", J.B), 8.0 * J.n + 8.0 * nq * J.nb * (B + 1), nq * J.nb * (1.5 * B * (B + 1) + 20.0 * B));
     return hipGetLastError();
 }
 
-hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s) {
+hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof *pf) {
     const int bpw = 256 / J.B;
     dim3 grid((J.nb + bpw - 1) / bpw, nchunk);
+    PF_BEGIN();
     switch (J.B) {
     case 16: k_leaf_extract<16><<<grid, 256, 0, s>>>(J); break;
     case 32: k_leaf_extract<32><<<grid, 256, 0, s>>>(J); break;
     case 64: k_leaf_extract<64><<<grid, 256, 0, s>>>(J); break;
     default: return hipErrorInvalidValue;
     }
+    const double nq = J.nq, B = J.B;
+    PF_END(kname("k_leaf_extract<%d>", J.B), 8.0 * nq * J.nb * (B + 1 + J.G0) + 8.0 * J.n + 8.0 * nchunk * J.n,
+           nq * J.n * (4.0 * B + 30.0));
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const double *partial, int nchunk, int n, const int *perm,
-                           const unsigned char *active, double *acc, int accumulate, hipStream_t s) {
-    (void)perm;
+hipError_t launch_finalize(const double *partial, int nchunk, int n, const unsigned char *active, double *acc,
+                           int accumulate, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
     k_finalize<<<(n + 255) / 256, 256, 0, s>>>(partial, nchunk, n, active, acc, accumulate);
+    PF_END("k_finalize", 8.0 * n * (nchunk + 2), (double)n * nchunk);
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s) {
+hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
     k_scatter<<<(n + 255) / 256, 256, 0, s>>>(acc, n, perm, equity);
+    PF_END("k_scatter", 20.0 * n, 0.0);
     return hipGetLastError();
 }
 
@@ -626,34 +651,48 @@ static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, 
     return hipGetLastError();
 }
 
-hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+static double level_coef_bytes(const CoefView &v, int nodes, int nq) { return 8.0 * nq * (double)nodes * v.count; }
+
+hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
+    hipError_t e;
     switch (J.F) {
-    case 16: return build_lds_f<16>(J, tw, twShift, s);
-    case 32: return build_lds_f<32>(J, tw, twShift, s);
-    case 64: return build_lds_f<64>(J, tw, twShift, s);
-    case 128: return build_lds_f<128>(J, tw, twShift, s);
-    case 256: return build_lds_f<256>(J, tw, twShift, s);
-    case 512: return build_lds_f<512>(J, tw, twShift, s);
-    case 1024: return build_lds_f<1024>(J, tw, twShift, s);
-    case 2048: return build_lds_f<2048>(J, tw, twShift, s);
-    case 4096: return build_lds_f<4096>(J, tw, twShift, s);
+    case 16: e = build_lds_f<16>(J, tw, twShift, s); break;
+    case 32: e = build_lds_f<32>(J, tw, twShift, s); break;
+    case 64: e = build_lds_f<64>(J, tw, twShift, s); break;
+    case 128: e = build_lds_f<128>(J, tw, twShift, s); break;
+    case 256: e = build_lds_f<256>(J, tw, twShift, s); break;
+    case 512: e = build_lds_f<512>(J, tw, twShift, s); break;
+    case 1024: e = build_lds_f<1024>(J, tw, twShift, s); break;
+    case 2048: e = build_lds_f<2048>(J, tw, twShift, s); break;
+    case 4096: e = build_lds_f<4096>(J, tw, twShift, s); break;
     default: return hipErrorInvalidValue;
     }
+    const double bytes = level_coef_bytes(J.child, J.nr_child, J.nq) + 8.0 * J.nq * (double)J.nr_parent * J.parent.count;
+    PF_END(kname("k_build_lds<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 1.5 * fft_flops(J.F));
+    return e;
 }
 
-hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
+    hipError_t e;
     switch (J.F) {
-    case 16: return prop_lds_f<16>(J, tw, twShift, s);
-    case 32: return prop_lds_f<32>(J, tw, twShift, s);
-    case 64: return prop_lds_f<64>(J, tw, twShift, s);
-    case 128: return prop_lds_f<128>(J, tw, twShift, s);
-    case 256: return prop_lds_f<256>(J, tw, twShift, s);
-    case 512: return prop_lds_f<512>(J, tw, twShift, s);
-    case 1024: return prop_lds_f<1024>(J, tw, twShift, s);
-    case 2048: return prop_lds_f<2048>(J, tw, twShift, s);
-    case 4096: return prop_lds_f<4096>(J, tw, twShift, s);
+    case 16: e = prop_lds_f<16>(J, tw, twShift, s); break;
+    case 32: e = prop_lds_f<32>(J, tw, twShift, s); break;
+    case 64: e = prop_lds_f<64>(J, tw, twShift, s); break;
+    case 128: e = prop_lds_f<128>(J, tw, twShift, s); break;
+    case 256: e = prop_lds_f<256>(J, tw, twShift, s); break;
+    case 512: e = prop_lds_f<512>(J, tw, twShift, s); break;
+    case 1024: e = prop_lds_f<1024>(J, tw, twShift, s); break;
+    case 2048: e = prop_lds_f<2048>(J, tw, twShift, s); break;
+    case 4096: e = prop_lds_f<4096>(J, tw, twShift, s); break;
     default: return hipErrorInvalidValue;
     }
+    const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)J.nr_parent * J.gpar.count;
+    const double bytes = gin + level_coef_bytes(J.child, J.nr_child, J.nq) +
+                         8.0 * J.nq * (double)J.nr_child * J.gchild.count;
+    PF_END(kname("k_prop_lds<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 2.5 * fft_flops(J.F));
+    return e;
 }
 
 template <class Loader>
@@ -691,32 +730,61 @@ static hipError_t row_pass(bool inv, const double2 *in, double2 *out, int F1, lo
 }
 
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                              hipStream_t s) {
+                              hipStream_t s, Prof *pf) {
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
+    const double F = J.F, nq = J.nq;
+    const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    PF_BEGIN();
     ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
+           nq * nrp * colFl);
+    PF_BEGIN();
     ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
+    PF_END("k_row<false>", 32.0 * nq * nrp * F, nq * nrp * rowFl);
     const long long tot = (long long)J.nq * npairs * J.F;
+    PF_BEGIN();
     k_pw_build<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufA, nrp, npairs, J.logF, J.nq);
     ICM_TRY(hipGetLastError());
+    PF_END("k_pw_build", 16.0 * nq * (nrp + npairs) * F, 30.0 * nq * nrp * F);
+    PF_BEGIN();
     ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * npairs, J.logF, tw, twShift, s));
+    PF_END("k_row<true>", 32.0 * nq * npairs * F, nq * npairs * (rowFl + 6.0 * F));
+    PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StParentPair{J}, npairs, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_inv<%d,StParentPair>", F1), 16.0 * nq * npairs * F + 8.0 * nq * nrp * J.parent.count, nq * npairs * kF2 * fft_flops(F1));
     return hipSuccess;
 }
 
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                             hipStream_t s) {
+                             hipStream_t s, Prof *pf) {
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
+    const double F = J.F, nq = J.nq;
+    const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * nq * nrp * J.gpar.count;
+    PF_BEGIN();
     ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
+           nq * nrp * colFl);
+    PF_BEGIN();
     ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    PF_BEGIN();
     ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * (nrp + npairs), J.logF, tw, twShift, s));
+    PF_END("k_row<false>", 32.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl);
     const long long tot = (long long)J.nq * nrp * J.F;
+    PF_BEGIN();
     k_pw_prop<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufB + r1, bufA, nrp, npairs, J.logF, J.nq);
     ICM_TRY(hipGetLastError());
+    PF_END("k_pw_prop", 16.0 * nq * (2 * nrp + npairs) * F, 40.0 * nq * nrp * F);
+    PF_BEGIN();
     ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
+    PF_END("k_row<true>", 32.0 * nq * nrp * F, nq * nrp * (rowFl + 6.0 * F));
+    PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_inv<%d,StGChildren>", F1), 16.0 * nq * nrp * F + 8.0 * nq * J.nr_child * J.gchild.count, nq * nrp * kF2 * fft_flops(F1));
     return hipSuccess;
 }
 

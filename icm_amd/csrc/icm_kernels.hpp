@@ -52,26 +52,37 @@ struct LeafJob {
     const unsigned char *active;  // optional [n] (sorted order)
 };
 
+// Optional per-launch timing: when on, every kernel launch is bracketed by
+// two events on the launch stream and tagged with its compulsory HBM bytes
+// and FLOPs (inputs read once + outputs written once; 5 N log2 N per complex
+// transform of size N).
+struct Prof {
+    virtual ~Prof() {}
+    virtual bool on() const = 0;
+    virtual void begin(hipStream_t s) = 0;
+    virtual void end(hipStream_t s, const char *name, double bytes, double flops) = 0;
+};
+
 // Leaf (block) build: block products of B players per (block, q).
-hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s);
+hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s, Prof *pf);
 // Leaf divide + dot + quadrature-weighted accumulation.
-hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s);
-// equity[perm[j]] = sum_c partial[c][j] (fixed order), masked for subsets.
-hipError_t launch_finalize(const double *partial, int nchunk, int n, const int *perm,
-                           const unsigned char *active, double *acc, int accumulate,
-                           hipStream_t s);
-hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s);
+hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof *pf);
+// acc[j] (+)= sum_c partial[c][j] (fixed order), masked for subsets.
+hipError_t launch_finalize(const double *partial, int nchunk, int n, const unsigned char *active,
+                           double *acc, int accumulate, hipStream_t s, Prof *pf);
+hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s,
+                          Prof *pf);
 
 // One tree level, LDS-resident transforms (F <= 4096).
-hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s);
-hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s);
+hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);
+hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);
 
 // One tree level through HBM with a three-pass four-step transform (F >= 8192).
-// bufA / bufB: scratch of at least large_scratch_elems(...) complex each.
+// bufA / bufB: scratch of (n_real[l] + ceil(n_real[l]/2)) * F * nq complex each.
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                              double2 *bufB, hipStream_t s);
+                              double2 *bufB, hipStream_t s, Prof *pf);
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                             double2 *bufB, hipStream_t s);
+                             double2 *bufB, hipStream_t s, Prof *pf);
 
 constexpr int kMaxLdsF = 4096;
 

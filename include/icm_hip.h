@@ -133,6 +133,17 @@ int icm_hip_plan_geometry(int n, int k, int B, int max_levels, int *n_real, int 
 /* erfc-trap nodes (src/cpu/icm.c:117-132), host-only. */
 void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w);
 
+/* ---- per-kernel timing (plays the role of the reference's instrumented
+ * run, IcmGpuRunStats phase fields src/gpu/icm_gpu.h:36-50, filled per kernel
+ * instead of per phase) ----
+ * on != 0 brackets every launch of later runs with events on the plan's
+ * stream and clears the aggregates; kernel_stat reports, per kernel name,
+ * launches, summed device time and summed compulsory bytes / FLOPs. */
+int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on);
+int icm_hip_plan_kernel_count(const IcmHipPlan *plan);
+int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
+                             double *total_ns, double *alg_bytes, double *alg_flops);
+
 #ifdef __cplusplus
 }
 #endif
