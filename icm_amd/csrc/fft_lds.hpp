@@ -76,6 +76,8 @@ template <bool INV> __device__ __forceinline__ void dft8(double2 *a) {
 }
 
 // 16-point DFT as 4 x 4: X[k1 + 4 k2] = sum_s w4^(s k2) w16^(s k1) DFT4_u(a[s + 4u])[k1].
+// Output is left in a[4 k1 + k2] (digit-reversed); callers map register j to
+// output index out16(j) = j / 4 + 4 (j % 4) when storing.
 template <bool INV> __device__ __forceinline__ void dft16(double2 *a) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) dft4<INV>(a[s], a[s + 4], a[s + 8], a[s + 12]);
@@ -91,14 +93,11 @@ template <bool INV> __device__ __forceinline__ void dft16(double2 *a) {
     a[15] = mul_k<INV>(a[15], -kC8, kS8);    // s=3,k1=3: w9
 #pragma unroll
     for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
-    // X[k1 + 4 k2] currently sits at a[4 k1 + k2]: transpose 4x4.
-    double2 t[16];
-#pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1)
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) t[k1 + 4 * k2] = a[4 * k1 + k2];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) a[i] = t[i];
+}
+
+// Output index held by register j after dft<R>.
+template <int R> __device__ __forceinline__ constexpr int dft_out(int j) {
+    return R == 16 ? (j >> 2) + 4 * (j & 3) : j;
 }
 
 template <int R, bool INV> __device__ __forceinline__ void dft(double2 *a) {
@@ -110,20 +109,48 @@ template <int R, bool INV> __device__ __forceinline__ void dft(double2 *a) {
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
-// One Stockham pass of radix R over all transforms of size FS in the tile.
+// v[s] *= w^s (s = 1..R-1), w = exp(-+2 pi i k / (Ns R)).  Only w, w^2, w^4
+// and w^8 come from the table (correctly rounded); the other powers are
+// products of at most three of them, so each twiddle carries a few ulp at
+// most.  Loading 4 instead of 15 entries keeps the register footprint small.
+// The table pointer is deliberately not __restrict__: its loads then stay
+// below the preceding barrier instead of being hoisted across passes.
+template <int R, bool INV>
+__device__ __forceinline__ void twiddle(double2 *v, const double2 *tw, int k, int twSh) {
+    double2 w[R];
+    w[1] = tw[k << twSh];
+    if constexpr (R >= 4) w[2] = tw[(2 * k) << twSh];
+    if constexpr (R >= 8) w[4] = tw[(4 * k) << twSh];
+    if constexpr (R >= 16) w[8] = tw[(8 * k) << twSh];
+    if constexpr (R >= 4) w[3] = c_mul(w[1], w[2]);
+    if constexpr (R >= 8) {
+        w[5] = c_mul(w[1], w[4]);
+        w[6] = c_mul(w[2], w[4]);
+        w[7] = c_mul(w[3], w[4]);
+    }
+    if constexpr (R >= 16) {
+#pragma unroll
+        for (int s = 9; s < 16; ++s) w[s] = c_mul(w[s - 8], w[8]);
+    }
+#pragma unroll
+    for (int s = 1; s < R; ++s) v[s] = INV ? c_mulc(v[s], w[s]) : c_mul(v[s], w[s]);
+}
+
+// One Stockham pass of radix R over all transforms of size FS in the tile,
+// executed by NT threads (each owning kTile / NT points).
 // Ns = product of the radices of earlier passes.
-template <int FS, int R, bool INV>
-__device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 *__restrict__ tw, int twShift) {
+template <int FS, int R, bool INV, int NT>
+__device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 *tw, int twShift) {
     constexpr int NBI = FS / R;               // butterflies per transform
     constexpr int LOG_NBI = ilog2c(NBI);
-    constexpr int PER = kPer / R;             // butterflies per thread
+    constexpr int PER = (kTile / NT) / R;     // butterflies per thread
     const int t = threadIdx.x;
     const int Ns = 1 << logNs;
     double2 v[PER][R];
     int item[PER], jj[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-        const int bj = t + u * kThreads;
+        const int bj = t + u * NT;
         item[u] = bj >> LOG_NBI;
         jj[u] = bj & (NBI - 1);
         const double2 *src = lds + item[u] * FS + jj[u];
@@ -135,38 +162,35 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int k = jj[u] & (Ns - 1);
-        if (logNs > 0) {
-#pragma unroll
-            for (int s = 1; s < R; ++s) {
-                double2 w = tw[(k * s) << twSh];
-                if (INV) w.y = -w.y;
-                v[u][s] = c_mul(v[u][s], w);
-            }
-        }
+        if (logNs > 0) twiddle<R, INV>(v[u], tw, k, twSh);
         dft<R, INV>(v[u]);
         double2 *dst = lds + item[u] * FS + (jj[u] - k) * R + k;
 #pragma unroll
-        for (int s = 0; s < R; ++s) dst[s * Ns] = v[u][s];
+        for (int j = 0; j < R; ++j) dst[dft_out<R>(j) * Ns] = v[u][j];
     }
     __syncthreads();
 }
 
-// In-place transform of every size-FS block in the 4096-point tile.
+// In-place transform of every size-FS block in the 4096-point tile by NT
+// threads (256: radix-16 passes; 512: radix-8 passes), after one mixed pass
+// when log2(FS) is not a multiple of the main radix's log.
 // Starts with a barrier so callers may write the tile and call directly;
 // ends with a barrier (the last pass's).
-template <int FS, bool INV>
-__device__ __forceinline__ void fft_tile(double2 *lds, const double2 *__restrict__ tw, int twShift) {
+template <int FS, bool INV, int NT = kThreads>
+__device__ __forceinline__ void fft_tile(double2 *lds, const double2 *tw, int twShift) {
+    constexpr int RMAX = kTile / NT;          // 16 or 8
+    constexpr int LR = ilog2c(RMAX);
     constexpr int LOG = ilog2c(FS);
-    constexpr int REM = LOG % 4;
+    constexpr int REM = LOG % LR;
     __syncthreads();
     int logNs = 0;
-    if constexpr (REM == 1) { fft_pass<FS, 2, INV>(lds, logNs, tw, twShift); logNs += 1; }
-    if constexpr (REM == 2) { fft_pass<FS, 4, INV>(lds, logNs, tw, twShift); logNs += 2; }
-    if constexpr (REM == 3) { fft_pass<FS, 8, INV>(lds, logNs, tw, twShift); logNs += 3; }
+    if constexpr (REM == 1) { fft_pass<FS, 2, INV, NT>(lds, logNs, tw, twShift); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass<FS, 4, INV, NT>(lds, logNs, tw, twShift); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass<FS, 8, INV, NT>(lds, logNs, tw, twShift); logNs += 3; }
 #pragma unroll
-    for (int p = 0; p < LOG / 4; ++p) {
-        fft_pass<FS, 16, INV>(lds, logNs, tw, twShift);
-        logNs += 4;
+    for (int p = 0; p < LOG / LR; ++p) {
+        fft_pass<FS, RMAX, INV, NT>(lds, logNs, tw, twShift);
+        logNs += LR;
     }
 }
 

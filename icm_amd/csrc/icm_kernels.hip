@@ -492,80 +492,150 @@ __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in,
     }
 }
 
-// Row pass over arrays of F = F1 * 4096 complex; grid (F1 rows, arrays).
-template <bool INV>
-__global__ __launch_bounds__(256) void k_row(const double2 *__restrict__ in, double2 *__restrict__ out, int logF,
-                                             const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 lds[kTile];
-    const int k1 = blockIdx.x;
-    const long long base = (long long)blockIdx.y * (1LL << logF) + (long long)k1 * kF2;
+// ---------------------------------------------------------------------------
+// Fused spectrum stage of a large level: one workgroup owns the row pair
+// {k1, F1 - k1} (rows 0 and F1/2 pair with themselves).  Every bin -K of row
+// k1 lies in row F1 - k1 (position F2-1-k2, or (F2-k2) mod F2 in row 0), so
+// the forward row FFTs, the real-pair split, the pointwise product and the
+// inverse row FFTs of both rows happen without leaving the workgroup.
+// Thread t owns positions k2 = t + 256u of row k1 and their mirrors.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mirror_k2(int k1, int k2) {
+    return (k1 == 0) ? ((kF2 - k2) & (kF2 - 1)) : (kF2 - 1 - k2);
+}
+
+// The fused row-pair kernels run 512 threads (8 points per thread) and keep
+// both rows' spectra in two LDS tiles (128 KiB): rows k1 -> T0, k1m -> T1.
+// Z[K] of position k2 of row k1 is T0[k2]; Z[-K] is T1[F2-1-k2] (or
+// T0[mirror] for the self-paired rows 0 and F1/2).
+constexpr int kRT = 512;
+constexpr int kRP = kTile / kRT;
+
+__device__ __forceinline__ int mirror_idx(int k1, int k1m, int k2) {
+    return (k1 == k1m) ? mirror_k2(k1, k2) : kTile + (kF2 - 1 - k2);
+}
+
+// Load rows k1 / k1m of one [k1][n2] array into the tiles and transform them.
+__device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__restrict__ Zb, int k1, int k1m,
+                                                 const double2 *tw, int twShift) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
-        lds[e] = in[base + e];
-    }
-    fft_tile<kF2, INV>(lds, tw, twShift);
-    const int F = 1 << logF;
-    const int sh = twShift - logF;
+    for (int u = 0; u < kRP; ++u) T[t + u * kRT] = Zb[(long long)k1 * kF2 + t + u * kRT];
+    if (k1 != k1m) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
-        double2 v = lds[e];
-        if (INV) v = c_mulc(v, tw[((e * k1) & (F - 1)) << sh]);
-        out[base + e] = v;
+        for (int u = 0; u < kRP; ++u) T[kTile + t + u * kRT] = Zb[(long long)k1m * kF2 + t + u * kRT];
     }
+    fft_tile<kF2, false, kRT>(T, tw, twShift);
+    if (k1 != k1m) fft_tile<kF2, false, kRT>(T + kTile, tw, twShift);
 }
 
-__device__ __forceinline__ int mirror_pos(int pos, int F1) {
-    const int k1 = pos >> 12, k2 = pos & (kF2 - 1);
-    return (k1 == 0) ? ((kF2 - k2) & (kF2 - 1)) : ((F1 - k1) * kF2 + (kF2 - 1 - k2));
+// Inverse-transform the tiles in place and store rows k1 / k1m of V with the
+// inverse twiddle w_F^(-n2 row).
+__device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m, int logF,
+                                                   const double2 *tw, int twShift) {
+    const int t = threadIdx.x;
+    const int F = 1 << logF, sh = twShift - logF;
+    fft_tile<kF2, true, kRT>(T, tw, twShift);
+    if (k1 != k1m) fft_tile<kF2, true, kRT>(T + kTile, tw, twShift);
+#pragma unroll
+    for (int u = 0; u < kRP; ++u) {
+        const int e = t + u * kRT;
+        Vb[(long long)k1 * kF2 + e] = c_mulc(T[e], tw[((e * k1) & (F - 1)) << sh]);
+    }
+    if (k1 != k1m) {
+#pragma unroll
+        for (int u = 0; u < kRP; ++u) {
+            const int e = t + u * kRT;
+            Vb[(long long)k1m * kF2 + e] = c_mulc(T[kTile + e], tw[((e * k1m) & (F - 1)) << sh]);
+        }
+    }
+    __syncthreads();
 }
 
-// U[pair] = P_p + i P_{p+1}, P = X Y from the packed child spectra.
-__global__ __launch_bounds__(256) void k_pw_build(const double2 *__restrict__ X, double2 *__restrict__ U, int nrp,
-                                                  int npairs, int logF, int nq) {
-    const int F = 1 << logF;
-    const int F1 = F >> 12;
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long total = (long long)nq * npairs * F;
-    if (idx >= total) return;
-    const int pos = (int)(idx & (F - 1));
-    const long long pq = idx >> logF;
-    const int pair = (int)(pq % npairs), qi = (int)(pq / npairs);
-    const int mp = mirror_pos(pos, F1);
+// Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
+__global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
+                                                       int nrp, int npairs, int logF,
+                                                       const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 T[2 * kTile];
+    const int F = 1 << logF, F1 = F >> 12;
+    const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
+    const int pair = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    double2 P1[kRP];
     const int p = 2 * pair;
-    const double2 *Zp = X + ((long long)qi * nrp + p) * F;
-    double2 zk = Zp[pos], zm = Zp[mp];
-    const double2 P1 = c_mul(split_x(zk, zm), split_y(zk, zm));
-    double2 P2 = make_double2(0.0, 0.0);
-    if (p + 1 < nrp) {
-        zk = Zp[F + pos];
-        zm = Zp[F + mp];
-        P2 = c_mul(split_x(zk, zm), split_y(zk, zm));
+    rowpair_load_fft(T, Y + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
+#pragma unroll
+    for (int u = 0; u < kRP; ++u) {
+        const int e = t + u * kRT;
+        const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+        P1[u] = c_mul(split_x(a, b), split_y(a, b));
     }
-    U[((long long)qi * npairs + pair) * F + pos] = pack_ai(P1, P2);
+    double2 P2[kRP];
+    if (p + 1 < nrp) {  // uniform over the workgroup
+        __syncthreads();
+        rowpair_load_fft(T, Y + ((long long)qi * nrp + p + 1) * F, k1, k1m, tw, twShift);
+#pragma unroll
+        for (int u = 0; u < kRP; ++u) {
+            const int e = t + u * kRT;
+            const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+            P2[u] = c_mul(split_x(a, b), split_y(a, b));
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kRP; ++u) P2[u] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kRP; ++u) {
+        const int e = t + u * kRT;
+        T[e] = pack_ai(P1[u], P2[u]);
+        if (k1 != k1m) T[mirror_idx(k1, k1m, e)] = pack_ai(c_conj(P1[u]), c_conj(P2[u]));
+    }
+    rowpair_ifft_store(T, V + ((long long)qi * npairs + pair) * F, k1, k1m, logF, tw, twShift);
 }
 
-// U1[p] = g_p conj(R_p) + i g_p conj(L_p).
-__global__ __launch_bounds__(256) void k_pw_prop(const double2 *__restrict__ X1, const double2 *__restrict__ X3,
-                                                 double2 *__restrict__ U1, int nrp, int npairs, int logF, int nq) {
-    const int F = 1 << logF;
-    const int F1 = F >> 12;
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long total = (long long)nq * nrp * F;
-    if (idx >= total) return;
-    const int pos = (int)(idx & (F - 1));
-    const long long pq = idx >> logF;
-    const int p = (int)(pq % nrp), qi = (int)(pq / nrp);
-    const int mp = mirror_pos(pos, F1);
-    const double2 *Z3 = X3 + ((long long)qi * npairs + (p >> 1)) * F;
-    const double2 z3k = Z3[pos], z3m = Z3[mp];
-    const double2 g = (p & 1) ? split_y(z3k, z3m) : split_x(z3k, z3m);
-    const double2 *Z1 = X1 + ((long long)qi * nrp + p) * F;
-    const double2 zk = Z1[pos], zm = Z1[mp];
-    const double2 Lh = split_x(zk, zm), Rh = split_y(zk, zm);
-    U1[((long long)qi * nrp + p) * F + pos] = pack_ai(c_mulc(g, Rh), c_mulc(g, Lh));
+// Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
+__global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict__ Y1, const double2 *__restrict__ Y3,
+                                                      double2 *__restrict__ V1, int nrp, int npairs, int logF,
+                                                      const double2 *__restrict__ tw, int twShift) {
+    __shared__ double2 T[2 * kTile];
+    const int F = 1 << logF, F1 = F >> 12;
+    const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
+    const int pair = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    double2 G1[kRP], G2[kRP];
+    rowpair_load_fft(T, Y3 + ((long long)qi * npairs + pair) * F, k1, k1m, tw, twShift);
+#pragma unroll
+    for (int u = 0; u < kRP; ++u) {
+        const int e = t + u * kRT;
+        const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+        G1[u] = split_x(a, b);
+        G2[u] = split_y(a, b);
+    }
+#pragma unroll 1
+    for (int which = 0; which < 2; ++which) {
+        const int p = 2 * pair + which;
+        if (p >= nrp) break;  // uniform over the workgroup
+        __syncthreads();
+        rowpair_load_fft(T, Y1 + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
+        double2 Ua[kRP], Ub[kRP];
+#pragma unroll
+        for (int u = 0; u < kRP; ++u) {
+            const int e = t + u * kRT;
+            const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+            const double2 g = which ? G2[u] : G1[u];
+            Ua[u] = c_mulc(g, split_y(a, b));   // g conj(R)
+            Ub[u] = c_mulc(g, split_x(a, b));   // g conj(L)
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kRP; ++u) {
+            const int e = t + u * kRT;
+            T[e] = pack_ai(Ua[u], Ub[u]);
+            if (k1 != k1m) T[mirror_idx(k1, k1m, e)] = pack_ai(c_conj(Ua[u]), c_conj(Ub[u]));
+        }
+        rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, logF, tw, twShift);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -721,14 +791,6 @@ static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq
     return hipGetLastError();
 }
 
-static hipError_t row_pass(bool inv, const double2 *in, double2 *out, int F1, long long nArrTotal, int logF,
-                           const double2 *tw, int twShift, hipStream_t s) {
-    dim3 grid(F1, (unsigned)nArrTotal);
-    if (inv) k_row<true><<<grid, kThreads, 0, s>>>(in, out, logF, tw, twShift);
-    else k_row<false><<<grid, kThreads, 0, s>>>(in, out, logF, tw, twShift);
-    return hipGetLastError();
-}
-
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
                               hipStream_t s, Prof *pf) {
     const int F1 = J.F >> 12;
@@ -740,16 +802,9 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
-    PF_END("k_row<false>", 32.0 * nq * nrp * F, nq * nrp * rowFl);
-    const long long tot = (long long)J.nq * npairs * J.F;
-    PF_BEGIN();
-    k_pw_build<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufA, nrp, npairs, J.logF, J.nq);
+    k_rowpair_build<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(bufA, bufB, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
-    PF_END("k_pw_build", 16.0 * nq * (nrp + npairs) * F, 30.0 * nq * nrp * F);
-    PF_BEGIN();
-    ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * npairs, J.logF, tw, twShift, s));
-    PF_END("k_row<true>", 32.0 * nq * npairs * F, nq * npairs * (rowFl + 6.0 * F));
+    PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StParentPair{J}, npairs, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StParentPair>", F1), 16.0 * nq * npairs * F + 8.0 * nq * nrp * J.parent.count, nq * npairs * kF2 * fft_flops(F1));
@@ -772,16 +827,10 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     PF_BEGIN();
-    ICM_TRY(row_pass(false, bufA, bufB, F1, (long long)J.nq * (nrp + npairs), J.logF, tw, twShift, s));
-    PF_END("k_row<false>", 32.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl);
-    const long long tot = (long long)J.nq * nrp * J.F;
-    PF_BEGIN();
-    k_pw_prop<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(bufB, bufB + r1, bufA, nrp, npairs, J.logF, J.nq);
+    k_rowpair_prop<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(bufA, bufA + r1, bufB, nrp, npairs, J.logF, tw,
+                                                                       twShift);
     ICM_TRY(hipGetLastError());
-    PF_END("k_pw_prop", 16.0 * nq * (2 * nrp + npairs) * F, 40.0 * nq * nrp * F);
-    PF_BEGIN();
-    ICM_TRY(row_pass(true, bufA, bufB, F1, (long long)J.nq * nrp, J.logF, tw, twShift, s));
-    PF_END("k_row<true>", 32.0 * nq * nrp * F, nq * nrp * (rowFl + 6.0 * F));
+    PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StGChildren>", F1), 16.0 * nq * nrp * F + 8.0 * nq * J.nr_child * J.gchild.count, nq * nrp * kF2 * fft_flops(F1));
