@@ -178,6 +178,21 @@ def main():
                for k_, v in sorted(kst.items(), key=lambda kv: -kv[1][1])}
     log("kernels (per step): " + json.dumps(kernels, indent=1))
 
+    # subset variant of the same workload: 64 random target players, pruned tree
+    rng = np.random.default_rng(11)
+    targets = rng.choice(n, 64, replace=False)
+    plan.set_targets(targets)
+    plan.run_device(d_p.data_ptr(), d_e.data_ptr(), Q)
+    torch.cuda.synchronize()
+    ts0 = time.perf_counter()
+    for i in range(args.steps):
+        plan.run_device(d_p.data_ptr(), d_e.data_ptr(), Q)
+    torch.cuda.synchronize()
+    sub_ms = (time.perf_counter() - ts0) * 1e3 / args.steps
+    plan.set_targets(None)
+    subset = {"targets": 64, "ms_per_step": sub_ms, "speedup_vs_full": ms / sub_ms}
+    log(f"subset (64 targets): {sub_ms:.2f} ms/step")
+
     cpu = None
     if not args.no_cpu_baseline:
         nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
@@ -199,6 +214,7 @@ def main():
                    "block_size": summ["B"], "q_batch": summ["q_batch"], "parallelism": "single-gpu"},
         "roofline": roof,
         "path_roofline": path_roof,
+        "subset": subset,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

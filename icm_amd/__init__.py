@@ -93,6 +93,7 @@ _SIGS = [
     ("icm_hip_plan_geometry", ctypes.c_int,
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I]),
     ("icm_hip_make_nodes", None, [ctypes.c_int, ctypes.c_double, _D, _D]),
+    ("icm_hip_plan_set_targets", ctypes.c_int, [ctypes.c_void_p, _I, ctypes.c_int]),
     ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
@@ -272,6 +273,19 @@ class Plan:
                                                      ctypes.byref(self.stats)) != 0:
             raise _err(self._lib, "icm_hip_equity_with_plan_device")
         return self.stats.total_ns
+
+    def set_targets(self, targets=None):
+        """Restrict later runs to target players (None: all players)."""
+        if targets is None:
+            rc = self._lib.icm_hip_plan_set_targets(self._h, None, 0)
+        else:
+            t = _to_int_array(targets, "targets")
+            if np.any(t < 0) or np.any(t >= self.n):
+                raise ValueError(f"target indices must be in [0, {self.n-1}]")
+            self._keep_targets = t
+            rc = self._lib.icm_hip_plan_set_targets(self._h, _ptr(t, ctypes.c_int), len(t))
+        if rc != 0:
+            raise _err(self._lib, "icm_hip_plan_set_targets")
 
     def set_profiling(self, on=True):
         """Per-kernel event timing for later runs (clears the aggregates)."""

@@ -1,15 +1,24 @@
 // fft_lds.hpp -- workgroup-cooperative FP64 complex FFTs on an LDS tile.
 //
-// One workgroup of 256 threads (four 64-lane wavefronts) owns a tile of 4096
-// complex doubles (64 KiB of LDS).  The tile holds 4096/FS independent
-// transforms of size FS, each contiguous.  Every Stockham pass gives each
-// thread 16 points: one radix-16 butterfly, or 16/r butterflies of radix r
-// for the one mixed pass when log2(FS) is not a multiple of 4.  Data goes
-// LDS -> registers -> butterfly -> LDS, two barriers per pass.
+// One workgroup of NT threads (256 or 512) owns a tile of 4096 complex
+// doubles in LDS.  The tile holds 4096/FS independent transforms of size FS.
+// Every Stockham pass gives each thread 4096/NT points: butterflies of the
+// main radix (16 for 256 threads, 8 for 512), or of a smaller radix for the
+// one mixed pass when log2(FS) is not a multiple of the main radix's log.
+// Data goes LDS -> registers -> butterfly -> LDS, two barriers per pass.
 //
-// Twiddles come from a table tw[m] = exp(-2 pi i m / 2^twShift) computed on
-// the host from long-double cos/sin (one rounding per entry), so every
-// twiddle is the correctly rounded FP64 value; no on-device sin/cos.
+// LDS layout (TileLayout): element idx of transform it sits at
+// it * IS + idx + idx/16 with the odd item stride IS = FS + FS/16 + 1.  The
+// idx/16 gap spreads the stride-8/16 writes of the Stockham passes over the
+// banks, and the odd IS spreads accesses that step across transforms (the
+// column kernels load one row of many transforms at a time).
+//
+// Twiddles come from host tables computed from long-double cos/sin (one
+// rounding per entry), no on-device sin/cos.  Table layout (see tw_big):
+//   tw[m],        m < 4096:  exp(-2 pi i m / 4096)     (all in-tile passes)
+//   tw[4096 + f], f < FT/4096: exp(-2 pi i f / FT)     (fine part, FT = 2^twShift)
+// In-tile passes index the dense 4096-entry part, so their loads share cache
+// lines; four-step twiddles w_F^m are one coarse entry times one fine entry.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,6 +27,24 @@ namespace icm {
 constexpr int kTile = 4096;   // complex points per workgroup tile
 constexpr int kThreads = 256; // threads per workgroup
 constexpr int kPer = kTile / kThreads;  // 16 points per thread
+
+// w_F^m for F = 2^logF <= FT = 2^twShift and 0 <= m < F: M = m FT / F split
+// into a coarse index (top 12 bits) and a fine index; one complex product.
+__device__ __forceinline__ double2 tw_big(const double2 *tw, int twShift, int m, int logF) {
+    const int M = m << (twShift - logF);
+    const int lf = twShift - 12;
+    const double2 c = tw[M >> lf];
+    if (lf <= 0) return c;
+    const double2 f = tw[kTile + (M & ((1 << lf) - 1))];
+    return make_double2(c.x * f.x - c.y * f.y, c.x * f.y + c.y * f.x);
+}
+
+template <int FS> struct TileLayout {
+    static constexpr int IS = FS + FS / 16 + 1;     // item stride (odd)
+    static constexpr int ITEMS = kTile / FS;
+    static constexpr int SIZE = ITEMS * IS;         // complex elements of LDS
+    __device__ static __forceinline__ int at(int item, int idx) { return item * IS + idx + (idx >> 4); }
+};
 
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -153,25 +180,24 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
         const int bj = t + u * NT;
         item[u] = bj >> LOG_NBI;
         jj[u] = bj & (NBI - 1);
-        const double2 *src = lds + item[u] * FS + jj[u];
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[u][s] = src[s * NBI];
+        for (int s = 0; s < R; ++s) v[u][s] = lds[TileLayout<FS>::at(item[u], jj[u] + s * NBI)];
     }
     __syncthreads();
-    const int twSh = twShift - (logNs + ilog2c(R));
+    const int twSh = 12 - (logNs + ilog2c(R));   // into the 4096-entry part
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int k = jj[u] & (Ns - 1);
         if (logNs > 0) twiddle<R, INV>(v[u], tw, k, twSh);
         dft<R, INV>(v[u]);
-        double2 *dst = lds + item[u] * FS + (jj[u] - k) * R + k;
+        const int d0 = (jj[u] - k) * R + k;
 #pragma unroll
-        for (int j = 0; j < R; ++j) dst[dft_out<R>(j) * Ns] = v[u][j];
+        for (int j = 0; j < R; ++j) lds[TileLayout<FS>::at(item[u], d0 + dft_out<R>(j) * Ns)] = v[u][j];
     }
     __syncthreads();
 }
 
-// In-place transform of every size-FS block in the 4096-point tile by NT
+// In-place transform of every size-FS transform of the tile (TileLayout<FS>) by NT
 // threads (256: radix-16 passes; 512: radix-8 passes), after one mixed pass
 // when log2(FS) is not a multiple of the main radix's log.
 // Starts with a barrier so callers may write the tile and call directly;

@@ -184,6 +184,10 @@ struct IcmHipPlan {
     double *d_acc = nullptr;             // [n] sorted order
     double *d_equity = nullptr;          // [n] original order
     unsigned char *d_active = nullptr;   // [n] sorted order or null
+    // subset pruning (q-independent): hot parents per propagate level and hot
+    // blocks, concatenated in d_lists; offsets/counts per level (index L = blocks)
+    int *d_lists = nullptr;
+    std::vector<int> list_off, list_cnt;
     const double *d_a_in = nullptr;      // optional a values [qb][n]
     bool raw_node = false;               // single node with logv = 0, w = 1 (per-node entry)
     double *d_inner = nullptr;           // optional Psi output [qb][n]
@@ -210,7 +214,7 @@ void plan_free(IcmHipPlan *p) {
     f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_blocks);
     for (double *c : p->d_coef) f(c);
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_partial); f(p->d_acc);
-    f(p->d_equity); f(p->d_active); f(p->d_inner);
+    f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -305,7 +309,7 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
 
     bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
               HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1));
-    ok = ok && dalloc(p, &p->d_tw, (size_t)maxF) && dalloc(p, &p->d_S, n) && dalloc(p, &p->d_perm, n) &&
+    ok = ok && dalloc(p, &p->d_tw, (size_t)(4096 + maxF / 4096)) && dalloc(p, &p->d_S, n) && dalloc(p, &p->d_perm, n) &&
          dalloc(p, &p->d_payout, k) && dalloc(p, &p->d_blocks, (size_t)qb * g.nb * (B + 1)) &&
          dalloc(p, &p->d_g[0], (size_t)qb * ge) && dalloc(p, &p->d_g[1], (size_t)qb * ge) &&
          dalloc(p, &p->d_partial, (size_t)p->nchunk * n) && dalloc(p, &p->d_acc, n) &&
@@ -315,14 +319,21 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
     if (!ok) { plan_free(p); return nullptr; }
 
-    // twiddles exp(-2 pi i m / maxF), rounded once from long double
-    std::vector<double2> tw(maxF);
+    // twiddles, rounded once from long double: 4096-entry coarse table
+    // exp(-2 pi i m / 4096) followed by the maxF/4096-entry fine table
+    // exp(-2 pi i f / maxF) (layout documented in fft_lds.hpp)
+    const int nfine = maxF / 4096;
+    std::vector<double2> tw(4096 + nfine);
     const long double PI = 3.141592653589793238462643383279502884L;
-    for (int m = 0; m < maxF; ++m) {
-        long double ang = -2.0L * PI * (long double)m / (long double)maxF;
+    for (int m = 0; m < 4096; ++m) {
+        long double ang = -2.0L * PI * (long double)m / 4096.0L;
         tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
     }
-    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * maxF, hipMemcpyHostToDevice)) &&
+    for (int f = 0; f < nfine; ++f) {
+        long double ang = -2.0L * PI * (long double)f / (long double)maxF;
+        tw[4096 + f] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
          HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice)) &&
          HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     if (!ok) { plan_free(p); return nullptr; }
@@ -387,6 +398,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.blocks = p->d_blocks;
         lj.n = g.n; lj.nb = g.nb; lj.B = B; lj.k = g.k; lj.nq = nq;
         lj.G0 = g.G[0];
+        lj.blist = nullptr;
+        lj.nbl = g.nb;
         if (!hip_ok(icm::launch_leaf_build(lj, s, &p->prof), "leaf_build")) return -1;
 
         // views of level l's coefficients
@@ -406,6 +419,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
             J.nr_child = g.n_real[l - 1];
             J.nr_parent = g.n_real[l];
+            J.plist = nullptr;
+            J.np = g.n_real[l];
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
@@ -428,6 +443,14 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.gchild = icm::CoefOut{p->d_g[cur], (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
             J.nr_child = g.n_real[l - 1];
             J.nr_parent = g.n_real[l];
+            if (p->d_lists) {
+                J.plist = p->d_lists + p->list_off[l];
+                J.np = p->list_cnt[l];
+            } else {
+                J.plist = nullptr;
+                J.np = g.n_real[l];
+            }
+            if (J.np == 0) continue;
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
@@ -447,7 +470,11 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.partial = p->d_partial;
         lj.inner_out = p->d_inner ? p->d_inner + (long long)q0 * g.n : nullptr;
         lj.active = p->d_active;
-        if (!hip_ok(icm::launch_leaf_extract(lj, nch, s, &p->prof), "leaf_extract")) return -1;
+        if (p->d_lists) {
+            lj.blist = p->d_lists + p->list_off[L];
+            lj.nbl = p->list_cnt[L];
+        }
+        if (lj.nbl > 0 && !hip_ok(icm::launch_leaf_extract(lj, nch, s, &p->prof), "leaf_extract")) return -1;
         if (!hip_ok(icm::launch_finalize(p->d_partial, nch, g.n, p->d_active, p->d_acc, 1, s, &p->prof),
                     "finalize"))
             return -1;
@@ -481,15 +508,51 @@ int equity_host(IcmHipPlan *p, int Q, const double *payout, double *equity, IcmH
     return 0;
 }
 
+// Subset: active mask (sorted order) plus the pruning lists of
+// src/cpu/icm.c:2430-2473 restated for this geometry: block b is hot if it
+// holds a target; a node is hot if either child is; a propagate level l runs
+// only over parents with a hot child, and the leaf extract only over hot
+// blocks.  The build is unchanged (every product feeds its parent).
 int set_targets(IcmHipPlan *p, const int *targets, int n_targets) {
-    const int n = p->geo.n;
+    const Geometry &g = p->geo;
+    const int n = g.n;
     std::vector<int> inv(n);
     for (int i = 0; i < n; ++i) inv[p->perm[i]] = i;
     std::vector<unsigned char> act(n, 0);
     for (int t = 0; t < n_targets; ++t)
         if (targets[t] >= 0 && targets[t] < n) act[inv[targets[t]]] = 1;  // out-of-range ignored (gpu_api.cu:636-641)
+    std::vector<std::vector<unsigned char>> hot(g.L);
+    hot[0].assign(g.nb, 0);
+    for (int i = 0; i < n; ++i)
+        if (act[i]) hot[0][i / g.B] = 1;
+    for (int l = 1; l < g.L; ++l) {
+        hot[l].assign(g.n_real[l], 0);
+        for (int j = 0; j < g.n_real[l]; ++j) {
+            const bool r = 2 * j + 1 < g.n_real[l - 1] && hot[l - 1][2 * j + 1];
+            hot[l][j] = (unsigned char)(hot[l - 1][2 * j] || r);
+        }
+    }
+    std::vector<int> lists;
+    p->list_off.assign(g.L + 1, 0);
+    p->list_cnt.assign(g.L + 1, 0);
+    for (int l = 1; l < g.L; ++l) {
+        p->list_off[l] = (int)lists.size();
+        for (int j = 0; j < g.n_real[l]; ++j)
+            if (hot[l][j]) lists.push_back(j);
+        p->list_cnt[l] = (int)lists.size() - p->list_off[l];
+    }
+    p->list_off[g.L] = (int)lists.size();
+    for (int b = 0; b < g.nb; ++b)
+        if (hot[0][b]) lists.push_back(b);
+    p->list_cnt[g.L] = (int)lists.size() - p->list_off[g.L];
     if (!p->d_active && !dalloc(p, &p->d_active, n)) return -1;
-    return HIP_OK(hipMemcpy(p->d_active, act.data(), n, hipMemcpyHostToDevice)) ? 0 : -1;
+    if (p->d_lists) { (void)hipFree(p->d_lists); p->d_lists = nullptr; }
+    if (!dalloc(p, &p->d_lists, std::max<size_t>(1, lists.size()))) return -1;
+    return (HIP_OK(hipMemcpy(p->d_active, act.data(), n, hipMemcpyHostToDevice)) &&
+            (lists.empty() ||
+             HIP_OK(hipMemcpy(p->d_lists, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice))))
+               ? 0
+               : -1;
 }
 
 }  // namespace
@@ -660,6 +723,16 @@ int icm_hip_plan_geometry(int n, int k, int B, int max_levels, int *n_real, int 
 }
 
 void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w) { make_nodes(Q, Smax, logv, w); }
+
+int icm_hip_plan_set_targets(IcmHipPlan *plan, const int *targets, int n_targets) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (!targets || n_targets <= 0) {   // back to all players
+        if (plan->d_active) { (void)hipFree(plan->d_active); plan->d_active = nullptr; }
+        if (plan->d_lists) { (void)hipFree(plan->d_lists); plan->d_lists = nullptr; }
+        return 0;
+    }
+    return set_targets(plan, targets, n_targets);
+}
 
 int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }

@@ -77,14 +77,15 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     __shared__ double shG[BPW * B];
     const int t = threadIdx.x;
     const int lb = t / B, li = t % B;
-    const int bfirst = blockIdx.x * BPW;
-    const int b = bfirst + lb;
+    const int sfirst = blockIdx.x * BPW;   // first block slot of this workgroup
+    auto block_of = [&](int slot) { return slot < J.nbl ? (J.blist ? J.blist[slot] : slot) : -1; };
+    const int b = block_of(sfirst + lb);
     const int j = b * B + li;
     const int chunk = blockIdx.y;
     const int q0 = chunk * J.q_per_chunk;
     const int q1 = min(q0 + J.q_per_chunk, J.nq);
-    const bool valid = (b < J.nb) && (j < J.n);
-    const int bsize = (b < J.nb) ? min(B, J.n - b * B) : 0;
+    const bool valid = (b >= 0) && (j < J.n);
+    const int bsize = (b >= 0) ? min(B, J.n - b * B) : 0;
     const int pk = min(J.G0, bsize);
     const double Sj = valid ? J.S[j] : 0.0;
     const bool act = valid && (!J.active || J.active[j]);
@@ -92,13 +93,13 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     double acc = 0.0;
     for (int q = q0; q < q1; ++q) {
         for (int e = t; e < BPW * (B + 1); e += 256) {
-            const int bb = bfirst + e / (B + 1);
-            shP[e] = (bb < J.nb) ? J.blocks[q * bplane + (long long)bfirst * (B + 1) + e] : 0.0;
+            const int bb = block_of(sfirst + e / (B + 1));
+            shP[e] = (bb >= 0) ? J.blocks[q * bplane + (long long)bb * (B + 1) + e % (B + 1)] : 0.0;
         }
         for (int e = t; e < BPW * B; e += 256) {
-            const int lbk = e / B, m = e % B;
-            const int bb = bfirst + lbk;
-            shG[e] = (bb < J.nb && m < J.G0)
+            const int m = e % B;
+            const int bb = block_of(sfirst + e / B);
+            shG[e] = (bb >= 0 && m < J.G0)
                          ? J.g[q * J.g_plane + (long long)bb * J.g_stride + m] : 0.0;
         }
         __syncthreads();
@@ -171,11 +172,17 @@ __device__ __forceinline__ double2 pack_ai(double2 A, double2 Bv) {
     return make_double2(A.x - Bv.y, A.y + Bv.x);
 }
 
+// Parent handled by slot i of a propagate launch (-1: none).
+__device__ __forceinline__ int slot_parent(const LevelJob &J, int i) {
+    if (i >= J.np) return -1;
+    return J.plist ? J.plist[i] : i;
+}
+
 // Child pair of parent p packed as L + i R; a missing right child is the
-// identity polynomial.
+// identity polynomial; p < 0 gives zeros.
 __device__ __forceinline__ double2 load_child_pair(const LevelJob &J, const double *cb, int p, int m) {
     double2 z = make_double2(0.0, 0.0);
-    if (p < J.nr_parent) {
+    if (p >= 0 && p < J.nr_parent) {
         const int cs = J.child.stride, cc = J.child.count;
         const long long off = (long long)(2 * p) * cs;
         const bool has_r = (2 * p + 1) < J.nr_child;
@@ -206,9 +213,14 @@ __device__ __forceinline__ double child_prod_at(const LevelJob &J, const double 
 // aliases, and it is replaced by the exact L0 R0; coefficient F (if stored) is
 // the exact product of the leading terms.
 // ---------------------------------------------------------------------------
+// LDS-level kernels: 512 threads, 8 tile points per thread (radix-8 passes).
+constexpr int kLT = 512;
+constexpr int kLP = kTile / kLT;
+
 template <int F>
-__global__ __launch_bounds__(256) void k_build_lds(LevelJob J, const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 lds[kTile];
+__global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
+    using TL = TileLayout<F>;
+    __shared__ double2 lds[TL::SIZE];
     constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
@@ -218,52 +230,48 @@ __global__ __launch_bounds__(256) void k_build_lds(LevelJob J, const double2 *__
     const int pair0 = blockIdx.x * ITEMS;
     const double *cb = J.child.base + (long long)qi * J.child.plane;
 
-    double2 P1[8];
+    double2 P1[kLP / 2];
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int e = t + u * kThreads;
+        for (int u = 0; u < kLP; ++u) {
+            const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[e] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
+            lds[TL::at(item, m)] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
         }
-        fft_tile<F, false>(lds, tw, twShift);
+        fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int ps = t + u * kThreads;
+        for (int u = 0; u < kLP / 2; ++u) {
+            const int ps = t + u * kLT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
-            double2 *Zb = lds + item * F;
+            const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
+            const double2 za = lds[ia], zb = lds[ib];
             double2 Pv;
-            if (jb == 0) {
-                const double2 z0 = Zb[0], zh = Zb[HALF];
-                Pv = make_double2(z0.x * z0.y, zh.x * zh.y);
-            } else {
-                const double2 zj = Zb[jb], zm = Zb[F - jb];
-                Pv = c_mul(split_x(zj, zm), split_y(zj, zm));
-            }
+            if (jb == 0) Pv = make_double2(za.x * za.y, zb.x * zb.y);
+            else Pv = c_mul(split_x(za, zb), split_y(za, zb));
             if (which == 0) {
                 P1[u] = Pv;
             } else if (jb == 0) {
-                Zb[0] = make_double2(P1[u].x, Pv.x);
-                Zb[HALF] = make_double2(P1[u].y, Pv.y);
+                lds[ia] = make_double2(P1[u].x, Pv.x);
+                lds[ib] = make_double2(P1[u].y, Pv.y);
             } else {
-                Zb[jb] = pack_ai(P1[u], Pv);
-                Zb[F - jb] = pack_ai(c_conj(P1[u]), c_conj(Pv));
+                lds[ia] = pack_ai(P1[u], Pv);
+                lds[ib] = pack_ai(c_conj(P1[u]), c_conj(Pv));
             }
         }
         if (which == 0) __syncthreads();
     }
-    fft_tile<F, true>(lds, tw, twShift);
+    fft_tile<F, true, kLT>(lds, tw, twShift);
     const double invF = 1.0 / (double)F;
     double *pb = J.parent.base + (long long)qi * J.parent.plane;
     const int ps_ = J.parent.stride, pc = J.parent.count;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
         const int p = 2 * (pair0 + item);
         if (m < pc) {
-            const double2 v = lds[e];
+            const double2 v = lds[TL::at(item, m)];
             if (p < J.nr_parent)
                 pb[(long long)p * ps_ + m] = (m == 0) ? child_prod_at(J, cb, p, 0) : v.x * invF;
             if (p + 1 < J.nr_parent)
@@ -286,8 +294,9 @@ __global__ __launch_bounds__(256) void k_build_lds(LevelJob J, const double2 *__
 // the inverse (Re = gL, Im = gR).
 // ---------------------------------------------------------------------------
 template <int F>
-__global__ __launch_bounds__(256) void k_prop_lds(LevelJob J, const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 lds[kTile];
+__global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
+    using TL = TileLayout<F>;
+    __shared__ double2 lds[TL::SIZE];
     constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
@@ -300,32 +309,30 @@ __global__ __launch_bounds__(256) void k_prop_lds(LevelJob J, const double2 *__r
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
 
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = 2 * (pair0 + item);
+        const int pa = slot_parent(J, 2 * (pair0 + item)), pb = slot_parent(J, 2 * (pair0 + item) + 1);
         double2 z = make_double2(0.0, 0.0);
         if (m < gcnt) {
-            if (p < J.nr_parent) z.x = gb[(long long)p * gs + m];
-            if (p + 1 < J.nr_parent) z.y = gb[(long long)(p + 1) * gs + m];
+            if (pa >= 0) z.x = gb[(long long)pa * gs + m];
+            if (pb >= 0) z.y = gb[(long long)pb * gs + m];
         }
-        lds[e] = z;
+        lds[TL::at(item, m)] = z;
     }
-    fft_tile<F, false>(lds, tw, twShift);
-    double2 G1[8], G2[8];
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+    double2 G1[kLP / 2], G2[kLP / 2];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int ps = t + u * kThreads;
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
-        const double2 *Zb = lds + item * F;
+        const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) {
-            const double2 z0 = Zb[0], zh = Zb[HALF];
-            G1[u] = make_double2(z0.x, zh.x);
-            G2[u] = make_double2(z0.y, zh.y);
+            G1[u] = make_double2(za.x, zb.x);
+            G2[u] = make_double2(za.y, zb.y);
         } else {
-            const double2 zj = Zb[jb], zm = Zb[F - jb];
-            G1[u] = split_x(zj, zm);
-            G2[u] = split_y(zj, zm);
+            G1[u] = split_x(za, zb);
+            G2[u] = split_y(za, zb);
         }
     }
     __syncthreads();
@@ -335,38 +342,37 @@ __global__ __launch_bounds__(256) void k_prop_lds(LevelJob J, const double2 *__r
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int e = t + u * kThreads;
+        for (int u = 0; u < kLP; ++u) {
+            const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[e] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
+            lds[TL::at(item, m)] = load_child_pair(J, cb, slot_parent(J, 2 * (pair0 + item) + which), m);
         }
-        fft_tile<F, false>(lds, tw, twShift);
+        fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int ps = t + u * kThreads;
+        for (int u = 0; u < kLP / 2; ++u) {
+            const int ps = t + u * kLT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
-            double2 *Zb = lds + item * F;
             const double2 g = which == 0 ? G1[u] : G2[u];
+            const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
+            const double2 za = lds[ia], zb = lds[ib];
             if (jb == 0) {
-                const double2 z0 = Zb[0], zh = Zb[HALF];
-                Zb[0] = make_double2(g.x * z0.y, g.x * z0.x);
-                Zb[HALF] = make_double2(g.y * zh.y, g.y * zh.x);
+                lds[ia] = make_double2(g.x * za.y, g.x * za.x);
+                lds[ib] = make_double2(g.y * zb.y, g.y * zb.x);
             } else {
-                const double2 zj = Zb[jb], zm = Zb[F - jb];
-                const double2 Lh = split_x(zj, zm), Rh = split_y(zj, zm);
+                const double2 Lh = split_x(za, zb), Rh = split_y(za, zb);
                 const double2 A = c_mulc(g, Rh), Bv = c_mulc(g, Lh);
-                Zb[jb] = pack_ai(A, Bv);
-                Zb[F - jb] = pack_ai(c_conj(A), c_conj(Bv));
+                lds[ia] = pack_ai(A, Bv);
+                lds[ib] = pack_ai(c_conj(A), c_conj(Bv));
             }
         }
-        fft_tile<F, true>(lds, tw, twShift);
+        fft_tile<F, true, kLT>(lds, tw, twShift);
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const int e = t + u * kThreads;
+        for (int u = 0; u < kLP; ++u) {
+            const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            const int p = 2 * (pair0 + item) + which;
-            if (p < J.nr_parent && m < gcc) {
-                const double2 v = lds[e];
+            const int p = slot_parent(J, 2 * (pair0 + item) + which);
+            if (p >= 0 && m < gcc) {
+                const double2 v = lds[TL::at(item, m)];
                 gcb[(long long)(2 * p) * gcs + m] = v.x * invF;
                 if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * gcs + m] = v.y * invF;
             }
@@ -385,22 +391,22 @@ __global__ __launch_bounds__(256) void k_prop_lds(LevelJob J, const double2 *__r
 // ---------------------------------------------------------------------------
 constexpr int kF2 = kTile;
 
-struct LdChildPair {      // array a = parent p
+struct LdChildPair {      // array a = slot (parent slot_parent(a))
     LevelJob J;
     __device__ double2 operator()(int a, int qi, int n) const {
         const double *cb = J.child.base + (long long)qi * J.child.plane;
-        return load_child_pair(J, cb, a, n);
+        return load_child_pair(J, cb, slot_parent(J, a), n);
     }
 };
-struct LdGPair {          // array a = pair of parents
+struct LdGPair {          // array a = pair of slots
     LevelJob J;
     __device__ double2 operator()(int a, int qi, int n) const {
         const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
-        const int p = 2 * a;
+        const int pa = slot_parent(J, 2 * a), pb = slot_parent(J, 2 * a + 1);
         double2 z = make_double2(0.0, 0.0);
         if (n < J.gpar.count) {
-            z.x = gb[(long long)p * J.gpar.stride + n];
-            if (p + 1 < J.nr_parent) z.y = gb[(long long)(p + 1) * J.gpar.stride + n];
+            if (pa >= 0) z.x = gb[(long long)pa * J.gpar.stride + n];
+            if (pb >= 0) z.y = gb[(long long)pb * J.gpar.stride + n];
         }
         return z;
     }
@@ -426,13 +432,14 @@ struct StParentPair {     // array a = pair of parents (build)
         }
     }
 };
-struct StGChildren {      // array a = parent p (propagate)
+struct StGChildren {      // array a = slot (propagate)
     LevelJob J;
     __device__ void operator()(int a, int qi, int n, double2 v) const {
         if (n >= J.gchild.count) return;
         const double invF = 1.0 / (double)J.F;
         double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
-        const int p = a;
+        const int p = slot_parent(J, a);
+        if (p < 0) return;
         gcb[(long long)(2 * p) * J.gchild.stride + n] = v.x * invF;
         if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * J.gchild.stride + n] = v.y * invF;
     }
@@ -440,8 +447,9 @@ struct StGChildren {      // array a = parent p (propagate)
 
 template <int F1, class Loader>
 __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
-                                                 const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 lds[kTile];
+                                                 const double2 *tw, int twShift) {
+    using TL = TileLayout<F1>;
+    __shared__ double2 lds[TL::SIZE];
     constexpr int TC = kTile / F1;
     constexpr int LOGTC = ilog2c(TC);
     const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z;
@@ -452,25 +460,25 @@ __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict_
         const int e = t + u * kThreads;
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n = r * kF2 + tile * TC + c;
-        lds[c * F1 + r] = ld(a, qi, n);
+        lds[TL::at(c, r)] = ld(a, qi, n);
     }
     fft_tile<F1, false>(lds, tw, twShift);
     double2 *o = out + ((long long)qi * nArr + a) * F;
-    const int sh = twShift - logF;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = t + u * kThreads;
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n2 = tile * TC + c;
-        const double2 w = tw[((n2 * r) & (F - 1)) << sh];
-        o[r * kF2 + n2] = c_mul(lds[c * F1 + r], w);
+        const double2 w = tw_big(tw, twShift, (n2 * r) & (F - 1), logF);
+        o[r * kF2 + n2] = c_mul(lds[TL::at(c, r)], w);
     }
 }
 
 template <int F1, class Storer>
 __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in, Storer st, int nArr, int logF,
-                                                 const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 lds[kTile];
+                                                 const double2 *tw, int twShift) {
+    using TL = TileLayout<F1>;
+    __shared__ double2 lds[TL::SIZE];
     constexpr int TC = kTile / F1;
     constexpr int LOGTC = ilog2c(TC);
     const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z;
@@ -481,14 +489,14 @@ __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in,
     for (int u = 0; u < kPer; ++u) {
         const int e = t + u * kThreads;
         const int r = e >> LOGTC, c = e & (TC - 1);
-        lds[c * F1 + r] = src[r * kF2 + tile * TC + c];
+        lds[TL::at(c, r)] = src[r * kF2 + tile * TC + c];
     }
     fft_tile<F1, true>(lds, tw, twShift);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = t + u * kThreads;
         const int r = e >> LOGTC, c = e & (TC - 1);
-        st(a, qi, r * kF2 + tile * TC + c, lds[c * F1 + r]);
+        st(a, qi, r * kF2 + tile * TC + c, lds[TL::at(c, r)]);
     }
 }
 
@@ -511,8 +519,11 @@ __device__ __forceinline__ int mirror_k2(int k1, int k2) {
 constexpr int kRT = 512;
 constexpr int kRP = kTile / kRT;
 
+using RTL = TileLayout<kF2>;   // one row per tile, T1 starts at RTL::IS
+
+__device__ __forceinline__ int row_at(int k2) { return RTL::at(0, k2); }
 __device__ __forceinline__ int mirror_idx(int k1, int k1m, int k2) {
-    return (k1 == k1m) ? mirror_k2(k1, k2) : kTile + (kF2 - 1 - k2);
+    return (k1 == k1m) ? row_at(mirror_k2(k1, k2)) : RTL::IS + row_at(kF2 - 1 - k2);
 }
 
 // Load rows k1 / k1m of one [k1][n2] array into the tiles and transform them.
@@ -520,13 +531,13 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
                                                  const double2 *tw, int twShift) {
     const int t = threadIdx.x;
 #pragma unroll
-    for (int u = 0; u < kRP; ++u) T[t + u * kRT] = Zb[(long long)k1 * kF2 + t + u * kRT];
+    for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = Zb[(long long)k1 * kF2 + t + u * kRT];
     if (k1 != k1m) {
 #pragma unroll
-        for (int u = 0; u < kRP; ++u) T[kTile + t + u * kRT] = Zb[(long long)k1m * kF2 + t + u * kRT];
+        for (int u = 0; u < kRP; ++u) T[RTL::IS + row_at(t + u * kRT)] = Zb[(long long)k1m * kF2 + t + u * kRT];
     }
     fft_tile<kF2, false, kRT>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, false, kRT>(T + kTile, tw, twShift);
+    if (k1 != k1m) fft_tile<kF2, false, kRT>(T + RTL::IS, tw, twShift);
 }
 
 // Inverse-transform the tiles in place and store rows k1 / k1m of V with the
@@ -534,19 +545,19 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
 __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m, int logF,
                                                    const double2 *tw, int twShift) {
     const int t = threadIdx.x;
-    const int F = 1 << logF, sh = twShift - logF;
+    const int F = 1 << logF;
     fft_tile<kF2, true, kRT>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, true, kRT>(T + kTile, tw, twShift);
-#pragma unroll
+    if (k1 != k1m) fft_tile<kF2, true, kRT>(T + RTL::IS, tw, twShift);
+#pragma unroll 2
     for (int u = 0; u < kRP; ++u) {
         const int e = t + u * kRT;
-        Vb[(long long)k1 * kF2 + e] = c_mulc(T[e], tw[((e * k1) & (F - 1)) << sh]);
+        Vb[(long long)k1 * kF2 + e] = c_mulc(T[row_at(e)], tw_big(tw, twShift, (e * k1) & (F - 1), logF));
     }
     if (k1 != k1m) {
-#pragma unroll
+#pragma unroll 2
         for (int u = 0; u < kRP; ++u) {
             const int e = t + u * kRT;
-            Vb[(long long)k1m * kF2 + e] = c_mulc(T[kTile + e], tw[((e * k1m) & (F - 1)) << sh]);
+            Vb[(long long)k1m * kF2 + e] = c_mulc(T[RTL::IS + row_at(e)], tw_big(tw, twShift, (e * k1m) & (F - 1), logF));
         }
     }
     __syncthreads();
@@ -555,8 +566,8 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
 __global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
                                                        int nrp, int npairs, int logF,
-                                                       const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 T[2 * kTile];
+                                                       const double2 *tw, int twShift) {
+    __shared__ double2 T[2 * RTL::IS];
     const int F = 1 << logF, F1 = F >> 12;
     const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
     const int pair = blockIdx.y, qi = blockIdx.z;
@@ -567,7 +578,7 @@ __global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict
 #pragma unroll
     for (int u = 0; u < kRP; ++u) {
         const int e = t + u * kRT;
-        const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+        const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
         P1[u] = c_mul(split_x(a, b), split_y(a, b));
     }
     double2 P2[kRP];
@@ -577,18 +588,18 @@ __global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict
 #pragma unroll
         for (int u = 0; u < kRP; ++u) {
             const int e = t + u * kRT;
-            const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+            const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
             P2[u] = c_mul(split_x(a, b), split_y(a, b));
         }
     } else {
 #pragma unroll
         for (int u = 0; u < kRP; ++u) P2[u] = make_double2(0.0, 0.0);
     }
-    __syncthreads();
+    __syncthreads();   // self-paired rows read mirrors written below
 #pragma unroll
     for (int u = 0; u < kRP; ++u) {
         const int e = t + u * kRT;
-        T[e] = pack_ai(P1[u], P2[u]);
+        T[row_at(e)] = pack_ai(P1[u], P2[u]);
         if (k1 != k1m) T[mirror_idx(k1, k1m, e)] = pack_ai(c_conj(P1[u]), c_conj(P2[u]));
     }
     rowpair_ifft_store(T, V + ((long long)qi * npairs + pair) * F, k1, k1m, logF, tw, twShift);
@@ -597,8 +608,8 @@ __global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict
 // Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
 __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict__ Y1, const double2 *__restrict__ Y3,
                                                       double2 *__restrict__ V1, int nrp, int npairs, int logF,
-                                                      const double2 *__restrict__ tw, int twShift) {
-    __shared__ double2 T[2 * kTile];
+                                                      const double2 *tw, int twShift) {
+    __shared__ double2 T[2 * RTL::IS];
     const int F = 1 << logF, F1 = F >> 12;
     const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
     const int pair = blockIdx.y, qi = blockIdx.z;
@@ -608,7 +619,7 @@ __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict_
 #pragma unroll
     for (int u = 0; u < kRP; ++u) {
         const int e = t + u * kRT;
-        const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
+        const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
         G1[u] = split_x(a, b);
         G2[u] = split_y(a, b);
     }
@@ -618,21 +629,30 @@ __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict_
         if (p >= nrp) break;  // uniform over the workgroup
         __syncthreads();
         rowpair_load_fft(T, Y1 + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
-        double2 Ua[kRP], Ub[kRP];
+        const double2 *Gw = which ? G2 : G1;
+        if (k1 != k1m) {
+            // the thread owns both positions it reads: update in place
 #pragma unroll
-        for (int u = 0; u < kRP; ++u) {
-            const int e = t + u * kRT;
-            const double2 a = T[e], b = T[mirror_idx(k1, k1m, e)];
-            const double2 g = which ? G2[u] : G1[u];
-            Ua[u] = c_mulc(g, split_y(a, b));   // g conj(R)
-            Ub[u] = c_mulc(g, split_x(a, b));   // g conj(L)
-        }
-        __syncthreads();
+            for (int u = 0; u < kRP; ++u) {
+                const int e = t + u * kRT;
+                const int ia = row_at(e), ib = mirror_idx(k1, k1m, e);
+                const double2 a = T[ia], b = T[ib];
+                const double2 Ua = c_mulc(Gw[u], split_y(a, b));   // g conj(R)
+                const double2 Ub = c_mulc(Gw[u], split_x(a, b));   // g conj(L)
+                T[ia] = pack_ai(Ua, Ub);
+                T[ib] = pack_ai(c_conj(Ua), c_conj(Ub));
+            }
+        } else {
+            double2 U[kRP];
 #pragma unroll
-        for (int u = 0; u < kRP; ++u) {
-            const int e = t + u * kRT;
-            T[e] = pack_ai(Ua[u], Ub[u]);
-            if (k1 != k1m) T[mirror_idx(k1, k1m, e)] = pack_ai(c_conj(Ua[u]), c_conj(Ub[u]));
+            for (int u = 0; u < kRP; ++u) {
+                const int e = t + u * kRT;
+                const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
+                U[u] = pack_ai(c_mulc(Gw[u], split_y(a, b)), c_mulc(Gw[u], split_x(a, b)));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = U[u];
         }
         rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, logF, tw, twShift);
     }
@@ -675,7 +695,7 @@ hipError_t launch_leaf_build(const LeafJob &J, hipStream_t s, Prof *pf) {
 
 hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof *pf) {
     const int bpw = 256 / J.B;
-    dim3 grid((J.nb + bpw - 1) / bpw, nchunk);
+    dim3 grid((J.nbl + bpw - 1) / bpw, nchunk);
     PF_BEGIN();
     switch (J.B) {
     case 16: k_leaf_extract<16><<<grid, 256, 0, s>>>(J); break;
@@ -684,8 +704,8 @@ hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof
     default: return hipErrorInvalidValue;
     }
     const double nq = J.nq, B = J.B;
-    PF_END(kname("k_leaf_extract<%d>", J.B), 8.0 * nq * J.nb * (B + 1 + J.G0) + 8.0 * J.n + 8.0 * nchunk * J.n,
-           nq * J.n * (4.0 * B + 30.0));
+    PF_END(kname("k_leaf_extract<%d>", J.B), 8.0 * nq * J.nbl * (B + 1 + J.G0) + 8.0 * J.n + 8.0 * nchunk * J.n,
+           nq * (double)J.nbl * B * (4.0 * B + 30.0));
     return hipGetLastError();
 }
 
@@ -709,15 +729,15 @@ static hipError_t build_lds_f(const LevelJob &J, const double2 *tw, int twShift,
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.nr_parent + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_build_lds<F><<<grid, kThreads, 0, s>>>(J, tw, twShift);
+    k_build_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 template <int F>
 static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
     constexpr int ITEMS = kTile / F;
-    const int npairs = (J.nr_parent + 1) / 2;
+    const int npairs = (J.np + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_prop_lds<F><<<grid, kThreads, 0, s>>>(J, tw, twShift);
+    k_prop_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
@@ -758,10 +778,11 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
     case 4096: e = prop_lds_f<4096>(J, tw, twShift, s); break;
     default: return hipErrorInvalidValue;
     }
-    const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)J.nr_parent * J.gpar.count;
-    const double bytes = gin + level_coef_bytes(J.child, J.nr_child, J.nq) +
-                         8.0 * J.nq * (double)J.nr_child * J.gchild.count;
-    PF_END(kname("k_prop_lds<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 2.5 * fft_flops(J.F));
+    const double frac = (double)J.np / (double)J.nr_parent;   // < 1 with subset pruning
+    const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)J.np * J.gpar.count;
+    const double bytes = gin + frac * (level_coef_bytes(J.child, J.nr_child, J.nq) +
+                                       8.0 * J.nq * (double)J.nr_child * J.gchild.count);
+    PF_END(kname("k_prop_lds<%d>", J.F), bytes, J.nq * (double)J.np * 2.5 * fft_flops(J.F));
     return e;
 }
 
@@ -814,7 +835,7 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
                              hipStream_t s, Prof *pf) {
     const int F1 = J.F >> 12;
-    const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
+    const int nrp = J.np, npairs = (nrp + 1) / 2;   // slots (all parents without pruning)
     const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);

@@ -32,6 +32,11 @@ struct LevelJob {
     int F;              // transform size of this level
     int logF;
     int nq;             // q-planes in the launch
+    // Propagate launches may run over a list of parents (subset pruning):
+    // slot i handles parent plist[i] (i < np); without a list, parent i
+    // (np = nr_parent).  Build launches always use the full level.
+    const int *plist;
+    int np;
 };
 
 struct LeafJob {
@@ -50,6 +55,8 @@ struct LeafJob {
     double *inner_out;      // optional [nq][n] Psi values (sorted order)
     int q_per_chunk;
     const unsigned char *active;  // optional [n] (sorted order)
+    const int *blist;       // optional list of blocks to extract (subset pruning)
+    int nbl;                // blocks in the launch (= nb without a list)
 };
 
 // Optional per-launch timing: when on, every kernel launch is bracketed by

@@ -97,6 +97,14 @@ int icm_hip_equity_subset(int n, const double *S, int Q, const double *payout, i
 int icm_hip_equity_with_plan_device(IcmHipPlan *plan, int Q, const double *d_payout,
                                     double *d_equity, IcmHipRunStats *stats);
 
+/* Restrict later runs of a plan to target players (0-based, original order;
+ * out-of-range entries ignored): non-targets come out 0 and the propagate /
+ * leaf kernels run only over subtrees holding a target (the pruning of
+ * src/cpu/icm.c:2430-2521).  targets == NULL or n_targets <= 0 restores all
+ * players.  No reference counterpart (the reference's subset call builds a
+ * fresh plan and masks only the outputs, src/gpu/gpu_api.cu:620-672). */
+int icm_hip_plan_set_targets(IcmHipPlan *plan, const int *targets, int n_targets);
+
 /* One quadrature node through the GPU tree: inner[i] = Psi_i for the given
  * a[i] (players in the given order), mirroring the IcmEngine hook
  * (src/cpu/icm.h:45-47, engine_hybrid_ctx src/cpu/icm.c:2307-2318). */

@@ -180,6 +180,31 @@ def test_subset_matches_full():
     assert np.all(sub[mask] == 0.0)
 
 
+@pytest.mark.parametrize("n,k,nt", [(65536, 16384, 64), (20000, 20000, 7), (4096, 100, 1)])
+def test_subset_pruned_large(n, k, nt):
+    """Pruned propagate through LDS and four-step levels vs the full pass and the oracle."""
+    rng = np.random.default_rng(n + k + nt)
+    S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+    p = np.arange(k, 0, -1, dtype=float)
+    p /= p.sum()
+    targets = rng.choice(n, nt, replace=False)
+    say(f"subset n={n} k={k} targets={nt}")
+    plan = icm_amd.Plan(S, k)
+    full = plan.run(p, Q=256)
+    plan.set_targets(targets)
+    sub = plan.run(p, Q=256)
+    plan.set_targets(None)
+    again = plan.run(p, Q=256)
+    plan.close()
+    np.testing.assert_allclose(sub[targets], full[targets], rtol=1e-10)
+    mask = np.ones(n, bool)
+    mask[targets] = False
+    assert np.all(sub[mask] == 0.0)
+    assert np.array_equal(again, full)
+    cpu = O.equity(S, p, Q=256, targets=targets, fft_min=64)
+    assert rel_err(sub[targets], cpu[targets]) < 1e-8
+
+
 def test_reference_python_semantics():
     """python/tests/test_icm.py behaviours through the drop-in."""
     eq = icm_amd.equity([1000, 1000, 1000], [0.5, 0.3, 0.2])
