@@ -193,6 +193,39 @@ def main():
     subset = {"targets": 64, "ms_per_step": sub_ms, "speedup_vs_full": ms / sub_ms}
     log(f"subset (64 targets): {sub_ms:.2f} ms/step")
 
+    # forward + adjoint: one pass returns E and dS = d(v.E)/dS; ratio to the forward
+    def fwd_vjp_ratio(Sx, px, reps):
+        pl = icm_amd.Plan(Sx, len(px))
+        dp = torch.tensor(px, dtype=torch.float64, device=dev)
+        de = torch.empty(len(Sx), dtype=torch.float64, device=dev)
+        dd = torch.empty(len(Sx), dtype=torch.float64, device=dev)
+        dv = torch.tensor(np.random.default_rng(1).normal(size=len(Sx)), dtype=torch.float64, device=dev)
+        pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
+        pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        for _ in range(reps):
+            pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter()
+        for _ in range(reps):
+            pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
+        torch.cuda.synchronize()
+        t_c = time.perf_counter()
+        pl.close()
+        fwd_ms = (t_b - t_a) * 1e3 / reps
+        vjp_ms = (t_c - t_b) * 1e3 / reps
+        return {"fwd_ms": fwd_ms, "fwd_plus_vjp_ms": vjp_ms, "ratio": vjp_ms / fwd_ms}
+
+    r5 = np.random.default_rng(5)
+    S5 = r5.lognormal(0.0, 1.5, 131072)
+    p5 = np.arange(1000, 0, -1, dtype=np.float64)
+    p5 /= p5.sum()
+    adjoint = {"n131072_k1000": fwd_vjp_ratio(S5, p5, 3)}
+    plan.close()
+    adjoint["n1048576_k1048576"] = fwd_vjp_ratio(S, p, 2)
+    log(f"adjoint: {adjoint}")
+
     cpu = None
     if not args.no_cpu_baseline:
         nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
@@ -215,10 +248,10 @@ def main():
         "roofline": roof,
         "path_roofline": path_roof,
         "subset": subset,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    plan.close()
 
 
 if __name__ == "__main__":

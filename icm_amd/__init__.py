@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-__all__ = ["ICMError", "init", "equity", "equity_subset", "node_inner", "Plan",
+__all__ = ["ICMError", "init", "equity", "equity_subset", "equity_vjp", "node_inner", "node_vjp", "Plan",
            "plan_geometry", "make_nodes", "library_path", "load_library"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -94,6 +94,13 @@ _SIGS = [
      [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I, _I]),
     ("icm_hip_make_nodes", None, [ctypes.c_int, ctypes.c_double, _D, _D]),
     ("icm_hip_plan_set_targets", ctypes.c_int, [ctypes.c_void_p, _I, ctypes.c_int]),
+    ("icm_hip_vjp_with_plan", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(RunStats)]),
+    ("icm_hip_vjp_with_plan_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.POINTER(RunStats)]),
+    ("icm_hip_equity_vjp", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]),
+    ("icm_hip_node_vjp", ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D, _D]),
     ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
@@ -229,6 +236,40 @@ def node_inner(a, payouts):
     return out
 
 
+def equity_vjp(stacks, payouts, v, Q=256):
+    """(equity, dS) with dS = d(v . equity)/dS, quadrature nodes held fixed."""
+    _auto_init()
+    s = _to_double_array(stacks, "stacks")
+    p = _to_double_array(payouts, "payouts")
+    vv = _to_double_array(v, "v")
+    n, k = len(s), len(p)
+    _validate(n, k, Q)
+    if len(vv) != n:
+        raise ValueError(f"cotangent length {len(vv)} != n={n}")
+    eq = np.zeros(n)
+    ds = np.zeros(n)
+    lib = load_library()
+    if lib.icm_hip_equity_vjp(n, _ptr(s), int(Q), _ptr(p), k, _ptr(vv), _ptr(eq), _ptr(ds)) != 0:
+        raise _err(lib, "icm_hip_equity_vjp")
+    return eq, ds
+
+
+def node_vjp(a, c, payouts):
+    """One node: (Psi_i, H_i = sum_j c_j dPsi_j/da_i) for given a, c (given order)."""
+    _auto_init()
+    a = _to_double_array(a, "a")
+    c = _to_double_array(c, "c")
+    p = _to_double_array(payouts, "payouts")
+    n, k = len(a), len(p)
+    _validate(n, k, 1)
+    psi = np.zeros(n)
+    h = np.zeros(n)
+    lib = load_library()
+    if lib.icm_hip_node_vjp(n, _ptr(a), _ptr(c), _ptr(p), k, _ptr(psi), _ptr(h)) != 0:
+        raise _err(lib, "icm_hip_node_vjp")
+    return psi, h
+
+
 class Plan:
     """Reusable plan (IcmGpuPlan, src/gpu/icm_gpu.h:79-111): sort, geometry and
     device memory are set up once for (stacks, k); ``run`` takes Q and payout."""
@@ -272,6 +313,28 @@ class Plan:
                                                      ctypes.c_void_p(d_equity_ptr),
                                                      ctypes.byref(self.stats)) != 0:
             raise _err(self._lib, "icm_hip_equity_with_plan_device")
+        return self.stats.total_ns
+
+    def vjp(self, payouts, v, Q=256):
+        """(equity, dS) for cotangent v (host arrays)."""
+        p = _to_double_array(payouts, "payouts")
+        vv = _to_double_array(v, "v")
+        if len(p) != self.k or len(vv) != self.n:
+            raise ValueError("payouts / cotangent length mismatch")
+        eq = np.zeros(self.n)
+        ds = np.zeros(self.n)
+        if self._lib.icm_hip_vjp_with_plan(self._h, int(Q), _ptr(p), _ptr(vv), _ptr(eq), _ptr(ds),
+                                           ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_vjp_with_plan")
+        return eq, ds
+
+    def vjp_device(self, d_payout_ptr, d_v_ptr, d_equity_ptr, d_ds_ptr, Q=256):
+        """Device-resident adjoint pass (raw device pointers; outputs may be 0)."""
+        if self._lib.icm_hip_vjp_with_plan_device(
+                self._h, int(Q), ctypes.c_void_p(d_payout_ptr), ctypes.c_void_p(d_v_ptr),
+                ctypes.c_void_p(d_equity_ptr or None), ctypes.c_void_p(d_ds_ptr or None),
+                ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_vjp_with_plan_device")
         return self.stats.total_ns
 
     def set_targets(self, targets=None):

@@ -191,6 +191,18 @@ struct IcmHipPlan {
     const double *d_a_in = nullptr;      // optional a values [qb][n]
     bool raw_node = false;               // single node with logv = 0, w = 1 (per-node entry)
     double *d_inner = nullptr;           // optional Psi output [qb][n]
+    // adjoint (VJP) buffers, allocated on the first VJP call
+    int qv = 0;                          // q-planes per VJP batch
+    double *d_dblocks = nullptr;         // [qv][nb][B+1] tangent block products
+    std::vector<double *> d_dcoef;       // level l: [qv][n_real][c] tangent products
+    double *d_bv[2] = {nullptr, nullptr};// B adjoint ping-pong, [qv][n_real][c]
+    long long bv_elems = 0;
+    double *d_v = nullptr;               // cotangent, sorted order
+    double *d_partial_ds = nullptr;      // [nchunk][n]
+    double *d_acc_ds = nullptr;          // [n] sorted
+    double *d_ds = nullptr;              // [n] original order
+    const double *d_c_in = nullptr;      // optional tangent weights (per-node entry)
+    double *d_h = nullptr;               // optional H output (per-node entry)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t bytes = 0;
@@ -215,6 +227,8 @@ void plan_free(IcmHipPlan *p) {
     for (double *c : p->d_coef) f(c);
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_partial); f(p->d_acc);
     f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
+    f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -497,6 +511,192 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
     return 0;
 }
 
+// VJP buffers: tangent block/level products, B adjoint ping-pong, dS arrays.
+bool ensure_vjp(IcmHipPlan *p) {
+    if (p->qv > 0) return true;
+    const Geometry &g = p->geo;
+    long long per = (long long)g.nb * (g.B + 1) * 8;
+    long long be = 0;
+    for (int l = 1; l <= g.L - 2; ++l) per += (long long)g.n_real[l] * g.c[l] * 8;
+    for (int l = 0; l < g.L; ++l) be = std::max(be, (long long)g.n_real[l] * g.c[l]);
+    per += 2 * be * 8;
+    size_t freeb = 0, totb = 0;
+    if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) return false;
+    int qv = (int)std::max<long long>(1, (long long)(0.6 * (double)freeb) / std::max<long long>(per, 1));
+    qv = std::min(qv, p->qb);
+    p->bv_elems = be;
+    bool ok = dalloc(p, &p->d_dblocks, (size_t)qv * g.nb * (g.B + 1)) && dalloc(p, &p->d_bv[0], (size_t)qv * be) &&
+              dalloc(p, &p->d_bv[1], (size_t)qv * be) && dalloc(p, &p->d_v, g.n) &&
+              dalloc(p, &p->d_partial_ds, (size_t)p->nchunk * g.n) && dalloc(p, &p->d_acc_ds, g.n) &&
+              dalloc(p, &p->d_ds, g.n);
+    p->d_dcoef.assign(g.L, nullptr);
+    for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * g.n_real[l] * g.c[l]);
+    if (!ok) return false;
+    p->qv = qv;
+    return true;
+}
+
+// Adjoint pass: equity and dS = d(v.E)/dS (quadrature nodes held fixed).
+// d_v: cotangent in original order (device).  Leaves d_equity and d_ds.
+int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig, IcmHipRunStats *stats) {
+    const Geometry &g = p->geo;
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!ensure_vjp(p)) return -1;
+    std::vector<double> logv(Q), w(Q);
+    make_nodes(Q, p->Smax, logv.data(), w.data());
+    if (p->raw_node) { logv.assign(1, 0.0); w.assign(1, 1.0); Q = 1; }
+    std::vector<double> lv_a, w_a;
+    for (int q = 0; q < Q; ++q)
+        if (w[q] != 0.0) { lv_a.push_back(logv[q]); w_a.push_back(w[q]); }
+    const int Qa = (int)lv_a.size();
+    const int Qs = std::max(Qa, 1);
+    if (!ensure_nodes(p, Qs)) return -1;
+    std::vector<double> nodes(2 * (size_t)Qs, 0.0);
+    std::copy(lv_a.begin(), lv_a.end(), nodes.begin());
+    std::copy(w_a.begin(), w_a.end(), nodes.begin() + Qs);
+    hipStream_t s = p->stream;
+    if (!HIP_OK(hipMemcpyAsync(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice, s)))
+        return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    const double *d_lv = p->d_nodes, *d_w = p->d_nodes + Qs;
+    if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
+    if (d_v_orig) {
+        if (!hip_ok(icm::launch_gather(d_v_orig, p->d_perm, g.n, p->d_v, s), "gather")) return -1;
+    } else if (!HIP_OK(hipMemsetAsync(p->d_v, 0, sizeof(double) * g.n, s))) {
+        return -1;
+    }
+    if (!HIP_OK(hipMemsetAsync(p->d_acc, 0, sizeof(double) * g.n, s))) return -1;
+    if (!HIP_OK(hipMemsetAsync(p->d_acc_ds, 0, sizeof(double) * g.n, s))) return -1;
+    const int B = g.B, L = g.L;
+    auto view = [&](int l, bool tangent) {
+        icm::CoefView v{};
+        if (l == 0) {
+            v.base = tangent ? p->d_dblocks : p->d_blocks;
+            v.plane = (long long)g.nb * (B + 1); v.stride = B + 1; v.count = g.c[0];
+        } else {
+            v.base = tangent ? p->d_dcoef[l] : p->d_coef[l];
+            v.plane = (long long)g.n_real[l] * g.c[l]; v.stride = g.c[l]; v.count = g.c[l];
+        }
+        return v;
+    };
+    auto level = [&](icm::LevelJob &J, bool build) -> bool {
+        hipError_t e;
+        if (build)
+            e = (J.F <= icm::kMaxLdsF) ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
+                                       : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s,
+                                                                 &p->prof);
+        else
+            e = (J.F <= icm::kMaxLdsF) ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
+                                       : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s,
+                                                                &p->prof);
+        return hip_ok(e, build ? "vjp build level" : "vjp propagate level");
+    };
+    const int qstep = std::min(p->qb, p->qv);
+    for (int q0 = 0; q0 < Qa; q0 += qstep) {
+        const int nq = std::min(qstep, Qa - q0);
+        icm::VjpLeafJob V{};
+        icm::LeafJob &lj = V.L;
+        lj.S = p->d_S; lj.logv = d_lv + q0; lj.wq = d_w + q0;
+        lj.a_in = p->d_a_in ? p->d_a_in + (long long)q0 * g.n : nullptr;
+        lj.blocks = p->d_blocks;
+        lj.n = g.n; lj.nb = g.nb; lj.B = B; lj.k = g.k; lj.nq = nq; lj.G0 = g.G[0];
+        lj.blist = nullptr; lj.nbl = g.nb;
+        V.v = p->d_v;
+        V.c_in = p->d_c_in ? p->d_c_in + (long long)q0 * g.n : nullptr;
+        V.dblocks = p->d_dblocks;
+        V.c0 = g.c[0];
+        if (!hip_ok(icm::launch_leaf_build_vjp(V, s, &p->prof), "leaf_build_vjp")) return -1;
+        for (int l = 1; l <= L - 2; ++l) {
+            icm::LevelJob J{};
+            J.nr_child = g.n_real[l - 1]; J.nr_parent = g.n_real[l]; J.np = g.n_real[l];
+            J.F = g.F[l]; J.logF = ilog2i(g.F[l]); J.nq = nq;
+            J.child = view(l - 1, false);
+            J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            if (!level(J, true)) return -1;                       // P_X = P_L P_R
+            J.parent.base = p->d_dcoef[l];
+            J.child = view(l - 1, true);
+            J.child_r = view(l - 1, false);
+            if (!level(J, true)) return -1;                       // dP_X = dP_L P_R
+            J.child = view(l - 1, false);
+            J.child_r = view(l - 1, true);
+            J.r_missing_zero = 1;
+            J.acc_out = 1;
+            if (!level(J, true)) return -1;                       //      + P_L dP_R
+        }
+        int ca = 0, cb = 0;  // current A / B buffers (A in d_g, B in d_bv)
+        for (int l = L - 1; l >= 1; --l) {
+            const bool root = (l == L - 1);
+            icm::CoefView Apar = root ? icm::CoefView{d_payout, 0, g.k, g.k}
+                                      : icm::CoefView{p->d_g[ca], (long long)g.n_real[l] * g.G[l], g.G[l], g.G[l]};
+            icm::CoefView Bpar{p->d_bv[cb], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            const int na = root ? 0 : ca ^ 1, nbv = root ? 0 : cb ^ 1;
+            icm::LevelJob J{};
+            J.nr_child = g.n_real[l - 1]; J.nr_parent = g.n_real[l]; J.np = g.n_real[l];
+            J.F = g.F[l]; J.logF = ilog2i(g.F[l]); J.nq = nq;
+            J.child = view(l - 1, false);
+            J.gpar = Apar;
+            J.gchild = icm::CoefOut{p->d_g[na], (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
+            if (!level(J, false)) return -1;                      // A_L = A_X * R
+            const int cc = g.c[l - 1];
+            icm::CoefOut Bout{p->d_bv[nbv], (long long)g.n_real[l - 1] * cc, cc, cc};
+            const int gfix = (g.F[l] < 2 * cc - 1) ? 1 : 0;
+            if (!root) {
+                icm::LevelJob K = J;
+                K.gpar = Bpar;
+                K.gchild = Bout;
+                K.gfix = gfix;
+                if (!level(K, false)) return -1;                  // B_L = B_X * R
+            }
+            icm::LevelJob K = J;
+            K.gpar = Apar;
+            K.child = view(l - 1, true);
+            K.r_missing_zero = 1;
+            K.gchild = Bout;
+            K.gfix = gfix;
+            K.acc_out = root ? 0 : 1;
+            if (!level(K, false)) return -1;                      //     + A_X * dR
+            ca = na;
+            cb = nbv;
+        }
+        if (L == 1) {
+            lj.g = d_payout; lj.g_plane = 0; lj.g_stride = g.k;
+            V.bvec = nullptr;
+        } else {
+            lj.g = p->d_g[ca]; lj.g_plane = (long long)g.n_real[0] * g.G[0]; lj.g_stride = g.G[0];
+            V.bvec = p->d_bv[cb]; V.b_plane = (long long)g.n_real[0] * g.c[0]; V.b_stride = g.c[0];
+        }
+        const int nchunk = std::min(p->nchunk, nq);
+        lj.q_per_chunk = (nq + nchunk - 1) / nchunk;
+        const int nch = (nq + lj.q_per_chunk - 1) / lj.q_per_chunk;
+        lj.partial = p->d_partial;
+        lj.inner_out = p->d_inner ? p->d_inner + (long long)q0 * g.n : nullptr;
+        V.partial_ds = p->d_partial_ds;
+        V.h_out = p->d_h ? p->d_h + (long long)q0 * g.n : nullptr;
+        if (!hip_ok(icm::launch_leaf_vjp(V, nch, s, &p->prof), "leaf_vjp")) return -1;
+        if (!hip_ok(icm::launch_finalize(p->d_partial, nch, g.n, nullptr, p->d_acc, 1, s, &p->prof), "finalize") ||
+            !hip_ok(icm::launch_finalize(p->d_partial_ds, nch, g.n, nullptr, p->d_acc_ds, 1, s, &p->prof),
+                    "finalize"))
+            return -1;
+    }
+    if (!hip_ok(icm::launch_scatter(p->d_acc, g.n, p->d_perm, p->d_equity, s, &p->prof), "scatter") ||
+        !hip_ok(icm::launch_scatter(p->d_acc_ds, g.n, p->d_perm, p->d_ds, s, &p->prof), "scatter"))
+        return -1;
+    if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (p->prof.enabled) p->prof.collect();
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+        memset(stats, 0, sizeof(*stats));
+        stats->total_ns = (double)ms * 1e6;
+        stats->peak_vram_bytes = p->bytes;
+        stats->engine = 1;
+        stats->B = B;
+    }
+    return 0;
+}
+
 int equity_host(IcmHipPlan *p, int Q, const double *payout, double *equity, IcmHipRunStats *stats) {
     if (!p || !payout || !equity) { set_err("null argument"); return -1; }
     if (!HIP_OK(hipSetDevice(g_device))) return -1;
@@ -656,6 +856,73 @@ int icm_hip_node_inner(int n, const double *a, const double *payout, int k, doub
     if (d_a) (void)hipFree(d_a);
     p->d_inner = nullptr;
     if (d_in) (void)hipFree(d_in);
+    plan_free(p);
+    return rc;
+}
+
+int icm_hip_vjp_with_plan_device(IcmHipPlan *plan, int Q, const double *d_payout, const double *d_v,
+                                 double *d_equity, double *d_dS, IcmHipRunStats *stats) {
+    if (!plan || !d_payout || !d_v) { set_err("null argument"); return -1; }
+    if (plan->d_active) { set_err("the adjoint pass needs all players (clear the plan's targets)"); return -1; }
+    if (run_vjp(plan, Q, d_payout, d_v, stats) != 0) return -1;
+    const size_t nb = sizeof(double) * plan->geo.n;
+    if (d_equity && !HIP_OK(hipMemcpy(d_equity, plan->d_equity, nb, hipMemcpyDeviceToDevice))) return -1;
+    if (d_dS && !HIP_OK(hipMemcpy(d_dS, plan->d_ds, nb, hipMemcpyDeviceToDevice))) return -1;
+    return 0;
+}
+
+int icm_hip_vjp_with_plan(IcmHipPlan *plan, int Q, const double *payout, const double *v, double *equity,
+                          double *dS, IcmHipRunStats *stats) {
+    if (!plan || !payout || !v) { set_err("null argument"); return -1; }
+    if (plan->d_active) { set_err("the adjoint pass needs all players (clear the plan's targets)"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!ensure_vjp(plan)) return -1;
+    const size_t nb = sizeof(double) * plan->geo.n;
+    // stage v in the dS output buffer (overwritten by the scatter at the end)
+    if (!HIP_OK(hipMemcpy(plan->d_payout, payout, sizeof(double) * plan->geo.k, hipMemcpyHostToDevice)) ||
+        !HIP_OK(hipMemcpy(plan->d_ds, v, nb, hipMemcpyHostToDevice)))
+        return -1;
+    if (run_vjp(plan, Q, plan->d_payout, plan->d_ds, stats) != 0) return -1;
+    if (equity && !HIP_OK(hipMemcpy(equity, plan->d_equity, nb, hipMemcpyDeviceToHost))) return -1;
+    if (dS && !HIP_OK(hipMemcpy(dS, plan->d_ds, nb, hipMemcpyDeviceToHost))) return -1;
+    return 0;
+}
+
+int icm_hip_equity_vjp(int n, const double *S, int Q, const double *payout, int k, const double *v,
+                       double *equity, double *dS) {
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    IcmHipPlan *p = plan_create(n, S, k, nullptr, true, Q);
+    if (!p) return -1;
+    int rc = icm_hip_vjp_with_plan(p, Q, payout, v, equity, dS, nullptr);
+    plan_free(p);
+    return rc;
+}
+
+int icm_hip_node_vjp(int n, const double *a, const double *c, const double *payout, int k, double *inner,
+                     double *H) {
+    if (!a || !c || !payout || n < 1 || k < 1 || k > n) { set_err("invalid arguments"); return -1; }
+    std::vector<double> ones(n, 1.0);
+    IcmHipPlan *p = plan_create(n, ones.data(), k, nullptr, false, 1);
+    if (!p) return -1;
+    double *d_a = nullptr, *d_c = nullptr, *d_in = nullptr, *d_h = nullptr;
+    int rc = -1;
+    p->qb = 1;
+    if (dalloc(p, &d_a, n) && dalloc(p, &d_c, n) && dalloc(p, &d_in, n) && dalloc(p, &d_h, n) &&
+        HIP_OK(hipMemcpy(d_a, a, sizeof(double) * n, hipMemcpyHostToDevice)) &&
+        HIP_OK(hipMemcpy(d_c, c, sizeof(double) * n, hipMemcpyHostToDevice)) &&
+        HIP_OK(hipMemcpy(p->d_payout, payout, sizeof(double) * k, hipMemcpyHostToDevice))) {
+        p->d_a_in = d_a;
+        p->d_c_in = d_c;
+        p->d_inner = d_in;
+        p->d_h = d_h;
+        p->raw_node = true;
+        if (run_vjp(p, 1, p->d_payout, nullptr, nullptr) == 0 &&
+            (!inner || HIP_OK(hipMemcpy(inner, d_in, sizeof(double) * n, hipMemcpyDeviceToHost))) &&
+            (!H || HIP_OK(hipMemcpy(H, d_h, sizeof(double) * n, hipMemcpyDeviceToHost))))
+            rc = 0;
+    }
+    p->d_inner = nullptr;
+    for (double *x : {d_a, d_c, d_in, d_h}) if (x) (void)hipFree(x);
     plan_free(p);
     return rc;
 }

@@ -139,6 +139,182 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     if (valid && J.partial) J.partial[(long long)chunk * J.n + j] = act ? acc : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint leaves.  With tangent weights c_j = v_j w_q S_j a_j exp(-logv_q)
+// (the coefficient of Psi_j in v.E) every factor f_j = a_j + (1-a_j) x gets
+// the tangent c_j (direction a_j, b_j held), so a block carries (P_b, dP_b)
+// with dP <- dP f + c P, P <- P f.
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(64) void k_leaf_build_vjp(VjpLeafJob V) {
+    const LeafJob &J = V.L;
+    __shared__ double sh[64 * (B + 1)];
+    const int lane = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int b0 = blockIdx.x * 64;
+    const int b = b0 + lane;
+    double P[B + 1], D[B + 1];
+    P[0] = 1.0;
+    D[0] = 0.0;
+#pragma unroll
+    for (int m = 1; m <= B; ++m) { P[m] = 0.0; D[m] = 0.0; }
+    if (b < J.nb) {
+        const double lv = J.logv[qi], wq = J.wq[qi];
+        const double ivv = exp(-lv);
+        const long long arow = (long long)qi * J.n;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int j = b * B + i;
+            double a = 1.0, c = 0.0;
+            if (j < J.n) {
+                a = J.a_in ? J.a_in[arow + j] : a_of(J.S[j], lv);
+                if (V.c_in) {
+                    c = V.c_in[arow + j];
+                } else {
+                    double pw = wq * J.S[j] * a * ivv;
+                    if (!isfinite(pw)) pw = 0.0;
+                    c = V.v[j] * pw;
+                }
+            }
+            const double bb = 1.0 - a;
+#pragma unroll
+            for (int m = i + 1; m >= 1; --m) {
+                D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
+                P[m] = a * P[m] + bb * P[m - 1];
+            }
+            D[0] = a * D[0] + c * P[0];
+            P[0] *= a;
+        }
+    }
+    const int nblk = min(64, J.nb - b0);
+    const long long off = (long long)qi * J.nb * (B + 1) + (long long)b0 * (B + 1);
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
+    __syncthreads();
+    for (int e = lane; e < nblk * (B + 1); e += 64) J.blocks[off + e] = sh[e];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = D[m];
+    __syncthreads();
+    for (int e = lane; e < nblk * (B + 1); e += 64) V.dblocks[off + e] = sh[e];
+}
+
+// Per player j of block b (A = forward g of the block, Bv = adjoint of P_b):
+//   Q = P_b / f_j,  W = (dP_b - c_j Q) / f_j          (same division branch)
+//   Psi = sum_{m<pk} A[m] Q[m]
+//   H   = sum_m Q[m] (Bv[m] - Bv[m+1]) + sum_m W[m] (A[m] - A[m+1])   = dPhi/da_j
+//   dS_j += v_j w exp(-logv) a_j (1 + S_j logv) Psi + H a_j logv
+//   E_j  += w S_j a_j exp(-logv) Psi
+// Q and W stream in the division's direction, so nothing is stored per lane.
+template <int B>
+__global__ __launch_bounds__(256) void k_leaf_vjp(VjpLeafJob V) {
+    const LeafJob &J = V.L;
+    constexpr int BPW = 256 / B;
+    __shared__ double shP[BPW * (B + 1)];
+    __shared__ double shD[BPW * (B + 1)];
+    __shared__ double shA[BPW * (B + 1)];
+    __shared__ double shB[BPW * (B + 2)];
+    const int t = threadIdx.x;
+    const int lb = t / B, li = t % B;
+    const int bfirst = blockIdx.x * BPW;
+    const int b = bfirst + lb;
+    const int j = b * B + li;
+    const int chunk = blockIdx.y;
+    const int q0 = chunk * J.q_per_chunk;
+    const int q1 = min(q0 + J.q_per_chunk, J.nq);
+    const bool valid = (b < J.nb) && (j < J.n);
+    const int bsize = (b < J.nb) ? min(B, J.n - b * B) : 0;
+    const int pk = min(J.G0, bsize);
+    const double Sj = valid ? J.S[j] : 0.0;
+    const double vj = (valid && V.v) ? V.v[j] : 0.0;
+    const long long bplane = (long long)J.nb * (B + 1);
+    double accE = 0.0, accD = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        for (int e = t; e < BPW * (B + 1); e += 256) {
+            const int lbk = e / (B + 1), m = e % (B + 1);
+            const int bb = bfirst + lbk;
+            const bool ok = bb < J.nb;
+            const long long o = q * bplane + (long long)bb * (B + 1) + m;
+            shP[e] = ok ? J.blocks[o] : 0.0;
+            shD[e] = ok ? V.dblocks[o] : 0.0;
+            shA[e] = (ok && m < J.G0) ? J.g[q * J.g_plane + (long long)bb * J.g_stride + m] : 0.0;
+        }
+        for (int e = t; e < BPW * (B + 2); e += 256) {
+            const int lbk = e / (B + 2), m = e % (B + 2);
+            const int bb = bfirst + lbk;
+            shB[e] = (bb < J.nb && m < V.c0 && V.bvec) ? V.bvec[q * V.b_plane + (long long)bb * V.b_stride + m]
+                                                         : 0.0;
+        }
+        __syncthreads();
+        if (valid) {
+            const double lv = J.logv[q], wq = J.wq[q];
+            const double ivv = exp(-lv);
+            const double a = J.a_in ? J.a_in[(long long)q * J.n + j] : a_of(Sj, lv);
+            const double bm = 1.0 - a;
+            double pw = wq * Sj * a * ivv;
+            if (!isfinite(pw)) pw = 0.0;
+            const double c = V.c_in ? V.c_in[(long long)q * J.n + j] : vj * pw;
+            const double *P = shP + lb * (B + 1);
+            const double *D = shD + lb * (B + 1);
+            const double *A = shA + lb * (B + 1);   // A[m] = 0 for m >= G0 (A[B] = 0)
+            const double *Bv = shB + lb * (B + 2);  // Bv[m] = 0 for m >= c0
+            double psi = 0.0, h = 0.0;
+            if (a > 0.5) {
+                const double ia = 1.0 / a, r = -bm * ia;
+                double qv = 0.0, wv = 0.0;
+                for (int m = 0; m < bsize; ++m) {
+                    qv = P[m] * ia + r * qv;
+                    if (m < pk) psi += A[m] * qv;
+                    h += qv * (Bv[m] - Bv[m + 1]);
+                    if (m <= bsize - 2) {
+                        wv = (D[m] - c * qv) * ia + r * wv;
+                        h += wv * (A[m] - A[m + 1]);
+                    }
+                }
+            } else if (a > 1e-15) {
+                const double ib = 1.0 / bm, r = -a * ib;
+                double qv = 0.0, wv = 0.0;
+                for (int m = bsize - 1; m >= 0; --m) {
+                    qv = (m == bsize - 1) ? P[bsize] * ib : P[m + 1] * ib + r * qv;
+                    if (m < pk) psi += A[m] * qv;
+                    h += qv * (Bv[m] - Bv[m + 1]);
+                    if (m >= 1) {
+                        wv = (D[m] - c * qv) * ib + r * wv;     // W[m-1]
+                        h += wv * (A[m - 1] - A[m]);
+                    }
+                }
+            } else {
+                for (int m = 0; m < bsize; ++m) {
+                    const double qv = P[m + 1];
+                    if (m < pk) psi += A[m] * qv;
+                    h += qv * (Bv[m] - Bv[m + 1]);
+                    if (m <= bsize - 2) {
+                        const double wv = D[m + 1] - c * (m + 2 <= B ? P[m + 2] : 0.0);
+                        h += wv * (A[m] - A[m + 1]);
+                    }
+                }
+            }
+            if (J.inner_out) J.inner_out[(long long)q * J.n + j] = psi;
+            if (V.h_out) V.h_out[(long long)q * J.n + j] = h;
+            accE += pw * psi;
+            double dterm = vj * wq * ivv * a * (1.0 + Sj * lv) * psi;
+            if (!isfinite(dterm)) dterm = 0.0;
+            accD += dterm + h * a * lv;
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        if (J.partial) J.partial[(long long)chunk * J.n + j] = accE;
+        if (V.partial_ds) V.partial_ds[(long long)chunk * J.n + j] = accD;
+    }
+}
+
+__global__ void k_gather(const double *__restrict__ src, const int *__restrict__ perm, int n,
+                         double *__restrict__ dst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dst[j] = src[perm[j]];
+}
+
 __global__ void k_finalize(const double *__restrict__ partial, int nchunk, int n,
                            const unsigned char *__restrict__ active, double *__restrict__ acc,
                            int accumulate) {
@@ -178,30 +354,63 @@ __device__ __forceinline__ int slot_parent(const LevelJob &J, int i) {
     return J.plist ? J.plist[i] : i;
 }
 
-// Child pair of parent p packed as L + i R; a missing right child is the
-// identity polynomial; p < 0 gives zeros.
-__device__ __forceinline__ double2 load_child_pair(const LevelJob &J, const double *cb, int p, int m) {
+// Child pair of parent p packed as L + i R (q-plane qi); a missing right
+// child is the identity polynomial (or 0 with r_missing_zero); p < 0 gives
+// zeros.
+__device__ __forceinline__ double2 load_child_pair(const LevelJob &J, int qi, int p, int m) {
     double2 z = make_double2(0.0, 0.0);
     if (p >= 0 && p < J.nr_parent) {
-        const int cs = J.child.stride, cc = J.child.count;
-        const long long off = (long long)(2 * p) * cs;
+        const int cc = J.child.count;
         const bool has_r = (2 * p + 1) < J.nr_child;
         if (m < cc) {
-            z.x = cb[off + m];
-            if (has_r) z.y = cb[off + cs + m];
+            z.x = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m];
+            if (has_r) {
+                const CoefView &R = J.child_r.base ? J.child_r : J.child;
+                z.y = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m];
+            }
         }
-        if (!has_r && m == 0) z.y = 1.0;
+        if (!has_r && m == 0 && !J.r_missing_zero) z.y = 1.0;
     }
     return z;
 }
 
-// Exact constant term L0 R0 and top term L[c-1] R[c-1] of parent p.
-__device__ __forceinline__ double child_prod_at(const LevelJob &J, const double *cb, int p, int m) {
-    const int cs = J.child.stride;
-    const long long off = (long long)(2 * p) * cs;
-    const double l = cb[off + m];
-    const double r = ((2 * p + 1) < J.nr_child) ? cb[off + cs + m] : (m == 0 ? 1.0 : 0.0);
+// Exact product term L[m] R[m] of parent p (m = 0 or m = c-1).
+__device__ __forceinline__ double child_prod_at(const LevelJob &J, int qi, int p, int m) {
+    const double l = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m];
+    double r;
+    if ((2 * p + 1) < J.nr_child) {
+        const CoefView &R = J.child_r.base ? J.child_r : J.child;
+        r = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m];
+    } else {
+        r = (m == 0 && !J.r_missing_zero) ? 1.0 : 0.0;
+    }
     return l * r;
+}
+
+__device__ __forceinline__ void put(double *dst, double v, int acc) {
+    if (acc) *dst += v;
+    else *dst = v;
+}
+
+// Wrapped term of a propagate output coefficient m = c-1 when F = 2c - 2:
+// the cyclic correlation read Y[0] where the linear one reads Y[F].
+//   side 0 (left output, correlated with R), side 1 (right output, with L).
+__device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int p, int side) {
+    const int c = J.child.count;
+    const double *Y = J.gpar.base + (long long)qi * J.gpar.plane + (long long)p * J.gpar.stride;
+    const double yF = (J.F < J.gpar.count) ? Y[J.F] : 0.0;
+    double x;
+    if (side == 0) {
+        if ((2 * p + 1) < J.nr_child) {
+            const CoefView &R = J.child_r.base ? J.child_r : J.child;
+            x = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + c - 1];
+        } else {
+            x = (c == 1 && !J.r_missing_zero) ? 1.0 : 0.0;
+        }
+    } else {
+        x = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + c - 1];
+    }
+    return x * (yF - Y[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,7 +437,6 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
     const int t = threadIdx.x;
     const int qi = blockIdx.y;
     const int pair0 = blockIdx.x * ITEMS;
-    const double *cb = J.child.base + (long long)qi * J.child.plane;
 
     double2 P1[kLP / 2];
 #pragma unroll
@@ -237,7 +445,7 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[TL::at(item, m)] = load_child_pair(J, cb, 2 * (pair0 + item) + which, m);
+            lds[TL::at(item, m)] = load_child_pair(J, qi, 2 * (pair0 + item) + which, m);
         }
         fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
@@ -273,14 +481,16 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
         if (m < pc) {
             const double2 v = lds[TL::at(item, m)];
             if (p < J.nr_parent)
-                pb[(long long)p * ps_ + m] = (m == 0) ? child_prod_at(J, cb, p, 0) : v.x * invF;
+                put(&pb[(long long)p * ps_ + m], (m == 0) ? child_prod_at(J, qi, p, 0) : v.x * invF, J.acc_out);
             if (p + 1 < J.nr_parent)
-                pb[(long long)(p + 1) * ps_ + m] = (m == 0) ? child_prod_at(J, cb, p + 1, 0) : v.y * invF;
+                put(&pb[(long long)(p + 1) * ps_ + m], (m == 0) ? child_prod_at(J, qi, p + 1, 0) : v.y * invF,
+                    J.acc_out);
         }
         if (m == 0 && pc > F) {
             const int top = J.child.count - 1;
-            if (p < J.nr_parent) pb[(long long)p * ps_ + F] = child_prod_at(J, cb, p, top);
-            if (p + 1 < J.nr_parent) pb[(long long)(p + 1) * ps_ + F] = child_prod_at(J, cb, p + 1, top);
+            if (p < J.nr_parent) put(&pb[(long long)p * ps_ + F], child_prod_at(J, qi, p, top), J.acc_out);
+            if (p + 1 < J.nr_parent)
+                put(&pb[(long long)(p + 1) * ps_ + F], child_prod_at(J, qi, p + 1, top), J.acc_out);
         }
     }
 }
@@ -304,7 +514,6 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
     const int t = threadIdx.x;
     const int qi = blockIdx.y;
     const int pair0 = blockIdx.x * ITEMS;
-    const double *cb = J.child.base + (long long)qi * J.child.plane;
     const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
 
@@ -345,7 +554,7 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[TL::at(item, m)] = load_child_pair(J, cb, slot_parent(J, 2 * (pair0 + item) + which), m);
+            lds[TL::at(item, m)] = load_child_pair(J, qi, slot_parent(J, 2 * (pair0 + item) + which), m);
         }
         fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
@@ -373,8 +582,13 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
             const int p = slot_parent(J, 2 * (pair0 + item) + which);
             if (p >= 0 && m < gcc) {
                 const double2 v = lds[TL::at(item, m)];
-                gcb[(long long)(2 * p) * gcs + m] = v.x * invF;
-                if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * gcs + m] = v.y * invF;
+                double vl = v.x * invF, vr = v.y * invF;
+                if (J.gfix && m == J.child.count - 1) {
+                    vl += prop_wrap_term(J, qi, p, 0);
+                    vr += prop_wrap_term(J, qi, p, 1);
+                }
+                put(&gcb[(long long)(2 * p) * gcs + m], vl, J.acc_out);
+                if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * gcs + m], vr, J.acc_out);
             }
         }
         __syncthreads();
@@ -394,8 +608,7 @@ constexpr int kF2 = kTile;
 struct LdChildPair {      // array a = slot (parent slot_parent(a))
     LevelJob J;
     __device__ double2 operator()(int a, int qi, int n) const {
-        const double *cb = J.child.base + (long long)qi * J.child.plane;
-        return load_child_pair(J, cb, slot_parent(J, a), n);
+        return load_child_pair(J, qi, slot_parent(J, a), n);
     }
 };
 struct LdGPair {          // array a = pair of slots
@@ -415,20 +628,21 @@ struct StParentPair {     // array a = pair of parents (build)
     LevelJob J;
     __device__ void operator()(int a, int qi, int n, double2 v) const {
         const double invF = 1.0 / (double)J.F;
-        const double *cb = J.child.base + (long long)qi * J.child.plane;
         double *pb = J.parent.base + (long long)qi * J.parent.plane;
         const int p = 2 * a;
         const int ps = J.parent.stride, pc = J.parent.count;
         if (n < pc) {
             if (p < J.nr_parent)
-                pb[(long long)p * ps + n] = (n == 0) ? child_prod_at(J, cb, p, 0) : v.x * invF;
+                put(&pb[(long long)p * ps + n], (n == 0) ? child_prod_at(J, qi, p, 0) : v.x * invF, J.acc_out);
             if (p + 1 < J.nr_parent)
-                pb[(long long)(p + 1) * ps + n] = (n == 0) ? child_prod_at(J, cb, p + 1, 0) : v.y * invF;
+                put(&pb[(long long)(p + 1) * ps + n], (n == 0) ? child_prod_at(J, qi, p + 1, 0) : v.y * invF,
+                    J.acc_out);
         }
         if (n == 0 && pc > J.F) {
             const int top = J.child.count - 1;
-            if (p < J.nr_parent) pb[(long long)p * ps + J.F] = child_prod_at(J, cb, p, top);
-            if (p + 1 < J.nr_parent) pb[(long long)(p + 1) * ps + J.F] = child_prod_at(J, cb, p + 1, top);
+            if (p < J.nr_parent) put(&pb[(long long)p * ps + J.F], child_prod_at(J, qi, p, top), J.acc_out);
+            if (p + 1 < J.nr_parent)
+                put(&pb[(long long)(p + 1) * ps + J.F], child_prod_at(J, qi, p + 1, top), J.acc_out);
         }
     }
 };
@@ -440,8 +654,13 @@ struct StGChildren {      // array a = slot (propagate)
         double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
         const int p = slot_parent(J, a);
         if (p < 0) return;
-        gcb[(long long)(2 * p) * J.gchild.stride + n] = v.x * invF;
-        if (2 * p + 1 < J.nr_child) gcb[(long long)(2 * p + 1) * J.gchild.stride + n] = v.y * invF;
+        double vl = v.x * invF, vr = v.y * invF;
+        if (J.gfix && n == J.child.count - 1) {
+            vl += prop_wrap_term(J, qi, p, 0);
+            vr += prop_wrap_term(J, qi, p, 1);
+        }
+        put(&gcb[(long long)(2 * p) * J.gchild.stride + n], vl, J.acc_out);
+        if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * J.gchild.stride + n], vr, J.acc_out);
     }
 };
 
@@ -706,6 +925,43 @@ hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof
     const double nq = J.nq, B = J.B;
     PF_END(kname("k_leaf_extract<%d>", J.B), 8.0 * nq * J.nbl * (B + 1 + J.G0) + 8.0 * J.n + 8.0 * nchunk * J.n,
            nq * (double)J.nbl * B * (4.0 * B + 30.0));
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_build_vjp(const VjpLeafJob &V, hipStream_t s, Prof *pf) {
+    const LeafJob &J = V.L;
+    dim3 grid((J.nb + 63) / 64, J.nq);
+    PF_BEGIN();
+    switch (J.B) {
+    case 16: k_leaf_build_vjp<16><<<grid, 64, 0, s>>>(V); break;
+    case 32: k_leaf_build_vjp<32><<<grid, 64, 0, s>>>(V); break;
+    case 64: k_leaf_build_vjp<64><<<grid, 64, 0, s>>>(V); break;
+    default: return hipErrorInvalidValue;
+    }
+    const double nq = J.nq, B = J.B;
+    PF_END(kname("k_leaf_build_vjp<%d>", J.B), 16.0 * J.n + 16.0 * nq * J.nb * (B + 1), nq * J.nb * 3.0 * B * (B + 1));
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_vjp(const VjpLeafJob &V, int nchunk, hipStream_t s, Prof *pf) {
+    const LeafJob &J = V.L;
+    const int bpw = 256 / J.B;
+    dim3 grid((J.nb + bpw - 1) / bpw, nchunk);
+    PF_BEGIN();
+    switch (J.B) {
+    case 16: k_leaf_vjp<16><<<grid, 256, 0, s>>>(V); break;
+    case 32: k_leaf_vjp<32><<<grid, 256, 0, s>>>(V); break;
+    case 64: k_leaf_vjp<64><<<grid, 256, 0, s>>>(V); break;
+    default: return hipErrorInvalidValue;
+    }
+    const double nq = J.nq, B = J.B;
+    PF_END(kname("k_leaf_vjp<%d>", J.B), 8.0 * nq * J.nb * (3 * (B + 1) + V.c0) + 16.0 * J.n + 16.0 * nchunk * J.n,
+           nq * J.n * (10.0 * B + 40.0));
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const double *src, const int *perm, int n, double *dst, hipStream_t s) {
+    k_gather<<<(n + 255) / 256, 256, 0, s>>>(src, perm, n, dst);
     return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ struct CoefOut {
 };
 
 struct LevelJob {
-    CoefView child;     // level l-1 products
+    CoefView child;     // level l-1 products (left children; right too unless child_r)
     CoefOut parent;     // level l products (build)
     CoefView gpar;      // level l g-vectors (propagate); plane 0 at the root
     CoefOut gchild;     // level l-1 g-vectors (propagate)
@@ -37,6 +37,12 @@ struct LevelJob {
     // (np = nr_parent).  Build launches always use the full level.
     const int *plist;
     int np;
+    // Composition for the adjoint pass (zero / null for the forward pass):
+    CoefView child_r;   // right children from another array (base == nullptr: child)
+    int r_missing_zero; // a missing right child is 0 instead of the identity
+    int acc_out;        // add to the outputs instead of storing
+    int gfix;           // propagate outputs keep c_child terms with F = 2c - 2:
+                        // add the one wrapped term X[c-1] (Y[F] - Y[0])
 };
 
 struct LeafJob {
@@ -59,6 +65,20 @@ struct LeafJob {
     int nbl;                // blocks in the launch (= nb without a list)
 };
 
+// Adjoint (VJP) leaf data: tangent block products and the two adjoints.
+struct VjpLeafJob {
+    LeafJob L;              // blocks = P_b, g = A_b (forward g), partial = equity partials
+    const double *v;        // cotangent, sorted order [n]
+    const double *c_in;     // optional [nq][n] tangent weights (replaces v * pw)
+    double *dblocks;        // [nq][nb][B+1] tangent block products
+    const double *bvec;     // B adjoint at the leaves: base, plane, stride (null: zero)
+    long long b_plane;
+    int b_stride;
+    int c0;                 // B entries per block (= level-0 coefficient count)
+    double *partial_ds;     // [nchunk][n] dS partials (sorted order)
+    double *h_out;          // optional [nq][n] H_j values
+};
+
 // Optional per-launch timing: when on, every kernel launch is bracketed by
 // two events on the launch stream and tagged with its compulsory HBM bytes
 // and FLOPs (inputs read once + outputs written once; 5 N log2 N per complex
@@ -79,6 +99,11 @@ hipError_t launch_finalize(const double *partial, int nchunk, int n, const unsig
                            double *acc, int accumulate, hipStream_t s, Prof *pf);
 hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s,
                           Prof *pf);
+
+// Adjoint leaves: (P_b, dP_b) per block; then Psi, H and dS per player.
+hipError_t launch_leaf_build_vjp(const VjpLeafJob &J, hipStream_t s, Prof *pf);
+hipError_t launch_leaf_vjp(const VjpLeafJob &J, int nchunk, hipStream_t s, Prof *pf);
+hipError_t launch_gather(const double *src, const int *perm, int n, double *dst, hipStream_t s);
 
 // One tree level, LDS-resident transforms (F <= 4096).
 hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);

@@ -110,6 +110,22 @@ int icm_hip_plan_set_targets(IcmHipPlan *plan, const int *targets, int n_targets
  * (src/cpu/icm.h:45-47, engine_hybrid_ctx src/cpu/icm.c:2307-2318). */
 int icm_hip_node_inner(int n, const double *a, const double *payout, int k, double *inner);
 
+/* ---- adjoint (no reference counterpart: the reference has no gradient) ----
+ * dS[i] = d(v . equity)/dS_i for a cotangent v[n] (original player order),
+ * with the quadrature nodes held fixed (they depend on max S only through
+ * y_hi).  The pass also returns the equities (either output may be NULL).
+ * Plans restricted with icm_hip_plan_set_targets are rejected. */
+int icm_hip_vjp_with_plan(IcmHipPlan *plan, int Q, const double *payout, const double *v, double *equity,
+                          double *dS, IcmHipRunStats *stats);
+int icm_hip_vjp_with_plan_device(IcmHipPlan *plan, int Q, const double *d_payout, const double *d_v,
+                                 double *d_equity, double *d_dS, IcmHipRunStats *stats);
+int icm_hip_equity_vjp(int n, const double *S, int Q, const double *payout, int k, const double *v,
+                       double *equity, double *dS);
+/* One node: inner[i] = Psi_i and H[i] = sum_j c_j dPsi_j/da_i for given a, c
+ * (players in the given order). */
+int icm_hip_node_vjp(int n, const double *a, const double *c, const double *payout, int k, double *inner,
+                     double *H);
+
 /* ---- reference GPU names (src/gpu/icm_gpu.h:66-111), same signatures ---- */
 int icm_gpu_init(int device_id);
 void icm_gpu_shutdown(void);

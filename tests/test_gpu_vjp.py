@@ -1,0 +1,142 @@
+"""GPU adjoint: dS = d(v . E)/dS through the C ABI.
+
+Pins:
+  * per node, H_j = sum_i c_i dPsi_i/da_j against central differences of the
+    oracle's Phi(a) = sum_i c_i Psi_i(a): Psi_i is multilinear in a, so Phi is
+    linear in each a_j and the central difference is exact up to rounding;
+  * the analytic gradient of the V1 closed form (k = n, payout n - r):
+    dV1_i/dS_i = sum_{j!=i} S_j/(S_i+S_j)^2, dV1_i/dS_j = -S_i/(S_i+S_j)^2;
+  * central finite differences (h = 1e-6 S_i) of the oracle's FP64 equities;
+  * the equities returned by the adjoint pass equal the forward pass.
+Quadrature nodes are held fixed (they depend on max S through y_hi), so the
+finite-difference coordinates exclude the largest stack.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import icm_amd
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def conditioned_a(n, k, seed):
+    """a_j with about k/2 expected 'x' factors, so Psi is O(payout) for any k
+    (with k << n and a ~ U(0,1) every Psi is ~1e-50 and differences are noise)."""
+    rng = np.random.default_rng(seed)
+    a = 1.0 - (k / (2.0 * n)) * rng.uniform(0.0, 2.0, n)
+    return np.clip(a, 0.02, 0.999), rng
+
+
+@pytest.mark.parametrize("n,k", [(10, 4), (40, 13), (300, 300), (2000, 150), (5000, 5000), (20000, 20000)])
+def test_node_adjoint_vs_exact_differences(n, k):
+    a, rng = conditioned_a(n, k, n + k)
+    c = rng.normal(size=n)
+    p = np.sort(rng.uniform(0, 1, k))[::-1]
+    say(f"node vjp n={n} k={k}")
+    psi, h = icm_amd.node_vjp(a, c, p)
+    fwd = icm_amd.node_inner(a, p)
+    assert np.max(np.abs(psi - fwd)) <= 1e-13 * p.max()
+    eng = O.ENGINE_NAIVE if n <= 2000 else O.ENGINE_HYBRID
+
+    def phi(aa):
+        return float(np.dot(c, O.inner(aa, p, engine=eng, B=32, fft_min=64)))
+
+    coords = range(n) if n <= 40 else rng.choice(n, 12, replace=False)
+    scale = 0.0
+    errs = []
+    for j in coords:
+        hh = 0.5 * min(a[j], 1.0 - a[j], 0.1)
+        if hh < 1e-6:
+            continue
+        ap = a.copy(); ap[j] += hh
+        am = a.copy(); am[j] -= hh
+        fd = (phi(ap) - phi(am)) / (2 * hh)
+        scale = max(scale, abs(fd))
+        errs.append(abs(h[j] - fd))
+    err = max(errs) / max(scale, 1e-300)
+    say(f"  max |H - FD| / max|FD| = {err:.2e} over {len(errs)} coordinates")
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("n", [200, 2000])
+def test_vjp_v1_analytic_gradient(n):
+    rng = np.random.default_rng(n)
+    S = rng.uniform(1.0, 100.0, n)
+    p = np.arange(n, 0, -1, dtype=float)
+    v = rng.normal(size=n)
+    eq, ds = icm_amd.equity_vjp(S, p, v, Q=256)
+    Si, Sj = S[:, None], S[None, :]
+    den = (Si + Sj) ** 2
+    off = ~np.eye(n, dtype=bool)
+    dii = np.where(off, Sj / den, 0.0).sum(axis=1)          # dV1_i/dS_i
+    dij = np.where(off, -Si / den, 0.0)                      # dV1_i/dS_j
+    exact = v * dii + (v[:, None] * dij).sum(axis=0)
+    v1 = O.v1_exact(S)
+    e_eq = np.max(np.abs(eq - v1) / v1)
+    e_ds = np.max(np.abs(ds - exact)) / np.max(np.abs(exact))
+    say(f"V1 n={n}: equity {e_eq:.2e}, gradient {e_ds:.2e}")
+    assert e_eq < 1e-9 and e_ds < 1e-8
+
+
+@pytest.mark.parametrize("n,k", [(300, 50), (1500, 1500)])
+def test_vjp_vs_oracle_finite_differences(n, k):
+    rng = np.random.default_rng(7 * n + k)
+    S = np.exp(rng.uniform(0.0, math.log(1e4), n))
+    p = 0.85 ** np.arange(k)
+    p /= p.sum()
+    v = rng.normal(size=n)
+    eq, ds = icm_amd.equity_vjp(S, p, v, Q=256)
+    fwd = icm_amd.equity(S, p, Q=256)
+    assert np.max(np.abs(eq - fwd)) <= 1e-12 * np.max(np.abs(fwd))
+    imax = int(np.argmax(S))
+    coords = [i for i in rng.choice(n, 17, replace=False) if i != imax][:16]
+    scale = np.max(np.abs(ds))
+
+    def fd(i, rel):
+        h = rel * S[i]
+        Sp = S.copy(); Sp[i] += h
+        Sm = S.copy(); Sm[i] -= h
+        # schoolbook products (fft_min=0): the least noisy oracle engine
+        return (np.dot(v, O.equity(Sp, p, Q=256, fft_min=0)) - np.dot(v, O.equity(Sm, p, Q=256, fft_min=0))) / (2 * h)
+
+    # The difference quotient's rounding noise scales like 1/h (about 1e-11
+    # absolute at h = 1e-6 S_i here, ~1e-5 x max|dS| at n = 1500), hence 3e-5 at h = 1e-6 S_i;
+    # h = 1e-5 S_i has 10x less noise and O(h^2) ~ 1e-10 bias.
+    e6 = max(abs(ds[i] - fd(i, 1e-6)) for i in coords) / scale
+    e5 = max(abs(ds[i] - fd(i, 1e-5)) for i in coords) / scale
+    say(f"FD n={n} k={k}: max |dS - FD| / max|dS| = {e6:.2e} (h=1e-6 S), {e5:.2e} (h=1e-5 S)")
+    assert e6 < 3e-5 and e5 < 1e-6
+
+
+def test_vjp_large_shape_self_consistency():
+    """n = 131072, k = 1000, lognormal(sigma 1.5) stacks: adjoint vs differences of the GPU forward."""
+    rng = np.random.default_rng(5)
+    n, k = 131072, 1000
+    S = rng.lognormal(0.0, 1.5, n)
+    p = np.arange(k, 0, -1, dtype=float)
+    p /= p.sum()
+    v = rng.normal(size=n)
+    plan = icm_amd.Plan(S, k)
+    eq, ds = plan.vjp(p, v, Q=256)
+    fwd = plan.run(p, Q=256)
+    plan.close()
+    assert np.max(np.abs(eq - fwd)) <= 1e-12 * np.max(np.abs(fwd))
+    imax = int(np.argmax(S))
+    coords = [i for i in rng.choice(n, 5, replace=False) if i != imax][:4]
+    scale = np.max(np.abs(ds))
+    for i in coords:
+        h = 1e-5 * S[i]
+        Sp = S.copy(); Sp[i] += h
+        Sm = S.copy(); Sm[i] -= h
+        fd = (np.dot(v, icm_amd.equity(Sp, p, Q=256)) - np.dot(v, icm_amd.equity(Sm, p, Q=256))) / (2 * h)
+        say(f"large n={n} coord {i}: dS={ds[i]:.6e} FD={fd:.6e}")
+        assert abs(ds[i] - fd) <= 1e-5 * scale + 1e-6 * abs(fd)
