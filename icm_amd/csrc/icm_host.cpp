@@ -613,6 +613,13 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             J.F = g.F[l]; J.logF = ilog2i(g.F[l]); J.nq = nq;
             J.child = view(l - 1, false);
             J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            if (J.F <= icm::kMaxLdsF) {                           // P_X and dP_X in one pass
+                J.tchild = view(l - 1, true);
+                J.tparent = icm::CoefOut{p->d_dcoef[l], J.parent.plane, J.parent.stride, J.parent.count};
+                if (!hip_ok(icm::launch_build_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp build level"))
+                    return -1;
+                continue;
+            }
             if (!level(J, true)) return -1;                       // P_X = P_L P_R
             J.parent.base = p->d_dcoef[l];
             J.child = view(l - 1, true);
@@ -637,10 +644,21 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             J.child = view(l - 1, false);
             J.gpar = Apar;
             J.gchild = icm::CoefOut{p->d_g[na], (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
-            if (!level(J, false)) return -1;                      // A_L = A_X * R
             const int cc = g.c[l - 1];
             icm::CoefOut Bout{p->d_bv[nbv], (long long)g.n_real[l - 1] * cc, cc, cc};
             const int gfix = (g.F[l] < 2 * cc - 1) ? 1 : 0;
+            if (J.F <= icm::kMaxLdsF) {                           // A and B adjoints in one pass
+                J.tchild = view(l - 1, true);
+                J.bpar = root ? icm::CoefView{} : Bpar;
+                J.bchild = Bout;
+                J.gfix = gfix;
+                if (!hip_ok(icm::launch_prop_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp propagate level"))
+                    return -1;
+                ca = na;
+                cb = nbv;
+                continue;
+            }
+            if (!level(J, false)) return -1;                      // A_L = A_X * R
             if (!root) {
                 icm::LevelJob K = J;
                 K.gpar = Bpar;

@@ -596,6 +596,270 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
 }
 
 // ---------------------------------------------------------------------------
+// Fused adjoint levels, LDS-resident (F <= 4096), one parent per item.
+//   build:  P_X = P_L P_R,  dP_X = dP_L P_R + P_L dP_R
+//           FFT(P_L + iP_R), FFT(dP_L + i dP_R), inverse of P_X + i dP_X
+//   propagate (A = forward g, B = adjoint of P):
+//           A_L = A*P_R, A_R = A*P_L, B_L = B*P_R + A*dP_R, B_R = B*P_L + A*dP_L
+//           FFT(A + iB), FFT(P_L + iP_R), FFT(dP_L + i dP_R),
+//           inverses of A_L + iA_R and B_L + iB_R
+// (* = the correlation of the propagate).  A missing right child has P = 1,
+// dP = 0.  Coefficient fixes as the forward kernels: exact 0 / top terms in
+// the build, the one wrapped B coefficient in the propagate.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 load_view_pair(const CoefView &V, int qi, int p, int m, int nr_child,
+                                                  bool missing_is_one) {
+    double2 z = make_double2(0.0, 0.0);
+    if (p < 0) return z;
+    const bool has_r = (2 * p + 1) < nr_child;
+    if (m < V.count) {
+        const double *b = V.base + (long long)qi * V.plane + (long long)(2 * p) * V.stride;
+        z.x = b[m];
+        if (has_r) z.y = b[V.stride + m];
+    }
+    if (!has_r && m == 0 && missing_is_one) z.y = 1.0;
+    return z;
+}
+
+__device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, int m) {
+    return V.base[(long long)qi * V.plane + (long long)node * V.stride + m];
+}
+
+template <int F>
+__global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    using TL = TileLayout<F>;
+    __shared__ double2 lds[TL::SIZE];
+    constexpr int ITEMS = kTile / F;
+    constexpr int HALF = F / 2;
+    constexpr int LOGF = ilog2c(F);
+    constexpr int LOGH = ilog2c(HALF);
+    const int t = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int p0 = blockIdx.x * ITEMS;
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
+    }
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+    double2 XL[kLP / 2], XR[kLP / 2];
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
+        if (jb == 0) { XL[u] = make_double2(za.x, zb.x); XR[u] = make_double2(za.y, zb.y); }
+        else { XL[u] = split_x(za, zb); XR[u] = split_y(za, zb); }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
+    }
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
+        const double2 za = lds[ia], zb = lds[ib];
+        if (jb == 0) {
+            lds[ia] = make_double2(XL[u].x * XR[u].x, za.x * XR[u].x + XL[u].x * za.y);
+            lds[ib] = make_double2(XL[u].y * XR[u].y, zb.x * XR[u].y + XL[u].y * zb.y);
+        } else {
+            const double2 DL = split_x(za, zb), DR = split_y(za, zb);
+            const double2 Pv = c_mul(XL[u], XR[u]);
+            const double2 Dv = c_add(c_mul(DL, XR[u]), c_mul(XL[u], DR));
+            lds[ia] = pack_ai(Pv, Dv);
+            lds[ib] = pack_ai(c_conj(Pv), c_conj(Dv));
+        }
+    }
+    fft_tile<F, true, kLT>(lds, tw, twShift);
+    const double invF = 1.0 / (double)F;
+    const int pc = J.parent.count, cc = J.child.count;
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = p0 + item;
+        if (p >= J.nr_parent) continue;
+        double *pp = J.parent.base + (long long)qi * J.parent.plane + (long long)p * J.parent.stride;
+        double *dp = J.tparent.base + (long long)qi * J.tparent.plane + (long long)p * J.tparent.stride;
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        auto exact = [&](int mm, double &pv, double &dv) {
+            const double l = view_at(J.child, qi, 2 * p, mm), dl = view_at(J.tchild, qi, 2 * p, mm);
+            const double r = has_r ? view_at(J.child, qi, 2 * p + 1, mm) : (mm == 0 ? 1.0 : 0.0);
+            const double dr = has_r ? view_at(J.tchild, qi, 2 * p + 1, mm) : 0.0;
+            pv = l * r;
+            dv = dl * r + l * dr;
+        };
+        if (m < pc) {
+            double pv, dv;
+            if (m == 0) {
+                exact(0, pv, dv);
+            } else {
+                const double2 v = lds[TL::at(item, m)];
+                pv = v.x * invF;
+                dv = v.y * invF;
+            }
+            pp[m] = pv;
+            dp[m] = dv;
+        }
+        if (m == 0 && pc > F) {
+            double pv, dv;
+            exact(cc - 1, pv, dv);
+            pp[F] = pv;
+            dp[F] = dv;
+        }
+    }
+}
+
+template <int F>
+__global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    using TL = TileLayout<F>;
+    __shared__ double2 lds[TL::SIZE];
+    constexpr int ITEMS = kTile / F;
+    constexpr int HALF = F / 2;
+    constexpr int LOGF = ilog2c(F);
+    constexpr int LOGH = ilog2c(HALF);
+    const int t = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int p0 = blockIdx.x * ITEMS;
+    // (1) A + iB of the parent
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = p0 + item;
+        double2 z = make_double2(0.0, 0.0);
+        if (p < J.nr_parent) {
+            if (m < J.gpar.count) z.x = view_at(J.gpar, qi, p, m);
+            if (J.bpar.base && m < J.bpar.count) z.y = view_at(J.bpar, qi, p, m);
+        }
+        lds[TL::at(item, m)] = z;
+    }
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+    double2 GA[kLP / 2], GB[kLP / 2];
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
+        if (jb == 0) { GA[u] = make_double2(za.x, zb.x); GB[u] = make_double2(za.y, zb.y); }
+        else { GA[u] = split_x(za, zb); GB[u] = split_y(za, zb); }
+    }
+    __syncthreads();
+    // (2) P_L + iP_R
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
+    }
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+    double2 XL[kLP / 2], XR[kLP / 2];
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
+        if (jb == 0) { XL[u] = make_double2(za.x, zb.x); XR[u] = make_double2(za.y, zb.y); }
+        else { XL[u] = split_x(za, zb); XR[u] = split_y(za, zb); }
+    }
+    __syncthreads();
+    // (3) dP_L + i dP_R; spectra of the outputs
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
+    }
+    fft_tile<F, false, kLT>(lds, tw, twShift);
+    double2 U2a[kLP / 2], U2b[kLP / 2];   // B_L + i B_R at bins jb and F - jb
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
+        const double2 za = lds[ia], zb = lds[ib];
+        if (jb == 0) {
+            // bins 0 and HALF carry real values (x: bin 0, y: bin HALF)
+            const double dl0 = za.x, dr0 = za.y, dlh = zb.x, drh = zb.y;
+            lds[ia] = make_double2(GA[u].x * XR[u].x, GA[u].x * XL[u].x);
+            lds[ib] = make_double2(GA[u].y * XR[u].y, GA[u].y * XL[u].y);
+            U2a[u] = make_double2(GB[u].x * XR[u].x + GA[u].x * dr0, GB[u].x * XL[u].x + GA[u].x * dl0);
+            U2b[u] = make_double2(GB[u].y * XR[u].y + GA[u].y * drh, GB[u].y * XL[u].y + GA[u].y * dlh);
+        } else {
+            const double2 DL = split_x(za, zb), DR = split_y(za, zb);
+            const double2 AL = c_mulc(GA[u], XR[u]), AR = c_mulc(GA[u], XL[u]);
+            const double2 BL = c_add(c_mulc(GB[u], XR[u]), c_mulc(GA[u], DR));
+            const double2 BR = c_add(c_mulc(GB[u], XL[u]), c_mulc(GA[u], DL));
+            lds[ia] = pack_ai(AL, AR);
+            lds[ib] = pack_ai(c_conj(AL), c_conj(AR));
+            U2a[u] = pack_ai(BL, BR);
+            U2b[u] = pack_ai(c_conj(BL), c_conj(BR));
+        }
+    }
+    fft_tile<F, true, kLT>(lds, tw, twShift);
+    const double invF = 1.0 / (double)F;
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = p0 + item;
+        if (p < J.nr_parent && m < J.gchild.count) {
+            const double2 v = lds[TL::at(item, m)];
+            double *g = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+            g[m] = v.x * invF;
+            if (2 * p + 1 < J.nr_child) g[J.gchild.stride + m] = v.y * invF;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
+        const int item = ps >> LOGH, jb = ps & (HALF - 1);
+        lds[TL::at(item, jb)] = U2a[u];
+        lds[TL::at(item, jb == 0 ? HALF : F - jb)] = U2b[u];
+    }
+    fft_tile<F, true, kLT>(lds, tw, twShift);
+    const int cc = J.child.count;
+#pragma unroll
+    for (int u = 0; u < kLP; ++u) {
+        const int e = t + u * kLT;
+        const int item = e >> LOGF, m = e & (F - 1);
+        const int p = p0 + item;
+        if (p < J.nr_parent && m < J.bchild.count) {
+            const double2 v = lds[TL::at(item, m)];
+            double bl = v.x * invF, br = v.y * invF;
+            if (J.gfix && m == cc - 1) {
+                // linear correlation reads Y[F], the cyclic one read Y[0]
+                const bool has_r = (2 * p + 1) < J.nr_child;
+                const double aF = (F < J.gpar.count) ? view_at(J.gpar, qi, p, F) : 0.0;
+                const double a0 = view_at(J.gpar, qi, p, 0);
+                const double bF = (J.bpar.base && F < J.bpar.count) ? view_at(J.bpar, qi, p, F) : 0.0;
+                const double b0 = J.bpar.base ? view_at(J.bpar, qi, p, 0) : 0.0;
+                const double pl = view_at(J.child, qi, 2 * p, cc - 1), dl = view_at(J.tchild, qi, 2 * p, cc - 1);
+                const double pr = has_r ? view_at(J.child, qi, 2 * p + 1, cc - 1) : (cc == 1 ? 1.0 : 0.0);
+                const double dr = has_r ? view_at(J.tchild, qi, 2 * p + 1, cc - 1) : 0.0;
+                bl += pr * (bF - b0) + dr * (aF - a0);
+                br += pl * (bF - b0) + dl * (aF - a0);
+            }
+            double *bb = J.bchild.base + (long long)qi * J.bchild.plane + (long long)(2 * p) * J.bchild.stride;
+            bb[m] = bl;
+            if (2 * p + 1 < J.nr_child) bb[J.bchild.stride + m] = br;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Large levels (F >= 8192): four-step transform with F = F1 x F2, F2 = 4096.
 //   forward: column FFTs (size F1, stride F2) + twiddle w_F^(n2 k1) -> [k1][n2]
 //            row FFTs (size F2) -> spectrum in [k1][k2] order (K = k1 + F1 k2)
@@ -1016,6 +1280,57 @@ hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, h
     }
     const double bytes = level_coef_bytes(J.child, J.nr_child, J.nq) + 8.0 * J.nq * (double)J.nr_parent * J.parent.count;
     PF_END(kname("k_build_lds<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 1.5 * fft_flops(J.F));
+    return e;
+}
+
+template <int F>
+static hipError_t build_lds_t_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    constexpr int ITEMS = kTile / F;
+    dim3 grid((J.nr_parent + ITEMS - 1) / ITEMS, J.nq);
+    k_build_lds_t<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    return hipGetLastError();
+}
+template <int F>
+static hipError_t prop_lds_t_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
+    constexpr int ITEMS = kTile / F;
+    dim3 grid((J.nr_parent + ITEMS - 1) / ITEMS, J.nq);
+    k_prop_lds_t<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    return hipGetLastError();
+}
+
+#define ICM_FSWITCH(FN)                                              \
+    switch (J.F) {                                                   \
+    case 16: e = FN<16>(J, tw, twShift, s); break;                   \
+    case 32: e = FN<32>(J, tw, twShift, s); break;                   \
+    case 64: e = FN<64>(J, tw, twShift, s); break;                   \
+    case 128: e = FN<128>(J, tw, twShift, s); break;                 \
+    case 256: e = FN<256>(J, tw, twShift, s); break;                 \
+    case 512: e = FN<512>(J, tw, twShift, s); break;                 \
+    case 1024: e = FN<1024>(J, tw, twShift, s); break;               \
+    case 2048: e = FN<2048>(J, tw, twShift, s); break;               \
+    case 4096: e = FN<4096>(J, tw, twShift, s); break;               \
+    default: return hipErrorInvalidValue;                            \
+    }
+
+hipError_t launch_build_lds_t(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
+    hipError_t e;
+    ICM_FSWITCH(build_lds_t_f)
+    const double bytes = 2.0 * (level_coef_bytes(J.child, J.nr_child, J.nq) +
+                                8.0 * J.nq * (double)J.nr_parent * J.parent.count);
+    PF_END(kname("k_build_lds_t<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 3.0 * fft_flops(J.F));
+    return e;
+}
+
+hipError_t launch_prop_lds_t(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
+    hipError_t e;
+    ICM_FSWITCH(prop_lds_t_f)
+    const double gin = (J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)J.nr_parent * J.gpar.count) +
+                       8.0 * J.nq * (double)J.nr_parent * J.bpar.count;
+    const double bytes = gin + 2.0 * level_coef_bytes(J.child, J.nr_child, J.nq) +
+                         8.0 * J.nq * (double)J.nr_child * (J.gchild.count + J.bchild.count);
+    PF_END(kname("k_prop_lds_t<%d>", J.F), bytes, J.nq * (double)J.nr_parent * 5.0 * fft_flops(J.F));
     return e;
 }
 

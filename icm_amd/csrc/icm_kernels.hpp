@@ -43,6 +43,11 @@ struct LevelJob {
     int acc_out;        // add to the outputs instead of storing
     int gfix;           // propagate outputs keep c_child terms with F = 2c - 2:
                         // add the one wrapped term X[c-1] (Y[F] - Y[0])
+    // Fused adjoint kernels (k_build_lds_t / k_prop_lds_t):
+    CoefView tchild;    // tangent products of the children
+    CoefOut tparent;    // tangent product of the parent (build)
+    CoefView bpar;      // B adjoint of the parent (propagate; base null = 0 at the root)
+    CoefOut bchild;     // B adjoint of the children (propagate)
 };
 
 struct LeafJob {
@@ -104,6 +109,10 @@ hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equ
 hipError_t launch_leaf_build_vjp(const VjpLeafJob &J, hipStream_t s, Prof *pf);
 hipError_t launch_leaf_vjp(const VjpLeafJob &J, int nchunk, hipStream_t s, Prof *pf);
 hipError_t launch_gather(const double *src, const int *perm, int n, double *dst, hipStream_t s);
+
+// Fused adjoint levels (F <= 4096): P and dP together; A and B together.
+hipError_t launch_build_lds_t(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);
+hipError_t launch_prop_lds_t(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);
 
 // One tree level, LDS-resident transforms (F <= 4096).
 hipError_t launch_build_lds(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);
