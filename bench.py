@@ -226,6 +226,53 @@ def main():
     adjoint["n1048576_k1048576"] = fwd_vjp_ratio(S, p, 2)
     log(f"adjoint: {adjoint}")
 
+    # ragged batch: 4096 independent instances (n log-uniform [2, 4096],
+    # k in {1, 3, 9, ceil(n/10), n}), one graph launch per batch, against
+    # looping the plain drop-in (icm_hip_equity per instance) on a sample
+    def batched_field(reps=5, sample=256):
+        from icm_amd.workloads import ragged_instances
+        stacks, pays = ragged_instances(4096, seed=4)
+        ks = [len(x) for x in pays]
+        t_c0 = time.perf_counter()
+        bp = icm_amd.BatchPlan(stacks, ks)
+        create_ms = (time.perf_counter() - t_c0) * 1e3
+        dP = torch.tensor(np.concatenate(pays), dtype=torch.float64, device=dev)
+        dE = torch.empty(int(bp.n_off[-1]), dtype=torch.float64, device=dev)
+        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)          # records the graph
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        for _ in range(reps):
+            bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+        torch.cuda.synchronize()
+        batch_ms = (time.perf_counter() - t_a) * 1e3 / reps
+        bp.set_profiling(True)
+        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+        bst = bp.kernel_stats()
+        bp.set_profiling(False)
+        summ_b = bp.summary()
+        bp.close()
+        rng = np.random.default_rng(9)
+        idx = rng.choice(len(stacks), sample, replace=False)
+        icm_amd.equity(stacks[idx[0]], pays[idx[0]], Q)
+        t_l = time.perf_counter()
+        for i in idx:
+            icm_amd.equity(stacks[i], pays[i], Q)
+        loop_s = time.perf_counter() - t_l
+        loop_inst_s = sample / loop_s
+        inst_s = len(stacks) / (batch_ms * 1e-3)
+        kern = {k_: {"launches": v[0], "ms": v[1] / 1e6,
+                     "GBps": (v[2] / (v[1] * 1e-9) / 1e9) if v[1] > 0 else 0.0}
+                for k_, v in sorted(bst.items(), key=lambda kv: -kv[1][1])}
+        return {"instances": len(stacks), "players": summ_b["n_players"], "Q": Q,
+                "ms_per_batch": batch_ms, "instances_per_s": inst_s,
+                "loop_dropin_instances_per_s": loop_inst_s,
+                "loop_sample": f"icm_hip_equity on {sample} random instances of the batch",
+                "speedup_vs_loop": inst_s / loop_inst_s, "plan_create_ms": create_ms,
+                "device_bytes": summ_b["device_bytes"], "kernels_eager": kern}
+
+    batched = batched_field()
+    log(f"batched: {batched}")
+
     cpu = None
     if not args.no_cpu_baseline:
         nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
@@ -249,6 +296,7 @@ def main():
         "path_roofline": path_roof,
         "subset": subset,
         "adjoint": adjoint,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -105,6 +105,18 @@ _SIGS = [
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
+    ("icm_hip_batch_plan_create", ctypes.c_void_p, [ctypes.c_int, _I, _D, _I, ctypes.POINTER(Options)]),
+    ("icm_hip_batch_plan_destroy", None, [ctypes.c_void_p]),
+    ("icm_hip_batch_equity", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.POINTER(RunStats)]),
+    ("icm_hip_batch_equity_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RunStats)]),
+    ("icm_hip_equity_batch", ctypes.c_int, [ctypes.c_int, _I, _D, _I, ctypes.c_int, _D, _D]),
+    ("icm_hip_batch_summary", ctypes.c_int,
+     [ctypes.c_void_p, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
+    ("icm_hip_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_batch_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_batch_kernel_stat", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
 ]
 
 _lib = None
@@ -377,6 +389,131 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def _ragged(stacks_list, payouts_list):
+    if len(stacks_list) != len(payouts_list) or len(stacks_list) == 0:
+        raise ValueError("need one payout vector per instance and at least one instance")
+    ss = [_to_double_array(s, "stacks") for s in stacks_list]
+    ps = [_to_double_array(p, "payouts") for p in payouts_list]
+    for s, p in zip(ss, ps):
+        _validate(len(s), len(p), 1)
+    n_off = np.zeros(len(ss) + 1, dtype=np.int32)
+    k_off = np.zeros(len(ss) + 1, dtype=np.int32)
+    n_off[1:] = np.cumsum([len(s) for s in ss])
+    k_off[1:] = np.cumsum([len(p) for p in ps])
+    return np.concatenate(ss), np.concatenate(ps), n_off, k_off
+
+
+class BatchPlan:
+    """Ragged batch of independent instances (instance i: stacks_list[i] and
+    k_list[i] payouts).  ``run(payouts_list, Q)`` returns one equity array per
+    instance; the first run for a Q is recorded as a HIP graph and replayed
+    by later runs."""
+
+    def __init__(self, stacks_list, k_list, block_size=0, q_batch=0, graphs=True, verbose=False):
+        _auto_init()
+        self._lib = load_library()
+        if len(stacks_list) != len(k_list) or len(stacks_list) == 0:
+            raise ValueError("need one k per instance and at least one instance")
+        ss = [_to_double_array(s, "stacks") for s in stacks_list]
+        for s, k in zip(ss, k_list):
+            _validate(len(s), int(k), 1)
+        self.n_off = np.zeros(len(ss) + 1, dtype=np.int32)
+        self.k_off = np.zeros(len(ss) + 1, dtype=np.int32)
+        self.n_off[1:] = np.cumsum([len(s) for s in ss])
+        self.k_off[1:] = np.cumsum([int(k) for k in k_list])
+        S = np.concatenate(ss)
+        opts = Options()
+        opts.enable_graphs = int(bool(graphs))
+        opts.force_uncached_fused_levels = int(block_size)
+        opts.force_uncached_cufft_levels = int(q_batch)
+        opts.verbose = int(bool(verbose))
+        self._h = self._lib.icm_hip_batch_plan_create(len(ss), _ptr(self.n_off, ctypes.c_int), _ptr(S),
+                                                      _ptr(self.k_off, ctypes.c_int), ctypes.byref(opts))
+        if not self._h:
+            raise _err(self._lib, "icm_hip_batch_plan_create")
+        self.n_inst = len(ss)
+        self.stats = RunStats()
+
+    def split(self, flat):
+        return [flat[self.n_off[i]:self.n_off[i + 1]] for i in range(self.n_inst)]
+
+    def run_flat(self, payout_flat, Q=256):
+        p = _to_double_array(payout_flat, "payouts")
+        if len(p) != self.k_off[-1]:
+            raise ValueError(f"flat payout length {len(p)} != {self.k_off[-1]}")
+        if Q < 1:
+            raise ValueError(f"Q must be positive, got {Q}")
+        out = np.zeros(int(self.n_off[-1]))
+        if self._lib.icm_hip_batch_equity(self._h, int(Q), _ptr(p), _ptr(out), ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_batch_equity")
+        return out
+
+    def run(self, payouts_list, Q=256):
+        if len(payouts_list) != self.n_inst:
+            raise ValueError("need one payout vector per instance")
+        ps = [_to_double_array(p, "payouts") for p in payouts_list]
+        for i, p in enumerate(ps):
+            if len(p) != self.k_off[i + 1] - self.k_off[i]:
+                raise ValueError(f"instance {i}: payout length {len(p)} != k")
+        return self.split(self.run_flat(np.concatenate(ps), Q))
+
+    def run_device(self, d_payout_ptr, d_equity_ptr, Q=256):
+        if self._lib.icm_hip_batch_equity_device(self._h, int(Q), ctypes.c_void_p(d_payout_ptr),
+                                                 ctypes.c_void_p(d_equity_ptr), ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_batch_equity_device")
+        return self.stats.total_ns
+
+    def summary(self):
+        vals = [ctypes.c_int(0) for _ in range(5)]
+        nbytes = ctypes.c_size_t(0)
+        if self._lib.icm_hip_batch_summary(self._h, *[ctypes.byref(v) for v in vals], ctypes.byref(nbytes)) != 0:
+            raise _err(self._lib, "icm_hip_batch_summary")
+        keys = ("n_inst", "n_players", "n_blocks", "max_levels", "q_batch")
+        out = {k: v.value for k, v in zip(keys, vals)}
+        out["device_bytes"] = nbytes.value
+        return out
+
+    def set_profiling(self, on=True):
+        if self._lib.icm_hip_batch_set_profiling(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_batch_set_profiling")
+
+    def kernel_stats(self):
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        for i in range(self._lib.icm_hip_batch_kernel_count(self._h)):
+            ln = ctypes.c_int(0)
+            ns, by, fl = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+            self._lib.icm_hip_batch_kernel_stat(self._h, i, buf, 64, ctypes.byref(ln), ctypes.byref(ns),
+                                                ctypes.byref(by), ctypes.byref(fl))
+            out[buf.value.decode()] = (ln.value, ns.value, by.value, fl.value)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.icm_hip_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def equity_batch(stacks_list, payouts_list, Q=256):
+    """One-shot ragged batch: list of equity arrays, one per instance."""
+    _auto_init()
+    S, P, n_off, k_off = _ragged(stacks_list, payouts_list)
+    if Q < 1:
+        raise ValueError(f"Q must be positive, got {Q}")
+    out = np.zeros(len(S))
+    lib = load_library()
+    if lib.icm_hip_equity_batch(len(n_off) - 1, _ptr(n_off, ctypes.c_int), _ptr(S), _ptr(k_off, ctypes.c_int),
+                                int(Q), _ptr(P), _ptr(out)) != 0:
+        raise _err(lib, "icm_hip_equity_batch")
+    return [out[n_off[i]:n_off[i + 1]] for i in range(len(n_off) - 1)]
 
 
 def plan_geometry(n, k, B=32, max_levels=64):

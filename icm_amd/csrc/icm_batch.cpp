@@ -1,0 +1,556 @@
+// icm_batch.cpp -- ragged batches: many independent ICM instances of uneven
+// n and k solved together.
+//
+// Every instance keeps its own tree (geometry of icm_host.cpp with one block
+// size B for the whole batch), but the work of all instances runs in one
+// launch per phase: leaf build, one launch per tree level for the build
+// (levels 1 .. Lmax-2) and for the propagate (Lmax-1 .. 1), leaf extract,
+// q-reduction and scatter.  A level launch is a flat list of work items
+// (instance job, first tile item); each 4096-point LDS tile is filled with
+// (q-plane, parent pair) items of one instance, so a small instance's level
+// costs a few tiles for all Q planes instead of one mostly empty tile per
+// plane.  The first run with a given Q records the launches as one HIP graph;
+// later runs replay it (one hipGraphLaunch per batch).
+//
+// Reference semantics per instance are those of icm_equity
+// (src/cpu/icm.h:26-41, src/cpu/icm.c:2523-2695); the batch entry itself has
+// no counterpart in the reference.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/icm_hip.h"
+#include "icm_common.hpp"
+#include "icm_kernels.hpp"
+
+using namespace icmh;
+
+struct IcmHipBatchPlan {
+    int ninst = 0, Ntot = 0, Ktot = 0, NB = 0, Lmax = 0, B = 32;
+    std::vector<int> n_off, k_off;
+    std::vector<Geometry> geo;
+    std::vector<int> perm;               // global sorted position -> flat original index
+    std::vector<double> S_sorted;        // per instance, descending, concatenated
+    std::vector<double> Smax;
+    std::vector<int> boff;               // first global block of each instance
+    std::vector<int> blk_inst;           // global block -> instance
+    int qb = 0;                          // q-planes per pass
+    int nchunk = 1;
+    int twShift = 12;
+    bool use_graph = true;
+    bool verbose = false;
+    // arena offsets (doubles) per level and instance, sized for qb planes
+    std::vector<std::vector<long long>> coff, goff;
+    std::vector<long long> coef_elems;   // per level
+    long long g_elems[2] = {0, 0};
+    // device memory
+    double2 *d_tw = nullptr;
+    double *d_S = nullptr;
+    int *d_perm = nullptr;
+    double *d_payout = nullptr;
+    double *d_nodes = nullptr;           // [ninst][2][Q]
+    int nodes_Q = 0;
+    icm::BatchInst *d_inst = nullptr;
+    int *d_blk_inst = nullptr;
+    double *d_blocks = nullptr;          // [qb][NB][B+1]
+    std::vector<double *> d_coef;        // level 1 .. Lmax-2
+    double *d_g[2] = {nullptr, nullptr};
+    double *d_partial = nullptr;         // [nchunk][Ntot]
+    double *d_acc = nullptr;             // [Ntot] sorted
+    double *d_equity = nullptr;          // [Ntot] original flat order
+    // launch schedule per q-pass size
+    struct Pass {
+        int nq = 0;
+        icm::LevelJob *d_jobs = nullptr;
+        int2 *d_items = nullptr;
+        // per level l: build items [boff_l, boff_l + bcnt_l), prop items likewise
+        std::vector<int> b_off, b_cnt, p_off, p_cnt;
+        std::vector<double> b_bytes, b_flops, p_bytes, p_flops;
+    };
+    std::vector<Pass> passes;            // index 0: nq = qb, 1: the remainder (if any)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_Q = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    size_t bytes = 0;
+    KernelProf prof;
+};
+
+namespace {
+
+template <class T>
+bool balloc(IcmHipBatchPlan *p, T **ptr, size_t count) {
+    if (count == 0) count = 1;
+    const size_t b = count * sizeof(T);
+    if (!hip_ok(hipMalloc((void **)ptr, b), "hipMalloc")) return false;
+    p->bytes += b;
+    return true;
+}
+
+void drop_graph(IcmHipBatchPlan *p) {
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->gexec = nullptr;
+    p->graph = nullptr;
+    p->graph_Q = 0;
+}
+
+void batch_free(IcmHipBatchPlan *p) {
+    if (!p) return;
+    drop_graph(p);
+    auto f = [](void *x) { if (x) (void)hipFree(x); };
+    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_inst); f(p->d_blk_inst);
+    f(p->d_blocks); for (double *c : p->d_coef) f(c);
+    f(p->d_g[0]); f(p->d_g[1]); f(p->d_partial); f(p->d_acc); f(p->d_equity);
+    for (auto &ps : p->passes) { f(ps.d_jobs); f(ps.d_items); }
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+// Level-l coefficients of instance i (l = 0: its blocks in the global array).
+icm::CoefView coef_view(const IcmHipBatchPlan *p, int i, int l) {
+    const Geometry &g = p->geo[i];
+    icm::CoefView v{};
+    if (l == 0) {
+        v.base = p->d_blocks + (long long)p->boff[i] * (p->B + 1);
+        v.plane = (long long)p->NB * (p->B + 1);
+        v.stride = p->B + 1;
+        v.count = g.c[0];
+    } else {
+        v.base = p->d_coef[l] + p->coff[l][i];
+        v.plane = (long long)g.n_real[l] * g.c[l];
+        v.stride = g.c[l];
+        v.count = g.c[l];
+    }
+    return v;
+}
+
+icm::CoefView g_view(const IcmHipBatchPlan *p, int i, int l) {
+    const Geometry &g = p->geo[i];
+    if (l == g.L - 1)   // root: the payout, shared by every q-plane
+        return icm::CoefView{p->d_payout + p->k_off[i], 0, g.k, g.k};
+    return icm::CoefView{p->d_g[l & 1] + p->goff[l][i], (long long)g.n_real[l] * g.G[l], g.G[l], g.G[l]};
+}
+
+double fft5(int F) { return 5.0 * F * (double)ilog2i(F); }
+
+// Job tables and work items for passes of nq q-planes.
+bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
+    ps.nq = nq;
+    const int Lm = p->Lmax;
+    std::vector<icm::LevelJob> jobs;
+    std::vector<int2> items;
+    ps.b_off.assign(Lm, 0); ps.b_cnt.assign(Lm, 0); ps.p_off.assign(Lm, 0); ps.p_cnt.assign(Lm, 0);
+    ps.b_bytes.assign(Lm, 0); ps.b_flops.assign(Lm, 0); ps.p_bytes.assign(Lm, 0); ps.p_flops.assign(Lm, 0);
+    auto add_items = [&](int job, int npair, int F) {
+        const long long nv = (long long)nq * npair;
+        const int per = icm::kMaxLdsF / F;
+        for (long long v0 = 0; v0 < nv; v0 += per) items.push_back(make_int2(job, (int)v0));
+    };
+    for (int l = 1; l <= Lm - 2; ++l) {
+        ps.b_off[l] = (int)items.size();
+        for (int i = 0; i < p->ninst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l > g.L - 2) continue;
+            icm::LevelJob J{};
+            J.child = coef_view(p, i, l - 1);
+            J.parent = icm::CoefOut{p->d_coef[l] + p->coff[l][i], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.np = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            const int npair = (g.n_real[l] + 1) / 2;
+            add_items((int)jobs.size(), npair, J.F);
+            jobs.push_back(J);
+            ps.b_bytes[l] += 8.0 * nq * ((double)g.n_real[l - 1] * g.c[l - 1] + (double)g.n_real[l] * g.c[l]);
+            ps.b_flops[l] += (double)nq * npair * 3.0 * fft5(J.F);
+        }
+        ps.b_cnt[l] = (int)items.size() - ps.b_off[l];
+    }
+    for (int l = Lm - 1; l >= 1; --l) {
+        ps.p_off[l] = (int)items.size();
+        for (int i = 0; i < p->ninst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l > g.L - 1) continue;
+            icm::LevelJob J{};
+            J.child = coef_view(p, i, l - 1);
+            J.gpar = g_view(p, i, l);
+            J.gchild = icm::CoefOut{p->d_g[(l - 1) & 1] + p->goff[l - 1][i],
+                                    (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.np = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            const int npair = (g.n_real[l] + 1) / 2;
+            add_items((int)jobs.size(), npair, J.F);
+            jobs.push_back(J);
+            const double gin = (l == g.L - 1) ? 8.0 * g.k : 8.0 * nq * (double)g.n_real[l] * g.G[l];
+            ps.p_bytes[l] += gin + 8.0 * nq * (double)g.n_real[l - 1] * (g.c[l - 1] + g.G[l - 1]);
+            ps.p_flops[l] += (double)nq * npair * 5.0 * fft5(J.F);
+        }
+        ps.p_cnt[l] = (int)items.size() - ps.p_off[l];
+    }
+    if (items.size() > 0x7fffffffULL) { set_err("batch too large: work items exceed 2^31"); return false; }
+    if (!balloc(p, &ps.d_jobs, jobs.size()) || !balloc(p, &ps.d_items, items.size())) return false;
+    return (jobs.empty() || HIP_OK(hipMemcpy(ps.d_jobs, jobs.data(), sizeof(icm::LevelJob) * jobs.size(),
+                                             hipMemcpyHostToDevice))) &&
+           (items.empty() || HIP_OK(hipMemcpy(ps.d_items, items.data(), sizeof(int2) * items.size(),
+                                              hipMemcpyHostToDevice)));
+}
+
+IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, const int *k_off,
+                              const IcmHipOptions *opts) {
+    if (n_inst < 1 || !n_off || !S || !k_off) { set_err("invalid arguments: need n_inst >= 1 and offsets"); return nullptr; }
+    if (n_off[0] != 0 || k_off[0] != 0) { set_err("offset arrays must start at 0"); return nullptr; }
+    for (int i = 0; i < n_inst; ++i) {
+        const long long n = (long long)n_off[i + 1] - n_off[i], k = (long long)k_off[i + 1] - k_off[i];
+        if (n < 1 || k < 1 || k > n) {
+            char b[128];
+            snprintf(b, sizeof b, "instance %d: need 1 <= k <= n (n=%lld, k=%lld)", i, n, k);
+            set_err(b);
+            return nullptr;
+        }
+    }
+    const int Ntot = n_off[n_inst];
+    for (int j = 0; j < Ntot; ++j)
+        if (!(S[j] > 0.0) || !std::isfinite(S[j])) { set_err("stacks must be positive and finite"); return nullptr; }
+    if (!ensure_device()) return nullptr;
+
+    IcmHipBatchPlan *p = new IcmHipBatchPlan();
+    p->ninst = n_inst;
+    p->Ntot = Ntot;
+    p->Ktot = k_off[n_inst];
+    p->n_off.assign(n_off, n_off + n_inst + 1);
+    p->k_off.assign(k_off, k_off + n_inst + 1);
+    p->verbose = opts && opts->verbose;
+    p->use_graph = !opts || opts->enable_graphs;
+    const int forcedB = opts ? opts->force_uncached_fused_levels : 0;
+    p->B = (forcedB == 16 || forcedB == 32 || forcedB == 64) ? forcedB : 32;
+    const int B = p->B;
+    p->geo.resize(n_inst);
+    p->boff.assign(n_inst, 0);
+    p->Smax.assign(n_inst, 0.0);
+    p->perm.resize(Ntot);
+    p->S_sorted.resize(Ntot);
+    int NB = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        const int n = n_off[i + 1] - n_off[i], k = k_off[i + 1] - k_off[i];
+        Geometry &g = p->geo[i];
+        if (!make_geometry(n, k, B, g)) { set_err("invalid geometry"); delete p; return nullptr; }
+        for (int l = 1; l < g.L; ++l) {
+            if (g.F[l] > icm::kMaxLdsF) {
+                char b[192];
+                snprintf(b, sizeof b,
+                         "instance %d (n=%d, k=%d) needs a level transform of %d points; ragged batches take "
+                         "instances whose levels fit %d (k <= 2048 or n <= 4096); use a single-instance plan",
+                         i, n, k, g.F[l], icm::kMaxLdsF);
+                set_err(b);
+                delete p;
+                return nullptr;
+            }
+        }
+        p->Lmax = std::max(p->Lmax, g.L);
+        p->boff[i] = NB;
+        NB += g.nb;
+        // stack-descending order, index tie-break (src/gpu/gpu_api.cu:492-502)
+        const int o = n_off[i];
+        std::vector<int> idx(n);
+        for (int j = 0; j < n; ++j) idx[j] = j;
+        std::sort(idx.begin(), idx.end(), [S, o](int a, int b) {
+            return S[o + a] > S[o + b] || (S[o + a] == S[o + b] && a < b);
+        });
+        for (int j = 0; j < n; ++j) {
+            p->perm[o + j] = o + idx[j];
+            p->S_sorted[o + j] = S[o + idx[j]];
+        }
+        p->Smax[i] = p->S_sorted[o];
+    }
+    p->NB = NB;
+    p->blk_inst.resize(NB);
+    for (int i = 0; i < n_inst; ++i)
+        for (int b = 0; b < p->geo[i].nb; ++b) p->blk_inst[p->boff[i] + b] = i;
+
+    // per-plane sizes and arena offsets (per instance, in planes of its own stride)
+    const int Lm = p->Lmax;
+    std::vector<long long> cplane(Lm, 0), gplane(Lm, 0);
+    for (int i = 0; i < n_inst; ++i) {
+        const Geometry &g = p->geo[i];
+        for (int l = 1; l <= g.L - 2; ++l) cplane[l] += (long long)g.n_real[l] * g.c[l];
+        for (int l = 0; l <= g.L - 2; ++l) gplane[l] += (long long)g.n_real[l] * g.G[l];
+    }
+    long long per_plane = (long long)NB * (B + 1);
+    for (int l = 1; l < Lm; ++l) per_plane += cplane[l];
+    long long gmax[2] = {0, 0};
+    for (int l = 0; l < Lm; ++l) gmax[l & 1] = std::max(gmax[l & 1], gplane[l]);
+    per_plane += gmax[0] + gmax[1];
+    size_t freeb = 0, totb = 0;
+    if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { delete p; return nullptr; }
+    const long long budget = (long long)(0.6 * (double)freeb);
+    int qb = (int)std::max<long long>(1, budget / std::max<long long>(8 * per_plane, 1));
+    qb = std::min(qb, 256);
+    const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
+    if (forcedQ > 0) qb = std::min(qb, forcedQ);
+    p->qb = qb;
+    p->coff.assign(Lm, std::vector<long long>(n_inst, 0));
+    p->goff.assign(Lm, std::vector<long long>(n_inst, 0));
+    p->coef_elems.assign(Lm, 0);
+    for (int l = 0; l < Lm; ++l) {
+        long long co = 0, go = 0;
+        for (int i = 0; i < n_inst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l >= 1 && l <= g.L - 2) { p->coff[l][i] = co; co += (long long)qb * g.n_real[l] * g.c[l]; }
+            if (l <= g.L - 2) { p->goff[l][i] = go; go += (long long)qb * g.n_real[l] * g.G[l]; }
+        }
+        p->coef_elems[l] = co;
+        p->g_elems[l & 1] = std::max(p->g_elems[l & 1], go);
+    }
+    const int bpw = 256 / B;
+    const int wg_blocks = (NB + bpw - 1) / bpw;
+    p->nchunk = std::min(std::max(1, 4096 / std::max(1, wg_blocks)), qb);
+
+    bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
+              HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1));
+    ok = ok && balloc(p, &p->d_tw, 4097) && balloc(p, &p->d_S, Ntot) && balloc(p, &p->d_perm, Ntot) &&
+         balloc(p, &p->d_payout, p->Ktot) && balloc(p, &p->d_inst, n_inst) && balloc(p, &p->d_blk_inst, NB) &&
+         balloc(p, &p->d_blocks, (size_t)qb * NB * (B + 1)) && balloc(p, &p->d_g[0], p->g_elems[0]) &&
+         balloc(p, &p->d_g[1], p->g_elems[1]) && balloc(p, &p->d_partial, (size_t)p->nchunk * Ntot) &&
+         balloc(p, &p->d_acc, Ntot) && balloc(p, &p->d_equity, Ntot);
+    p->d_coef.assign(Lm, nullptr);
+    for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_coef[l], p->coef_elems[l]);
+    if (!ok) { batch_free(p); return nullptr; }
+
+    // twiddles of the in-tile transforms (every level fits one tile)
+    std::vector<double2> tw(4097);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < 4096; ++m) {
+        const long double ang = -2.0L * PI * (long double)m / 4096.0L;
+        tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    tw[4096] = make_double2(1.0, 0.0);
+    p->twShift = 12;
+    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
+         HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * Ntot, hipMemcpyHostToDevice)) &&
+         HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * Ntot, hipMemcpyHostToDevice)) &&
+         HIP_OK(hipMemcpy(p->d_blk_inst, p->blk_inst.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+    if (!ok) { batch_free(p); return nullptr; }
+    if (p->verbose)
+        fprintf(stderr, "[icm_hip batch] instances=%d players=%d blocks=%d Lmax=%d B=%d qb=%d nchunk=%d bytes=%.3f GB\n",
+                n_inst, Ntot, NB, Lm, B, qb, p->nchunk, p->bytes / 1e9);
+    return p;
+}
+
+// Nodes for Q (per instance: its own S_max) and the instance table.
+bool prepare_Q(IcmHipBatchPlan *p, int Q) {
+    if (p->nodes_Q == Q) return true;
+    drop_graph(p);
+    for (auto &ps : p->passes) {
+        if (ps.d_jobs) (void)hipFree(ps.d_jobs);
+        if (ps.d_items) (void)hipFree(ps.d_items);
+    }
+    p->passes.clear();
+    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
+    p->nodes_Q = 0;
+    if (!balloc(p, &p->d_nodes, (size_t)p->ninst * 2 * Q)) return false;
+    std::vector<double> nodes((size_t)p->ninst * 2 * Q);
+    std::vector<icm::BatchInst> inst(p->ninst);
+    for (int i = 0; i < p->ninst; ++i) {
+        double *lv = nodes.data() + (size_t)i * 2 * Q;
+        make_nodes(Q, p->Smax[i], lv, lv + Q);
+        const Geometry &g = p->geo[i];
+        icm::BatchInst &I = inst[i];
+        I.logv = p->d_nodes + (size_t)i * 2 * Q;
+        I.w = I.logv + Q;
+        I.g0 = (g.L == 1) ? icm::CoefView{p->d_payout + p->k_off[i], 0, g.k, g.k} : g_view(p, i, 0);
+        I.n = g.n; I.k = g.k; I.nb = g.nb; I.L = g.L;
+        I.poff = p->n_off[i];
+        I.boff = p->boff[i];
+        I.G0 = g.G[0];
+    }
+    if (!HIP_OK(hipMemcpy(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice)) ||
+        !HIP_OK(hipMemcpy(p->d_inst, inst.data(), sizeof(icm::BatchInst) * inst.size(), hipMemcpyHostToDevice)))
+        return false;
+    const int nfull = Q / p->qb, rem = Q % p->qb;
+    p->passes.resize((nfull > 0 ? 1 : 0) + (rem > 0 ? 1 : 0));
+    int idx = 0;
+    if (nfull > 0 && !make_pass(p, p->qb, p->passes[idx++])) return false;
+    if (rem > 0 && !make_pass(p, rem, p->passes[idx++])) return false;
+    p->nodes_Q = Q;
+    return true;
+}
+
+// Enqueues the whole batch for Q nodes on the plan's stream (payout already
+// in d_payout); leaves equities in d_equity.
+bool enqueue(IcmHipBatchPlan *p, int Q) {
+    hipStream_t s = p->stream;
+    const int B = p->B, NB = p->NB;
+    for (int q0 = 0; q0 < Q; q0 += p->qb) {
+        const int nq = std::min(p->qb, Q - q0);
+        const IcmHipBatchPlan::Pass &ps = (nq == p->qb) ? p->passes[0] : p->passes.back();
+        const double lb_bytes = 8.0 * ((double)p->Ntot + (double)nq * NB * (B + 1));
+        if (!hip_ok(icm::launch_leaf_build_batch(B, p->d_inst, p->d_blk_inst, p->d_S, p->d_blocks, NB, nq, q0, s,
+                                                 &p->prof, lb_bytes),
+                    "batch leaf build"))
+            return false;
+        for (int l = 1; l <= p->Lmax - 2; ++l)
+            if (!hip_ok(icm::launch_level_batch(true, ps.d_jobs, ps.d_items + ps.b_off[l], ps.b_cnt[l], p->d_tw,
+                                                p->twShift, s, &p->prof, ps.b_bytes[l], ps.b_flops[l]),
+                        "batch build level"))
+                return false;
+        for (int l = p->Lmax - 1; l >= 1; --l)
+            if (!hip_ok(icm::launch_level_batch(false, ps.d_jobs, ps.d_items + ps.p_off[l], ps.p_cnt[l], p->d_tw,
+                                                p->twShift, s, &p->prof, ps.p_bytes[l], ps.p_flops[l]),
+                        "batch propagate level"))
+                return false;
+        const int nchunk = std::min(p->nchunk, nq);
+        const int qpc = (nq + nchunk - 1) / nchunk;
+        const int nch = (nq + qpc - 1) / qpc;
+        double g0 = 0.0;
+        for (int i = 0; i < p->ninst; ++i) g0 += (double)p->geo[i].nb * p->geo[i].G[0];
+        const double le_bytes = 8.0 * ((double)nq * (NB * (B + 1) + g0) + (double)p->Ntot * (1 + nch));
+        if (!hip_ok(icm::launch_leaf_extract_batch(B, p->d_inst, p->d_blk_inst, p->d_S, p->d_blocks, NB, nq, q0,
+                                                   nch, qpc, p->d_partial, p->Ntot, s, &p->prof, le_bytes),
+                    "batch leaf extract") ||
+            !hip_ok(icm::launch_finalize(p->d_partial, nch, p->Ntot, nullptr, p->d_acc, q0 > 0 ? 1 : 0, s, &p->prof),
+                    "finalize"))
+            return false;
+    }
+    return hip_ok(icm::launch_scatter(p->d_acc, p->Ntot, p->d_perm, p->d_equity, s, &p->prof), "scatter");
+}
+
+// Runs the batch for Q nodes with the payout already in d_payout.
+int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats) {
+    if (Q < 1) { set_err("Q must be positive"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!prepare_Q(p, Q)) return -1;
+    hipStream_t s = p->stream;
+    const bool graph = p->use_graph && !p->prof.enabled;
+    if (graph && p->graph_Q != Q) {
+        drop_graph(p);
+        if (!HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) return -1;
+        const bool ok = enqueue(p, Q);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        if (!ok) { if (gr) (void)hipGraphDestroy(gr); return -1; }
+        if (!hip_ok(e, "hipStreamEndCapture")) return -1;
+        p->graph = gr;
+        if (!HIP_OK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0))) { drop_graph(p); return -1; }
+        p->graph_Q = Q;
+    }
+    if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
+    if (graph) {
+        if (!HIP_OK(hipGraphLaunch(p->gexec, s))) return -1;
+    } else if (!enqueue(p, Q)) {
+        return -1;
+    }
+    if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (p->prof.enabled) p->prof.collect();
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+        memset(stats, 0, sizeof(*stats));
+        stats->total_ns = (double)ms * 1e6;
+        stats->peak_vram_bytes = p->bytes;
+        stats->engine = 1;
+        stats->B = p->B;
+        stats->uncached_fused_levels = 2 * p->Lmax - 3;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+IcmHipBatchPlan *icm_hip_batch_plan_create(int n_inst, const int *n_off, const double *S, const int *k_off,
+                                           const IcmHipOptions *opts) {
+    return batch_create(n_inst, n_off, S, k_off, opts);
+}
+
+void icm_hip_batch_plan_destroy(IcmHipBatchPlan *plan) { batch_free(plan); }
+
+int icm_hip_batch_equity(IcmHipBatchPlan *plan, int Q, const double *payout, double *equity,
+                         IcmHipRunStats *stats) {
+    if (!plan || !payout || !equity) { set_err("null argument"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!HIP_OK(hipMemcpyAsync(plan->d_payout, payout, sizeof(double) * plan->Ktot, hipMemcpyHostToDevice,
+                               plan->stream)))
+        return -1;
+    if (batch_run(plan, Q, stats) != 0) return -1;
+    return HIP_OK(hipMemcpy(equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost)) ? 0 : -1;
+}
+
+int icm_hip_batch_equity_device(IcmHipBatchPlan *plan, int Q, const double *d_payout, double *d_equity,
+                                IcmHipRunStats *stats) {
+    if (!plan || !d_payout || !d_equity) { set_err("null argument"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!HIP_OK(hipMemcpyAsync(plan->d_payout, d_payout, sizeof(double) * plan->Ktot, hipMemcpyDeviceToDevice,
+                               plan->stream)))
+        return -1;
+    if (batch_run(plan, Q, stats) != 0) return -1;
+    if (!HIP_OK(hipMemcpyAsync(d_equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice,
+                               plan->stream)))
+        return -1;
+    return HIP_OK(hipStreamSynchronize(plan->stream)) ? 0 : -1;
+}
+
+int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const int *k_off, int Q,
+                         const double *payout, double *equity) {
+    if (n_inst >= 1 && n_off && equity) memset(equity, 0, sizeof(double) * (size_t)std::max(0, n_off[n_inst]));
+    IcmHipBatchPlan *p = batch_create(n_inst, n_off, S, k_off, nullptr);
+    if (!p) return -1;
+    p->use_graph = false;   // one run: recording a graph would not pay
+    const int r = icm_hip_batch_equity(p, Q, payout, equity, nullptr);
+    batch_free(p);
+    return r;
+}
+
+int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->prof.enabled = on != 0;
+    plan->prof.agg.clear();
+    return 0;
+}
+
+int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan) { return plan ? (int)plan->prof.agg.size() : 0; }
+
+int icm_hip_batch_kernel_stat(const IcmHipBatchPlan *plan, int index, char *name, int name_len, int *launches,
+                              double *total_ns, double *alg_bytes, double *alg_flops) {
+    if (!plan || index < 0 || index >= (int)plan->prof.agg.size()) { set_err("bad kernel index"); return -1; }
+    const auto &a = plan->prof.agg[index];
+    if (name && name_len > 0) {
+        strncpy(name, a.name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (launches) *launches = a.launches;
+    if (total_ns) *total_ns = a.ns;
+    if (alg_bytes) *alg_bytes = a.bytes;
+    if (alg_flops) *alg_flops = a.flops;
+    return 0;
+}
+
+int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_players, int *n_blocks, int *max_levels,
+                          int *q_batch, size_t *device_bytes) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (n_inst) *n_inst = plan->ninst;
+    if (n_players) *n_players = plan->Ntot;
+    if (n_blocks) *n_blocks = plan->NB;
+    if (max_levels) *max_levels = plan->Lmax;
+    if (q_batch) *q_batch = plan->qb;
+    if (device_bytes) *device_bytes = plan->bytes;
+    return 0;
+}
+
+}  // extern "C"

@@ -70,6 +70,34 @@ __global__ __launch_bounds__(64) void k_leaf_build(LeafJob J) {
 // One lane per player; the lane loops over its chunk of q-planes and keeps the
 // sum in a register, so the q-sum order is fixed (no atomics).
 // ---------------------------------------------------------------------------
+// Psi of one player of a block of bsize players: sum_{m<pk} g[m] (P/f)[m],
+// f = a + (1-a) x, dividing in the stable direction.
+template <int B>
+__device__ __forceinline__ double leaf_psi(const double *P, const double *g, double a, int bsize, int pk) {
+    const double bm = 1.0 - a;
+    double eq = 0.0;
+    if (a > 0.5) {
+        const double ia = 1.0 / a, c = -bm * ia;
+        double qv = P[0] * ia;
+        eq = g[0] * qv;
+        for (int m = 1; m < pk; ++m) {
+            qv = c * qv + P[m] * ia;
+            eq += g[m] * qv;
+        }
+    } else if (a > 1e-15) {
+        const double ib = 1.0 / bm, c = -a * ib;
+        double qv = P[bsize] * ib;
+        if (bsize - 1 < pk) eq += g[bsize - 1] * qv;
+        for (int m = bsize - 2; m >= 0; --m) {
+            qv = c * qv + P[m + 1] * ib;
+            if (m < pk) eq += g[m] * qv;
+        }
+    } else {
+        for (int m = 0; m < pk; ++m) eq += g[m] * P[m + 1];
+    }
+    return eq;
+}
+
 template <int B>
 __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     constexpr int BPW = 256 / B;
@@ -106,29 +134,7 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
         if (act) {
             const double lv = J.logv[q];
             const double a = J.a_in ? J.a_in[(long long)q * J.n + j] : a_of(Sj, lv);
-            const double bm = 1.0 - a;
-            const double *P = shP + lb * (B + 1);
-            const double *g = shG + lb * B;
-            double eq = 0.0;
-            if (a > 0.5) {
-                const double ia = 1.0 / a, c = -bm * ia;
-                double qv = P[0] * ia;
-                eq = g[0] * qv;
-                for (int m = 1; m < pk; ++m) {
-                    qv = c * qv + P[m] * ia;
-                    eq += g[m] * qv;
-                }
-            } else if (a > 1e-15) {
-                const double ib = 1.0 / bm, c = -a * ib;
-                double qv = P[bsize] * ib;
-                if (bsize - 1 < pk) eq += g[bsize - 1] * qv;
-                for (int m = bsize - 2; m >= 0; --m) {
-                    qv = c * qv + P[m + 1] * ib;
-                    if (m < pk) eq += g[m] * qv;
-                }
-            } else {
-                for (int m = 0; m < pk; ++m) eq += g[m] * P[m + 1];
-            }
+            const double eq = leaf_psi<B>(shP + lb * (B + 1), shG + lb * B, a, bsize, pk);
             if (J.inner_out) J.inner_out[(long long)q * J.n + j] = eq;
             double pw = J.wq[q] * Sj * a * exp(-lv);
             if (!isfinite(pw)) pw = 0.0;
@@ -137,6 +143,101 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
         __syncthreads();
     }
     if (valid && J.partial) J.partial[(long long)chunk * J.n + j] = act ? acc : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Ragged-batch leaves.  Blocks of all instances are numbered globally
+// (instance i owns blocks boff .. boff + nb - 1, players poff .. poff + n - 1
+// of the concatenated sorted stacks); blk_inst maps a block to its instance.
+// Block products live at blocks[q][global block][B+1].
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__restrict__ inst,
+                                                         const int *__restrict__ blk_inst,
+                                                         const double *__restrict__ S, double *blocks, int NB,
+                                                         int q0) {
+    __shared__ double sh[64 * (B + 1)];
+    const int lane = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int b0 = blockIdx.x * 64;
+    const int gb = b0 + lane;
+    double P[B + 1];
+    P[0] = 1.0;
+#pragma unroll
+    for (int m = 1; m <= B; ++m) P[m] = 0.0;
+    if (gb < NB) {
+        const BatchInst &I = inst[blk_inst[gb]];
+        const double lv = I.logv[q0 + qi];
+        const int b = gb - I.boff;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int j = b * B + i;
+            const double a = (j < I.n) ? a_of(S[I.poff + j], lv) : 1.0;
+            const double bb = 1.0 - a;
+#pragma unroll
+            for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
+            P[0] *= a;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
+    __syncthreads();
+    const int nblk = min(64, NB - b0);
+    double *out = blocks + (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1);
+    for (int e = lane; e < nblk * (B + 1); e += 64) out[e] = sh[e];
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__restrict__ inst,
+                                                            const int *__restrict__ blk_inst,
+                                                            const double *__restrict__ S,
+                                                            const double *__restrict__ blocks, int NB, int nq,
+                                                            int q0, int q_per_chunk, double *partial, int Ntot) {
+    constexpr int BPW = 256 / B;
+    __shared__ double shP[BPW * (B + 1)];
+    __shared__ double shG[BPW * B];
+    const int t = threadIdx.x;
+    const int lb = t / B, li = t % B;
+    const int gfirst = blockIdx.x * BPW;
+    const int gb = gfirst + lb;
+    const int chunk = blockIdx.y;
+    const int qa = chunk * q_per_chunk;
+    const int qe = min(qa + q_per_chunk, nq);
+    const BatchInst *I = (gb < NB) ? &inst[blk_inst[gb]] : nullptr;
+    const int b = I ? gb - I->boff : 0;
+    const int j = b * B + li;
+    const bool valid = I && j < I->n;
+    const int bsize = I ? min(B, I->n - b * B) : 0;
+    const int pk = I ? min(I->G0, bsize) : 0;
+    const double Sj = valid ? S[I->poff + j] : 0.0;
+    const long long bplane = (long long)NB * (B + 1);
+    double acc = 0.0;
+    for (int q = qa; q < qe; ++q) {
+        for (int e = t; e < BPW * (B + 1); e += 256) {
+            const int g2 = gfirst + e / (B + 1);
+            shP[e] = (g2 < NB) ? blocks[q * bplane + (long long)g2 * (B + 1) + e % (B + 1)] : 0.0;
+        }
+        for (int e = t; e < BPW * B; e += 256) {
+            const int g2 = gfirst + e / B, m = e % B;
+            double v = 0.0;
+            if (g2 < NB) {
+                const BatchInst &I2 = inst[blk_inst[g2]];
+                if (m < I2.G0) v = I2.g0.base[q * I2.g0.plane + (long long)(g2 - I2.boff) * I2.g0.stride + m];
+            }
+            shG[e] = v;
+        }
+        __syncthreads();
+        if (valid) {
+            const double lv = I->logv[q0 + q];
+            const double a = a_of(Sj, lv);
+            const double eq = leaf_psi<B>(shP + lb * (B + 1), shG + lb * B, a, bsize, pk);
+            double pw = I->w[q0 + q] * Sj * a * exp(-lv);
+            if (!isfinite(pw)) pw = 0.0;
+            acc += pw * eq;
+        }
+        __syncthreads();
+    }
+    if (valid) partial[(long long)chunk * Ntot + I->poff + j] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -425,18 +526,39 @@ __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int 
 // LDS-level kernels: 512 threads, 8 tile points per thread (radix-8 passes).
 constexpr int kLT = 512;
 constexpr int kLP = kTile / kLT;
+// Largest padded tile (F = 16): LDS of the kernels that switch over F.
+constexpr int kTileMaxSize = TileLayout<16>::SIZE;
 
-template <int F>
-__global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
+// Tile item -> (q-plane, pair of parents).
+struct PlainMap {   // one q-plane per workgroup, consecutive pairs
+    int pair0, qi;
+    __device__ __forceinline__ void at(int item, int &q, int &pair) const {
+        q = qi;
+        pair = pair0 + item;
+    }
+};
+struct FlatMap {    // v = q * npair + pair over (q-plane, pair), from v0
+    int v0, npair, nv;
+    __device__ __forceinline__ void at(int item, int &q, int &pair) const {
+        const int v = v0 + item;
+        if (v < nv) {
+            q = v / npair;
+            pair = v - q * npair;
+        } else {
+            q = 0;
+            pair = 1 << 29;   // past every parent: loads give 0, nothing is stored
+        }
+    }
+};
+
+template <int F, class Map>
+__device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
+                                               int twShift) {
     using TL = TileLayout<F>;
-    __shared__ double2 lds[TL::SIZE];
-    constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
-    const int qi = blockIdx.y;
-    const int pair0 = blockIdx.x * ITEMS;
 
     double2 P1[kLP / 2];
 #pragma unroll
@@ -445,7 +567,9 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[TL::at(item, m)] = load_child_pair(J, qi, 2 * (pair0 + item) + which, m);
+            int qi, pr;
+            M.at(item, qi, pr);
+            lds[TL::at(item, m)] = load_child_pair(J, qi, 2 * pr + which, m);
         }
         fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
@@ -471,13 +595,15 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
     }
     fft_tile<F, true, kLT>(lds, tw, twShift);
     const double invF = 1.0 / (double)F;
-    double *pb = J.parent.base + (long long)qi * J.parent.plane;
     const int ps_ = J.parent.stride, pc = J.parent.count;
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = 2 * (pair0 + item);
+        int qi, pr;
+        M.at(item, qi, pr);
+        const int p = 2 * pr;
+        double *pb = J.parent.base + (long long)qi * J.parent.plane;
         if (m < pc) {
             const double2 v = lds[TL::at(item, m)];
             if (p < J.nr_parent)
@@ -495,6 +621,12 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
     }
 }
 
+template <int F>
+__global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[TileLayout<F>::SIZE];
+    build_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+}
+
 // ---------------------------------------------------------------------------
 // Propagate, LDS-resident (F <= 4096): for parent p with children L, R,
 //   gL[m] = sum_t R[t] g[m+t],  gR[m] = sum_t L[t] g[m+t],  m < G_child,
@@ -503,25 +635,24 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
 // each parent's children share another; the two outputs of a parent share
 // the inverse (Re = gL, Im = gR).
 // ---------------------------------------------------------------------------
-template <int F>
-__global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
+template <int F, class Map>
+__device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
+                                              int twShift) {
     using TL = TileLayout<F>;
-    __shared__ double2 lds[TL::SIZE];
-    constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
-    const int qi = blockIdx.y;
-    const int pair0 = blockIdx.x * ITEMS;
-    const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
 
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int pa = slot_parent(J, 2 * (pair0 + item)), pb = slot_parent(J, 2 * (pair0 + item) + 1);
+        int qi, pr;
+        M.at(item, qi, pr);
+        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+        const int pa = slot_parent(J, 2 * pr), pb = slot_parent(J, 2 * pr + 1);
         double2 z = make_double2(0.0, 0.0);
         if (m < gcnt) {
             if (pa >= 0) z.x = gb[(long long)pa * gs + m];
@@ -546,7 +677,6 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
     }
     __syncthreads();
     const double invF = 1.0 / (double)F;
-    double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
     const int gcs = J.gchild.stride, gcc = J.gchild.count;
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
@@ -554,7 +684,9 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            lds[TL::at(item, m)] = load_child_pair(J, qi, slot_parent(J, 2 * (pair0 + item) + which), m);
+            int qi, pr;
+            M.at(item, qi, pr);
+            lds[TL::at(item, m)] = load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
         }
         fft_tile<F, false, kLT>(lds, tw, twShift);
 #pragma unroll
@@ -579,8 +711,11 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
             const int item = e >> LOGF, m = e & (F - 1);
-            const int p = slot_parent(J, 2 * (pair0 + item) + which);
+            int qi, pr;
+            M.at(item, qi, pr);
+            const int p = slot_parent(J, 2 * pr + which);
             if (p >= 0 && m < gcc) {
+                double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
                 const double2 v = lds[TL::at(item, m)];
                 double vl = v.x * invF, vr = v.y * invF;
                 if (J.gfix && m == J.child.count - 1) {
@@ -593,6 +728,59 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
         }
         __syncthreads();
     }
+}
+
+template <int F>
+__global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[TileLayout<F>::SIZE];
+    prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+}
+
+// ---------------------------------------------------------------------------
+// Ragged batches: one launch per tree level over every instance that has the
+// level.  Workgroup w takes work item items[w] = (job, v0): instance job's
+// level, tile items v0 .. v0 + 4096/F - 1 over its (q-plane, pair) space, so
+// small instances fill tiles with q-planes instead of leaving them empty.
+// F is uniform per workgroup (one job), so the switch does not diverge.
+// ---------------------------------------------------------------------------
+#define ICM_F_CASES(BODY)                                                     \
+    switch (J.F) {                                                            \
+    case 16: BODY(16); break;                                                 \
+    case 32: BODY(32); break;                                                 \
+    case 64: BODY(64); break;                                                 \
+    case 128: BODY(128); break;                                               \
+    case 256: BODY(256); break;                                               \
+    case 512: BODY(512); break;                                               \
+    case 1024: BODY(1024); break;                                             \
+    case 2048: BODY(2048); break;                                             \
+    case 4096: BODY(4096); break;                                             \
+    default: break;                                                           \
+    }
+
+__global__ __launch_bounds__(512) void k_build_lds_batch(const LevelJob *__restrict__ jobs,
+                                                         const int2 *__restrict__ items, const double2 *tw,
+                                                         int twShift) {
+    __shared__ double2 lds[kTileMaxSize];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const int npair = (J.nr_parent + 1) >> 1;
+    const FlatMap M{it.y, npair, J.nq * npair};
+#define ICM_BUILD_F(FF) build_lds_body<FF>(J, lds, M, tw, twShift)
+    ICM_F_CASES(ICM_BUILD_F)
+#undef ICM_BUILD_F
+}
+
+__global__ __launch_bounds__(512) void k_prop_lds_batch(const LevelJob *__restrict__ jobs,
+                                                        const int2 *__restrict__ items, const double2 *tw,
+                                                        int twShift) {
+    __shared__ double2 lds[kTileMaxSize];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const int npair = (J.np + 1) >> 1;
+    const FlatMap M{it.y, npair, J.nq * npair};
+#define ICM_PROP_F(FF) prop_lds_body<FF>(J, lds, M, tw, twShift)
+    ICM_F_CASES(ICM_PROP_F)
+#undef ICM_PROP_F
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,6 +1615,63 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StGChildren>", F1), 16.0 * nq * nrp * F + 8.0 * nq * J.nr_child * J.gchild.count, nq * nrp * kF2 * fft_flops(F1));
     return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Ragged-batch launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_leaf_build_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                   double *blocks, int NB, int nq, int q0, hipStream_t s, Prof *pf, double bytes) {
+    if (NB < 1 || nq < 1) return hipSuccess;
+    PF_BEGIN();
+    dim3 grid((NB + 63) / 64, nq);
+    switch (B) {
+    case 16: k_leaf_build_batch<16><<<grid, 64, 0, s>>>(inst, blk_inst, S, blocks, NB, q0); break;
+    case 32: k_leaf_build_batch<32><<<grid, 64, 0, s>>>(inst, blk_inst, S, blocks, NB, q0); break;
+    case 64: k_leaf_build_batch<64><<<grid, 64, 0, s>>>(inst, blk_inst, S, blocks, NB, q0); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    PF_END("k_leaf_build_batch", bytes, 0.0);
+    return e;
+}
+
+hipError_t launch_leaf_extract_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                     const double *blocks, int NB, int nq, int q0, int nchunk, int q_per_chunk,
+                                     double *partial, int Ntot, hipStream_t s, Prof *pf, double bytes) {
+    if (NB < 1 || nq < 1) return hipSuccess;
+    PF_BEGIN();
+    const int bpw = 256 / B;
+    dim3 grid((NB + bpw - 1) / bpw, nchunk);
+    switch (B) {
+    case 16:
+        k_leaf_extract_batch<16><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial,
+                                                     Ntot);
+        break;
+    case 32:
+        k_leaf_extract_batch<32><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial,
+                                                     Ntot);
+        break;
+    case 64:
+        k_leaf_extract_batch<64><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial,
+                                                     Ntot);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    PF_END("k_leaf_extract_batch", bytes, 0.0);
+    return e;
+}
+
+hipError_t launch_level_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, const double2 *tw,
+                              int twShift, hipStream_t s, Prof *pf, double bytes, double flops) {
+    if (nitems < 1) return hipSuccess;
+    PF_BEGIN();
+    if (build) k_build_lds_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
+    else k_prop_lds_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
+    hipError_t e = hipGetLastError();
+    PF_END(build ? "k_build_lds_batch" : "k_prop_lds_batch", bytes, flops);
+    return e;
 }
 
 }  // namespace icm

@@ -84,6 +84,17 @@ struct VjpLeafJob {
     double *h_out;          // optional [nq][n] H_j values
 };
 
+// One instance of a ragged batch (device table, one entry per instance).
+struct BatchInst {
+    const double *logv;     // nodes of the instance, indexed by absolute q
+    const double *w;
+    CoefView g0;            // g at the leaves (the payout, plane 0, when L == 1)
+    int n, k, nb, L;
+    int poff;               // first player in the concatenated sorted stacks
+    int boff;               // first block in the global block numbering
+    int G0;                 // g entries per block
+};
+
 // Optional per-launch timing: when on, every kernel launch is bracketed by
 // two events on the launch stream and tagged with its compulsory HBM bytes
 // and FLOPs (inputs read once + outputs written once; 5 N log2 N per complex
@@ -124,6 +135,18 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
                               double2 *bufB, hipStream_t s, Prof *pf);
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
                              double2 *bufB, hipStream_t s, Prof *pf);
+
+// Ragged batches.  Leaves: global blocks [q][NB][B+1] for q-planes
+// q0 .. q0 + nq - 1; extract writes partial[chunk][Ntot].  Levels: one
+// launch per tree level over nitems work items (job, first tile item);
+// bytes / flops are the launch's compulsory totals, for the profiler.
+hipError_t launch_leaf_build_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                   double *blocks, int NB, int nq, int q0, hipStream_t s, Prof *pf, double bytes);
+hipError_t launch_leaf_extract_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                     const double *blocks, int NB, int nq, int q0, int nchunk, int q_per_chunk,
+                                     double *partial, int Ntot, hipStream_t s, Prof *pf, double bytes);
+hipError_t launch_level_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, const double2 *tw,
+                              int twShift, hipStream_t s, Prof *pf, double bytes, double flops);
 
 constexpr int kMaxLdsF = 4096;
 

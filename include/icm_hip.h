@@ -168,6 +168,39 @@ int icm_hip_plan_kernel_count(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
                              double *total_ns, double *alg_bytes, double *alg_flops);
 
+/* ---- ragged batches (the batch entry with offset arrays that this engine
+ * adds beside the reference's single-instance API; no reference counterpart)
+ * n_inst independent instances.  Instance i owns players n_off[i] ..
+ * n_off[i+1]-1 of S and payout entries k_off[i] .. k_off[i+1]-1 of payout
+ * (n_off[0] = k_off[0] = 0, 1 <= k_i <= n_i); equities come back in the flat
+ * order of S.  Each instance gets what icm_equity(n_i, S_i, Q, payout_i,
+ * k_i, equity_i) computes, with its own nodes (its own max stack).  Every
+ * tree level of every instance must fit one LDS transform (<= 4096 points:
+ * k_i <= 2048, or n_i <= 4096); other instances are rejected at plan
+ * creation with a message naming them.  opts: NULL or enable_graphs != 0
+ * records the first run for each Q as one HIP graph and replays it;
+ * force_uncached_fused_levels = B, force_uncached_cufft_levels = q-planes
+ * per pass, verbose.  Calls on one batch plan must not overlap. */
+typedef struct IcmHipBatchPlan IcmHipBatchPlan;
+IcmHipBatchPlan *icm_hip_batch_plan_create(int n_inst, const int *n_off, const double *S, const int *k_off,
+                                           const IcmHipOptions *opts);
+void icm_hip_batch_plan_destroy(IcmHipBatchPlan *plan);
+int icm_hip_batch_equity(IcmHipBatchPlan *plan, int Q, const double *payout, double *equity,
+                         IcmHipRunStats *stats);
+/* d_payout[K] and d_equity[N] are device pointers (K = k_off[n_inst], N = n_off[n_inst]). */
+int icm_hip_batch_equity_device(IcmHipBatchPlan *plan, int Q, const double *d_payout, double *d_equity,
+                                IcmHipRunStats *stats);
+/* One-shot: plan, run (no graph), free. */
+int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const int *k_off, int Q,
+                         const double *payout, double *equity);
+int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_players, int *n_blocks,
+                          int *max_levels, int *q_batch, size_t *device_bytes);
+/* Per-kernel timing of batch runs (runs eagerly, without the graph, while on). */
+int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on);
+int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan);
+int icm_hip_batch_kernel_stat(const IcmHipBatchPlan *plan, int index, char *name, int name_len, int *launches,
+                              double *total_ns, double *alg_bytes, double *alg_flops);
+
 #ifdef __cplusplus
 }
 #endif
