@@ -96,6 +96,76 @@ def cpu_baseline(n, k, Q, S, p, nthreads):
                       f"({q_hi - q_lo} nodes, one per thread, {dt:.1f} s), extrapolated to Q={Q}"}
 
 
+# ragged batch: 4096 independent instances (n log-uniform [2, 4096],
+# k in {1, 3, 9, ceil(n/10), n}), one graph launch per batch, against
+# looping the plain drop-in (icm_hip_equity per instance) on a sample
+def batched_field(icm_amd, torch, dev, Q, reps=5, sample=256):
+    from icm_amd.workloads import ragged_instances
+    stacks, pays = ragged_instances(4096, seed=4)
+    ks = [len(x) for x in pays]
+    t_c0 = time.perf_counter()
+    bp = icm_amd.BatchPlan(stacks, ks)
+    create_ms = (time.perf_counter() - t_c0) * 1e3
+    dP = torch.tensor(np.concatenate(pays), dtype=torch.float64, device=dev)
+    dE = torch.empty(int(bp.n_off[-1]), dtype=torch.float64, device=dev)
+    bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)          # records the graph
+    torch.cuda.synchronize()
+    t_a = time.perf_counter()
+    for _ in range(reps):
+        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t_a) * 1e3 / reps
+    bp.set_profiling(True)
+    bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+    bst = bp.kernel_stats()
+    bp.set_profiling(False)
+    summ_b = bp.summary()
+    bp.close()
+    rng = np.random.default_rng(9)
+    idx = rng.choice(len(stacks), sample, replace=False)
+    icm_amd.equity(stacks[idx[0]], pays[idx[0]], Q)
+    t_l = time.perf_counter()
+    for i in idx:
+        icm_amd.equity(stacks[i], pays[i], Q)
+    loop_s = time.perf_counter() - t_l
+    loop_inst_s = sample / loop_s
+    inst_s = len(stacks) / (batch_ms * 1e-3)
+    kern = {k_: {"launches": v[0], "ms": v[1] / 1e6,
+                 "GBps": (v[2] / (v[1] * 1e-9) / 1e9) if v[1] > 0 else 0.0}
+            for k_, v in sorted(bst.items(), key=lambda kv: -kv[1][1])}
+    return {"instances": len(stacks), "players": summ_b["n_players"], "Q": Q,
+            "ms_per_batch": batch_ms, "instances_per_s": inst_s,
+            "loop_dropin_instances_per_s": loop_inst_s,
+            "loop_sample": f"icm_hip_equity on {sample} random instances of the batch",
+            "speedup_vs_loop": inst_s / loop_inst_s, "plan_create_ms": create_ms,
+            "device_bytes": summ_b["device_bytes"], "kernels_eager": kern}
+
+
+# forward + adjoint: one pass returns E and dS = d(v.E)/dS; ratio to the forward
+def fwd_vjp_ratio(icm_amd, torch, dev, Q, Sx, px, reps):
+    pl = icm_amd.Plan(Sx, len(px))
+    dp = torch.tensor(px, dtype=torch.float64, device=dev)
+    de = torch.empty(len(Sx), dtype=torch.float64, device=dev)
+    dd = torch.empty(len(Sx), dtype=torch.float64, device=dev)
+    dv = torch.tensor(np.random.default_rng(1).normal(size=len(Sx)), dtype=torch.float64, device=dev)
+    pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
+    pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t_a = time.perf_counter()
+    for _ in range(reps):
+        pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t_b = time.perf_counter()
+    for _ in range(reps):
+        pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t_c = time.perf_counter()
+    pl.close()
+    fwd_ms = (t_b - t_a) * 1e3 / reps
+    vjp_ms = (t_c - t_b) * 1e3 / reps
+    return {"fwd_ms": fwd_ms, "fwd_plus_vjp_ms": vjp_ms, "ratio": vjp_ms / fwd_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +176,11 @@ def main():
     ap.add_argument("--Q", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--only", choices=["all", "forward", "batched"], default="all",
+                    help="forward: the headline pass only (for profiler runs); batched: the ragged batch only")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a plain copy kernel of known size, "
+                         "the scale of profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -120,10 +195,25 @@ def main():
     Q = args.Q
     S, p = workload(n, k)
     icm_amd.init()
+    dev = torch.device("cuda:0")
+    if args.calib_copy_mib > 0:
+        cnt = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(cnt, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {cnt * 8} bytes read, {cnt * 8} bytes written")
+        del src, dst
+    if args.only == "batched":
+        batched = batched_field(icm_amd, torch, dev, Q)
+        log(f"batched: {batched}")
+        print(json.dumps({"metric": "instances/s, ragged batch of 4096 (config 4)",
+                          "value": batched["instances_per_s"], "unit": "instances/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f64", "batched": batched}), flush=True)
+        return
     plan = icm_amd.Plan(S, k)
     summ = plan.summary()
     log(f"plan: {summ}")
-    dev = torch.device("cuda:0")
     d_p = torch.tensor(p, dtype=torch.float64, device=dev)
     d_e = torch.empty(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
@@ -154,12 +244,13 @@ def main():
     hbm_frac = bytes_per_launch / avg_s / (HBM_PEAK_GBS * 1e9)
     fp_frac = flops_per_launch / avg_s / (FP64_PEAK_TFLOPS * 1e12)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_fwd_n1048576_k1048576.json")
-    if os.path.isfile(tfile):
-        try:
-            traffic = json.load(open(tfile)).get("per_launch_bytes", {}).get(name)
-        except Exception:
-            traffic = None
+    # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # this workload (scaled by a 1 GiB copy), committed as a profile summary
+    tfile = os.path.join(ROOT, "profiles", f"traffic_fwd_n{n}_k{k}.md")
+    if os.path.isfile(tfile) and Q == 256:
+        for line in open(tfile):
+            if line.startswith("per_launch_bytes_json:"):
+                traffic = json.loads(line.split(":", 1)[1]).get(name)
     if hbm_frac >= fp_frac:
         roof = {"bound": "hbm", "achieved": bytes_per_launch / avg_s / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": hbm_frac}
@@ -178,100 +269,34 @@ def main():
                for k_, v in sorted(kst.items(), key=lambda kv: -kv[1][1])}
     log("kernels (per step): " + json.dumps(kernels, indent=1))
 
-    # subset variant of the same workload: 64 random target players, pruned tree
-    rng = np.random.default_rng(11)
-    targets = rng.choice(n, 64, replace=False)
-    plan.set_targets(targets)
-    plan.run_device(d_p.data_ptr(), d_e.data_ptr(), Q)
-    torch.cuda.synchronize()
-    ts0 = time.perf_counter()
-    for i in range(args.steps):
+    subset = adjoint = batched = None
+    if args.only == "all":
+        # subset variant of the same workload: 64 random target players, pruned tree
+        rng = np.random.default_rng(11)
+        targets = rng.choice(n, 64, replace=False)
+        plan.set_targets(targets)
         plan.run_device(d_p.data_ptr(), d_e.data_ptr(), Q)
-    torch.cuda.synchronize()
-    sub_ms = (time.perf_counter() - ts0) * 1e3 / args.steps
-    plan.set_targets(None)
-    subset = {"targets": 64, "ms_per_step": sub_ms, "speedup_vs_full": ms / sub_ms}
-    log(f"subset (64 targets): {sub_ms:.2f} ms/step")
+        torch.cuda.synchronize()
+        ts0 = time.perf_counter()
+        for i in range(args.steps):
+            plan.run_device(d_p.data_ptr(), d_e.data_ptr(), Q)
+        torch.cuda.synchronize()
+        sub_ms = (time.perf_counter() - ts0) * 1e3 / args.steps
+        plan.set_targets(None)
+        subset = {"targets": 64, "ms_per_step": sub_ms, "speedup_vs_full": ms / sub_ms}
+        log(f"subset (64 targets): {sub_ms:.2f} ms/step")
 
-    # forward + adjoint: one pass returns E and dS = d(v.E)/dS; ratio to the forward
-    def fwd_vjp_ratio(Sx, px, reps):
-        pl = icm_amd.Plan(Sx, len(px))
-        dp = torch.tensor(px, dtype=torch.float64, device=dev)
-        de = torch.empty(len(Sx), dtype=torch.float64, device=dev)
-        dd = torch.empty(len(Sx), dtype=torch.float64, device=dev)
-        dv = torch.tensor(np.random.default_rng(1).normal(size=len(Sx)), dtype=torch.float64, device=dev)
-        pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
-        pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
-        torch.cuda.synchronize()
-        t_a = time.perf_counter()
-        for _ in range(reps):
-            pl.run_device(dp.data_ptr(), de.data_ptr(), Q)
-        torch.cuda.synchronize()
-        t_b = time.perf_counter()
-        for _ in range(reps):
-            pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
-        torch.cuda.synchronize()
-        t_c = time.perf_counter()
-        pl.close()
-        fwd_ms = (t_b - t_a) * 1e3 / reps
-        vjp_ms = (t_c - t_b) * 1e3 / reps
-        return {"fwd_ms": fwd_ms, "fwd_plus_vjp_ms": vjp_ms, "ratio": vjp_ms / fwd_ms}
-
-    r5 = np.random.default_rng(5)
-    S5 = r5.lognormal(0.0, 1.5, 131072)
-    p5 = np.arange(1000, 0, -1, dtype=np.float64)
-    p5 /= p5.sum()
-    adjoint = {"n131072_k1000": fwd_vjp_ratio(S5, p5, 3)}
+        r5 = np.random.default_rng(5)
+        S5 = r5.lognormal(0.0, 1.5, 131072)
+        p5 = np.arange(1000, 0, -1, dtype=np.float64)
+        p5 /= p5.sum()
+        adjoint = {"n131072_k1000": fwd_vjp_ratio(icm_amd, torch, dev, Q, S5, p5, 3)}
     plan.close()
-    adjoint["n1048576_k1048576"] = fwd_vjp_ratio(S, p, 2)
-    log(f"adjoint: {adjoint}")
-
-    # ragged batch: 4096 independent instances (n log-uniform [2, 4096],
-    # k in {1, 3, 9, ceil(n/10), n}), one graph launch per batch, against
-    # looping the plain drop-in (icm_hip_equity per instance) on a sample
-    def batched_field(reps=5, sample=256):
-        from icm_amd.workloads import ragged_instances
-        stacks, pays = ragged_instances(4096, seed=4)
-        ks = [len(x) for x in pays]
-        t_c0 = time.perf_counter()
-        bp = icm_amd.BatchPlan(stacks, ks)
-        create_ms = (time.perf_counter() - t_c0) * 1e3
-        dP = torch.tensor(np.concatenate(pays), dtype=torch.float64, device=dev)
-        dE = torch.empty(int(bp.n_off[-1]), dtype=torch.float64, device=dev)
-        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)          # records the graph
-        torch.cuda.synchronize()
-        t_a = time.perf_counter()
-        for _ in range(reps):
-            bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
-        torch.cuda.synchronize()
-        batch_ms = (time.perf_counter() - t_a) * 1e3 / reps
-        bp.set_profiling(True)
-        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
-        bst = bp.kernel_stats()
-        bp.set_profiling(False)
-        summ_b = bp.summary()
-        bp.close()
-        rng = np.random.default_rng(9)
-        idx = rng.choice(len(stacks), sample, replace=False)
-        icm_amd.equity(stacks[idx[0]], pays[idx[0]], Q)
-        t_l = time.perf_counter()
-        for i in idx:
-            icm_amd.equity(stacks[i], pays[i], Q)
-        loop_s = time.perf_counter() - t_l
-        loop_inst_s = sample / loop_s
-        inst_s = len(stacks) / (batch_ms * 1e-3)
-        kern = {k_: {"launches": v[0], "ms": v[1] / 1e6,
-                     "GBps": (v[2] / (v[1] * 1e-9) / 1e9) if v[1] > 0 else 0.0}
-                for k_, v in sorted(bst.items(), key=lambda kv: -kv[1][1])}
-        return {"instances": len(stacks), "players": summ_b["n_players"], "Q": Q,
-                "ms_per_batch": batch_ms, "instances_per_s": inst_s,
-                "loop_dropin_instances_per_s": loop_inst_s,
-                "loop_sample": f"icm_hip_equity on {sample} random instances of the batch",
-                "speedup_vs_loop": inst_s / loop_inst_s, "plan_create_ms": create_ms,
-                "device_bytes": summ_b["device_bytes"], "kernels_eager": kern}
-
-    batched = batched_field()
-    log(f"batched: {batched}")
+    if args.only == "all":
+        adjoint["n1048576_k1048576"] = fwd_vjp_ratio(icm_amd, torch, dev, Q, S, p, 2)
+        log(f"adjoint: {adjoint}")
+        batched = batched_field(icm_amd, torch, dev, Q)
+        log(f"batched: {batched}")
 
     cpu = None
     if not args.no_cpu_baseline:
