@@ -111,6 +111,10 @@ _SIGS = [
     ("icm_hip_batch_equity_device", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(RunStats)]),
     ("icm_hip_equity_batch", ctypes.c_int, [ctypes.c_int, _I, _D, _I, ctypes.c_int, _D, _D]),
+    ("icm_hip_batch_vjp", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(RunStats)]),
+    ("icm_hip_batch_vjp_device", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.POINTER(RunStats)]),
     ("icm_hip_batch_summary", ctypes.c_int,
      [ctypes.c_void_p, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
     ("icm_hip_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -459,6 +463,36 @@ class BatchPlan:
                 raise ValueError(f"instance {i}: payout length {len(p)} != k")
         return self.split(self.run_flat(np.concatenate(ps), Q))
 
+    def vjp_flat(self, payout_flat, v_flat, Q=256):
+        """(equity, dS) flat arrays for a flat cotangent v (same order as the stacks)."""
+        p = _to_double_array(payout_flat, "payouts")
+        v = _to_double_array(v_flat, "v")
+        if len(p) != self.k_off[-1] or len(v) != self.n_off[-1]:
+            raise ValueError("flat payout / cotangent length mismatch")
+        if Q < 1:
+            raise ValueError(f"Q must be positive, got {Q}")
+        eq = np.zeros(int(self.n_off[-1]))
+        ds = np.zeros(int(self.n_off[-1]))
+        if self._lib.icm_hip_batch_vjp(self._h, int(Q), _ptr(p), _ptr(v), _ptr(eq), _ptr(ds),
+                                       ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_batch_vjp")
+        return eq, ds
+
+    def vjp(self, payouts_list, v_list, Q=256):
+        """Per-instance (equity, dS) lists for per-instance cotangents."""
+        if len(payouts_list) != self.n_inst or len(v_list) != self.n_inst:
+            raise ValueError("need one payout vector and one cotangent per instance")
+        eq, ds = self.vjp_flat(np.concatenate([_to_double_array(p, "payouts") for p in payouts_list]),
+                               np.concatenate([_to_double_array(v, "v") for v in v_list]), Q)
+        return self.split(eq), self.split(ds)
+
+    def vjp_device(self, d_payout_ptr, d_v_ptr, d_equity_ptr, d_ds_ptr, Q=256):
+        if self._lib.icm_hip_batch_vjp_device(self._h, int(Q), ctypes.c_void_p(d_payout_ptr),
+                                              ctypes.c_void_p(d_v_ptr), ctypes.c_void_p(d_equity_ptr or None),
+                                              ctypes.c_void_p(d_ds_ptr or None), ctypes.byref(self.stats)) != 0:
+            raise _err(self._lib, "icm_hip_batch_vjp_device")
+        return self.stats.total_ns
+
     def run_device(self, d_payout_ptr, d_equity_ptr, Q=256):
         if self._lib.icm_hip_batch_equity_device(self._h, int(Q), ctypes.c_void_p(d_payout_ptr),
                                                  ctypes.c_void_p(d_equity_ptr), ctypes.byref(self.stats)) != 0:
@@ -514,6 +548,39 @@ def equity_batch(stacks_list, payouts_list, Q=256):
                                 int(Q), _ptr(P), _ptr(out)) != 0:
         raise _err(lib, "icm_hip_equity_batch")
     return [out[n_off[i]:n_off[i + 1]] for i in range(len(n_off) - 1)]
+
+
+def jacobian_batch(stacks_list, payouts_list, Q=256, graphs=False):
+    """Full Jacobians dE/dS (n_i x n_i, row j = dE_j/dS) of every instance,
+    from one batched adjoint pass: instance i is replicated n_i times with
+    the unit cotangents e_0 .. e_{n_i - 1}."""
+    ss = [_to_double_array(s, "stacks") for s in stacks_list]
+    ps = [_to_double_array(p, "payouts") for p in payouts_list]
+    if len(ss) != len(ps):
+        raise ValueError("need one payout vector per instance")
+    rep_s, rep_k, rep_p, rep_v = [], [], [], []
+    for s, p in zip(ss, ps):
+        n = len(s)
+        for j in range(n):
+            rep_s.append(s)
+            rep_k.append(len(p))
+            rep_p.append(p)
+            e = np.zeros(n)
+            e[j] = 1.0
+            rep_v.append(e)
+    bp = BatchPlan(rep_s, rep_k, graphs=graphs)
+    try:
+        eq, ds = bp.vjp_flat(np.concatenate(rep_p), np.concatenate(rep_v), Q)
+    finally:
+        bp.close()
+    out, eqs, off = [], [], 0
+    for s in ss:
+        n = len(s)
+        rows = ds[off:off + n * n].reshape(n, n)
+        eqs.append(eq[off:off + n].copy())
+        out.append(rows.copy())
+        off += n * n
+    return out, eqs
 
 
 def plan_geometry(n, k, B=32, max_levels=64):

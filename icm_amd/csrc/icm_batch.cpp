@@ -73,9 +73,20 @@ struct IcmHipBatchPlan {
         std::vector<double> b_bytes, b_flops, p_bytes, p_flops;
     };
     std::vector<Pass> passes;            // index 0: nq = qb, 1: the remainder (if any)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_Q = 0;
+    // adjoint pass (allocated on the first VJP call)
+    bool vjp_ready = false;
+    double *d_dblocks = nullptr;         // [qb][NB][B+1] tangent block products
+    std::vector<double *> d_dcoef;       // tangent products per level (same offsets as d_coef)
+    std::vector<std::vector<long long>> bvoff;
+    long long bv_elems[2] = {0, 0};
+    double *d_bv[2] = {nullptr, nullptr};// B adjoint ping-pong by level parity
+    double *d_vin = nullptr;             // cotangent, flat original order
+    double *d_v = nullptr;               // cotangent, sorted
+    double *d_partial_ds = nullptr, *d_acc_ds = nullptr, *d_ds = nullptr;
+    std::vector<Pass> vpasses;
+    hipGraph_t graph[2] = {nullptr, nullptr};       // [0] forward, [1] adjoint
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int graph_Q[2] = {0, 0};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     size_t bytes = 0;
@@ -93,22 +104,38 @@ bool balloc(IcmHipBatchPlan *p, T **ptr, size_t count) {
     return true;
 }
 
-void drop_graph(IcmHipBatchPlan *p) {
-    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
-    p->gexec = nullptr;
-    p->graph = nullptr;
-    p->graph_Q = 0;
+void drop_graph(IcmHipBatchPlan *p, int mode) {
+    if (p->gexec[mode]) (void)hipGraphExecDestroy(p->gexec[mode]);
+    if (p->graph[mode]) (void)hipGraphDestroy(p->graph[mode]);
+    p->gexec[mode] = nullptr;
+    p->graph[mode] = nullptr;
+    p->graph_Q[mode] = 0;
+}
+
+void drop_graphs(IcmHipBatchPlan *p) {
+    drop_graph(p, 0);
+    drop_graph(p, 1);
+}
+
+void free_passes(std::vector<IcmHipBatchPlan::Pass> &v) {
+    for (auto &ps : v) {
+        if (ps.d_jobs) (void)hipFree(ps.d_jobs);
+        if (ps.d_items) (void)hipFree(ps.d_items);
+    }
+    v.clear();
 }
 
 void batch_free(IcmHipBatchPlan *p) {
     if (!p) return;
-    drop_graph(p);
+    drop_graphs(p);
     auto f = [](void *x) { if (x) (void)hipFree(x); };
     f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_inst); f(p->d_blk_inst);
     f(p->d_blocks); for (double *c : p->d_coef) f(c);
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_partial); f(p->d_acc); f(p->d_equity);
-    for (auto &ps : p->passes) { f(ps.d_jobs); f(ps.d_items); }
+    f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_vin); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    free_passes(p->passes);
+    free_passes(p->vpasses);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -140,7 +167,96 @@ icm::CoefView g_view(const IcmHipBatchPlan *p, int i, int l) {
     return icm::CoefView{p->d_g[l & 1] + p->goff[l][i], (long long)g.n_real[l] * g.G[l], g.G[l], g.G[l]};
 }
 
+// Tangent products of instance i at level l (l = 0: its tangent blocks).
+icm::CoefView tcoef_view(const IcmHipBatchPlan *p, int i, int l) {
+    icm::CoefView v = coef_view(p, i, l);
+    v.base = (l == 0) ? p->d_dblocks + (long long)p->boff[i] * (p->B + 1) : p->d_dcoef[l] + p->coff[l][i];
+    return v;
+}
+
+// B adjoint of instance i at level l (zero at the root: base null).
+icm::CoefView bv_view(const IcmHipBatchPlan *p, int i, int l) {
+    const Geometry &g = p->geo[i];
+    if (l >= g.L - 1) return icm::CoefView{};
+    return icm::CoefView{p->d_bv[l & 1] + p->bvoff[l][i], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+}
+
 double fft5(int F) { return 5.0 * F * (double)ilog2i(F); }
+
+// Adjoint job tables: fused levels, one parent per tile item.
+bool make_vjp_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
+    ps.nq = nq;
+    const int Lm = p->Lmax;
+    std::vector<icm::LevelJob> jobs;
+    std::vector<int2> items;
+    ps.b_off.assign(Lm, 0); ps.b_cnt.assign(Lm, 0); ps.p_off.assign(Lm, 0); ps.p_cnt.assign(Lm, 0);
+    ps.b_bytes.assign(Lm, 0); ps.b_flops.assign(Lm, 0); ps.p_bytes.assign(Lm, 0); ps.p_flops.assign(Lm, 0);
+    auto add_items = [&](int job, int nparent, int F) {
+        const long long nv = (long long)nq * nparent;
+        const int per = icm::kMaxLdsF / F;
+        for (long long v0 = 0; v0 < nv; v0 += per) items.push_back(make_int2(job, (int)v0));
+    };
+    for (int l = 1; l <= Lm - 2; ++l) {
+        ps.b_off[l] = (int)items.size();
+        for (int i = 0; i < p->ninst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l > g.L - 2) continue;
+            icm::LevelJob J{};
+            J.child = coef_view(p, i, l - 1);
+            J.tchild = tcoef_view(p, i, l - 1);
+            const long long pl = (long long)g.n_real[l] * g.c[l];
+            J.parent = icm::CoefOut{p->d_coef[l] + p->coff[l][i], pl, g.c[l], g.c[l]};
+            J.tparent = icm::CoefOut{p->d_dcoef[l] + p->coff[l][i], pl, g.c[l], g.c[l]};
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.np = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            add_items((int)jobs.size(), g.n_real[l], J.F);
+            jobs.push_back(J);
+            ps.b_bytes[l] += 16.0 * nq * ((double)g.n_real[l - 1] * g.c[l - 1] + (double)g.n_real[l] * g.c[l]);
+            ps.b_flops[l] += (double)nq * g.n_real[l] * 3.0 * fft5(J.F);
+        }
+        ps.b_cnt[l] = (int)items.size() - ps.b_off[l];
+    }
+    for (int l = Lm - 1; l >= 1; --l) {
+        ps.p_off[l] = (int)items.size();
+        for (int i = 0; i < p->ninst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l > g.L - 1) continue;
+            icm::LevelJob J{};
+            J.child = coef_view(p, i, l - 1);
+            J.tchild = tcoef_view(p, i, l - 1);
+            J.gpar = g_view(p, i, l);
+            J.bpar = bv_view(p, i, l);
+            const int cc = g.c[l - 1];
+            J.gchild = icm::CoefOut{p->d_g[(l - 1) & 1] + p->goff[l - 1][i],
+                                    (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
+            J.bchild = icm::CoefOut{p->d_bv[(l - 1) & 1] + p->bvoff[l - 1][i], (long long)g.n_real[l - 1] * cc, cc,
+                                    cc};
+            J.gfix = (g.F[l] < 2 * cc - 1) ? 1 : 0;
+            J.nr_child = g.n_real[l - 1];
+            J.nr_parent = g.n_real[l];
+            J.np = g.n_real[l];
+            J.F = g.F[l];
+            J.logF = ilog2i(g.F[l]);
+            J.nq = nq;
+            add_items((int)jobs.size(), g.n_real[l], J.F);
+            jobs.push_back(J);
+            const double ain = (l == g.L - 1) ? 8.0 * g.k : 8.0 * nq * (double)g.n_real[l] * (g.G[l] + g.c[l]);
+            ps.p_bytes[l] += ain + 8.0 * nq * (double)g.n_real[l - 1] * (3.0 * cc + g.G[l - 1]);
+            ps.p_flops[l] += (double)nq * g.n_real[l] * 5.0 * fft5(J.F);
+        }
+        ps.p_cnt[l] = (int)items.size() - ps.p_off[l];
+    }
+    if (items.size() > 0x7fffffffULL) { set_err("batch too large: work items exceed 2^31"); return false; }
+    if (!balloc(p, &ps.d_jobs, jobs.size()) || !balloc(p, &ps.d_items, items.size())) return false;
+    return (jobs.empty() || HIP_OK(hipMemcpy(ps.d_jobs, jobs.data(), sizeof(icm::LevelJob) * jobs.size(),
+                                             hipMemcpyHostToDevice))) &&
+           (items.empty() || HIP_OK(hipMemcpy(ps.d_items, items.data(), sizeof(int2) * items.size(),
+                                              hipMemcpyHostToDevice)));
+}
 
 // Job tables and work items for passes of nq q-planes.
 bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
@@ -351,23 +467,11 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     return p;
 }
 
-// Nodes for Q (per instance: its own S_max) and the instance table.
-bool prepare_Q(IcmHipBatchPlan *p, int Q) {
-    if (p->nodes_Q == Q) return true;
-    drop_graph(p);
-    for (auto &ps : p->passes) {
-        if (ps.d_jobs) (void)hipFree(ps.d_jobs);
-        if (ps.d_items) (void)hipFree(ps.d_items);
-    }
-    p->passes.clear();
-    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
-    p->nodes_Q = 0;
-    if (!balloc(p, &p->d_nodes, (size_t)p->ninst * 2 * Q)) return false;
-    std::vector<double> nodes((size_t)p->ninst * 2 * Q);
+// Instance table (needs d_nodes for the current Q; B views once the adjoint
+// buffers exist).
+bool upload_inst(IcmHipBatchPlan *p, int Q) {
     std::vector<icm::BatchInst> inst(p->ninst);
     for (int i = 0; i < p->ninst; ++i) {
-        double *lv = nodes.data() + (size_t)i * 2 * Q;
-        make_nodes(Q, p->Smax[i], lv, lv + Q);
         const Geometry &g = p->geo[i];
         icm::BatchInst &I = inst[i];
         I.logv = p->d_nodes + (size_t)i * 2 * Q;
@@ -377,16 +481,71 @@ bool prepare_Q(IcmHipBatchPlan *p, int Q) {
         I.poff = p->n_off[i];
         I.boff = p->boff[i];
         I.G0 = g.G[0];
+        I.b0 = p->vjp_ready ? bv_view(p, i, 0) : icm::CoefView{};
+        I.c0 = g.c[0];
+    }
+    return HIP_OK(hipMemcpy(p->d_inst, inst.data(), sizeof(icm::BatchInst) * inst.size(), hipMemcpyHostToDevice));
+}
+
+bool make_passes(IcmHipBatchPlan *p, int Q, bool vjp) {
+    std::vector<IcmHipBatchPlan::Pass> &v = vjp ? p->vpasses : p->passes;
+    free_passes(v);
+    const int nfull = Q / p->qb, rem = Q % p->qb;
+    v.resize((nfull > 0 ? 1 : 0) + (rem > 0 ? 1 : 0));
+    int idx = 0;
+    auto mk = [&](int nq, IcmHipBatchPlan::Pass &ps) { return vjp ? make_vjp_pass(p, nq, ps) : make_pass(p, nq, ps); };
+    if (nfull > 0 && !mk(p->qb, v[idx++])) return false;
+    if (rem > 0 && !mk(rem, v[idx++])) return false;
+    return true;
+}
+
+// Nodes for Q (per instance: its own S_max), instance table and job tables.
+bool prepare_Q(IcmHipBatchPlan *p, int Q) {
+    if (p->nodes_Q == Q) return true;
+    drop_graphs(p);
+    free_passes(p->passes);
+    free_passes(p->vpasses);
+    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
+    p->nodes_Q = 0;
+    if (!balloc(p, &p->d_nodes, (size_t)p->ninst * 2 * Q)) return false;
+    std::vector<double> nodes((size_t)p->ninst * 2 * Q);
+    for (int i = 0; i < p->ninst; ++i) {
+        double *lv = nodes.data() + (size_t)i * 2 * Q;
+        make_nodes(Q, p->Smax[i], lv, lv + Q);
     }
     if (!HIP_OK(hipMemcpy(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice)) ||
-        !HIP_OK(hipMemcpy(p->d_inst, inst.data(), sizeof(icm::BatchInst) * inst.size(), hipMemcpyHostToDevice)))
+        !upload_inst(p, Q) || !make_passes(p, Q, false))
         return false;
-    const int nfull = Q / p->qb, rem = Q % p->qb;
-    p->passes.resize((nfull > 0 ? 1 : 0) + (rem > 0 ? 1 : 0));
-    int idx = 0;
-    if (nfull > 0 && !make_pass(p, p->qb, p->passes[idx++])) return false;
-    if (rem > 0 && !make_pass(p, rem, p->passes[idx++])) return false;
+    if (p->vjp_ready && !make_passes(p, Q, true)) return false;
     p->nodes_Q = Q;
+    return true;
+}
+
+// Adjoint buffers: tangent products like the forward's, B adjoint ping-pong.
+bool ensure_vjp(IcmHipBatchPlan *p, int Q) {
+    if (p->vjp_ready) return true;
+    const int Lm = p->Lmax, qb = p->qb, B = p->B;
+    p->bvoff.assign(Lm, std::vector<long long>(p->ninst, 0));
+    for (int l = 0; l < Lm; ++l) {
+        long long o = 0;
+        for (int i = 0; i < p->ninst; ++i) {
+            const Geometry &g = p->geo[i];
+            if (l <= g.L - 2) { p->bvoff[l][i] = o; o += (long long)qb * g.n_real[l] * g.c[l]; }
+        }
+        p->bv_elems[l & 1] = std::max(p->bv_elems[l & 1], o);
+    }
+    bool ok = balloc(p, &p->d_dblocks, (size_t)qb * p->NB * (B + 1)) && balloc(p, &p->d_bv[0], p->bv_elems[0]) &&
+              balloc(p, &p->d_bv[1], p->bv_elems[1]) && balloc(p, &p->d_vin, p->Ntot) &&
+              balloc(p, &p->d_v, p->Ntot) && balloc(p, &p->d_partial_ds, (size_t)p->nchunk * p->Ntot) &&
+              balloc(p, &p->d_acc_ds, p->Ntot) && balloc(p, &p->d_ds, p->Ntot);
+    p->d_dcoef.assign(Lm, nullptr);
+    for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_dcoef[l], p->coef_elems[l]);
+    if (!ok) return false;
+    p->vjp_ready = true;
+    if (p->nodes_Q == Q) {   // tables of this Q exist: add the B views and the adjoint jobs
+        drop_graphs(p);
+        return upload_inst(p, Q) && make_passes(p, Q, true);
+    }
     return true;
 }
 
@@ -429,29 +588,81 @@ bool enqueue(IcmHipBatchPlan *p, int Q) {
     return hip_ok(icm::launch_scatter(p->d_acc, p->Ntot, p->d_perm, p->d_equity, s, &p->prof), "scatter");
 }
 
-// Runs the batch for Q nodes with the payout already in d_payout.
-int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats) {
+// Adjoint: gathers the cotangent (d_vin, original order), then per q-pass
+// (P, dP) leaves, fused build levels, fused propagate levels (A and B),
+// leaf adjoint; leaves equities in d_equity and dS in d_ds.
+bool enqueue_vjp(IcmHipBatchPlan *p, int Q) {
+    hipStream_t s = p->stream;
+    const int B = p->B, NB = p->NB;
+    if (!hip_ok(icm::launch_gather(p->d_vin, p->d_perm, p->Ntot, p->d_v, s), "gather")) return false;
+    for (int q0 = 0; q0 < Q; q0 += p->qb) {
+        const int nq = std::min(p->qb, Q - q0);
+        const IcmHipBatchPlan::Pass &ps = (nq == p->qb) ? p->vpasses[0] : p->vpasses.back();
+        const double lb_bytes = 8.0 * (2.0 * p->Ntot + 2.0 * nq * NB * (B + 1));
+        if (!hip_ok(icm::launch_leaf_build_vjp_batch(B, p->d_inst, p->d_blk_inst, p->d_S, p->d_v, p->d_blocks,
+                                                     p->d_dblocks, NB, nq, q0, s, &p->prof, lb_bytes),
+                    "batch adjoint leaf build"))
+            return false;
+        for (int l = 1; l <= p->Lmax - 2; ++l)
+            if (!hip_ok(icm::launch_level_t_batch(true, ps.d_jobs, ps.d_items + ps.b_off[l], ps.b_cnt[l], p->d_tw,
+                                                  p->twShift, s, &p->prof, ps.b_bytes[l], ps.b_flops[l]),
+                        "batch adjoint build level"))
+                return false;
+        for (int l = p->Lmax - 1; l >= 1; --l)
+            if (!hip_ok(icm::launch_level_t_batch(false, ps.d_jobs, ps.d_items + ps.p_off[l], ps.p_cnt[l], p->d_tw,
+                                                  p->twShift, s, &p->prof, ps.p_bytes[l], ps.p_flops[l]),
+                        "batch adjoint propagate level"))
+                return false;
+        const int nchunk = std::min(p->nchunk, nq);
+        const int qpc = (nq + nchunk - 1) / nchunk;
+        const int nch = (nq + qpc - 1) / qpc;
+        double gb = 0.0;
+        for (int i = 0; i < p->ninst; ++i) gb += (double)p->geo[i].nb * (p->geo[i].G[0] + p->geo[i].c[0]);
+        const double le_bytes = 8.0 * ((double)nq * (2.0 * NB * (B + 1) + gb) + (double)p->Ntot * (2 + 2 * nch));
+        if (!hip_ok(icm::launch_leaf_vjp_batch(B, p->d_inst, p->d_blk_inst, p->d_S, p->d_v, p->d_blocks,
+                                               p->d_dblocks, NB, nq, q0, nch, qpc, p->d_partial, p->d_partial_ds,
+                                               p->Ntot, s, &p->prof, le_bytes),
+                    "batch adjoint leaves") ||
+            !hip_ok(icm::launch_finalize(p->d_partial, nch, p->Ntot, nullptr, p->d_acc, q0 > 0 ? 1 : 0, s, &p->prof),
+                    "finalize") ||
+            !hip_ok(icm::launch_finalize(p->d_partial_ds, nch, p->Ntot, nullptr, p->d_acc_ds, q0 > 0 ? 1 : 0, s,
+                                         &p->prof),
+                    "finalize"))
+            return false;
+    }
+    return hip_ok(icm::launch_scatter(p->d_acc, p->Ntot, p->d_perm, p->d_equity, s, &p->prof), "scatter") &&
+           hip_ok(icm::launch_scatter(p->d_acc_ds, p->Ntot, p->d_perm, p->d_ds, s, &p->prof), "scatter");
+}
+
+// Runs the batch (mode 0: forward, 1: adjoint) for Q nodes with the payout
+// (and cotangent) already on the device.
+int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode = 0) {
     if (Q < 1) { set_err("Q must be positive"); return -1; }
     if (!HIP_OK(hipSetDevice(g_device))) return -1;
     if (!prepare_Q(p, Q)) return -1;
+    if (mode == 1 && !ensure_vjp(p, Q)) return -1;
     hipStream_t s = p->stream;
+    auto run = [&]() { return mode == 1 ? enqueue_vjp(p, Q) : enqueue(p, Q); };
     const bool graph = p->use_graph && !p->prof.enabled;
-    if (graph && p->graph_Q != Q) {
-        drop_graph(p);
+    if (graph && p->graph_Q[mode] != Q) {
+        drop_graph(p, mode);
         if (!HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal))) return -1;
-        const bool ok = enqueue(p, Q);
+        const bool ok = run();
         hipGraph_t gr = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &gr);
         if (!ok) { if (gr) (void)hipGraphDestroy(gr); return -1; }
         if (!hip_ok(e, "hipStreamEndCapture")) return -1;
-        p->graph = gr;
-        if (!HIP_OK(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0))) { drop_graph(p); return -1; }
-        p->graph_Q = Q;
+        p->graph[mode] = gr;
+        if (!HIP_OK(hipGraphInstantiate(&p->gexec[mode], p->graph[mode], nullptr, nullptr, 0))) {
+            drop_graph(p, mode);
+            return -1;
+        }
+        p->graph_Q[mode] = Q;
     }
     if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
     if (graph) {
-        if (!HIP_OK(hipGraphLaunch(p->gexec, s))) return -1;
-    } else if (!enqueue(p, Q)) {
+        if (!HIP_OK(hipGraphLaunch(p->gexec[mode], s))) return -1;
+    } else if (!run()) {
         return -1;
     }
     if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
@@ -504,6 +715,40 @@ int icm_hip_batch_equity_device(IcmHipBatchPlan *plan, int Q, const double *d_pa
                                plan->stream)))
         return -1;
     return HIP_OK(hipStreamSynchronize(plan->stream)) ? 0 : -1;
+}
+
+int icm_hip_batch_vjp(IcmHipBatchPlan *plan, int Q, const double *payout, const double *v, double *equity,
+                      double *dS, IcmHipRunStats *stats) {
+    if (!plan || !payout || !v) { set_err("null argument"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!ensure_vjp(plan, Q)) return -1;
+    if (!HIP_OK(hipMemcpyAsync(plan->d_payout, payout, sizeof(double) * plan->Ktot, hipMemcpyHostToDevice,
+                               plan->stream)) ||
+        !HIP_OK(hipMemcpyAsync(plan->d_vin, v, sizeof(double) * plan->Ntot, hipMemcpyHostToDevice, plan->stream)))
+        return -1;
+    if (batch_run(plan, Q, stats, 1) != 0) return -1;
+    if (equity && !HIP_OK(hipMemcpy(equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost)))
+        return -1;
+    if (dS && !HIP_OK(hipMemcpy(dS, plan->d_ds, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost))) return -1;
+    return 0;
+}
+
+int icm_hip_batch_vjp_device(IcmHipBatchPlan *plan, int Q, const double *d_payout, const double *d_v,
+                             double *d_equity, double *d_dS, IcmHipRunStats *stats) {
+    if (!plan || !d_payout || !d_v) { set_err("null argument"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    if (!ensure_vjp(plan, Q)) return -1;
+    hipStream_t s = plan->stream;
+    if (!HIP_OK(hipMemcpyAsync(plan->d_payout, d_payout, sizeof(double) * plan->Ktot, hipMemcpyDeviceToDevice, s)) ||
+        !HIP_OK(hipMemcpyAsync(plan->d_vin, d_v, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice, s)))
+        return -1;
+    if (batch_run(plan, Q, stats, 1) != 0) return -1;
+    if (d_equity &&
+        !HIP_OK(hipMemcpyAsync(d_equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice, s)))
+        return -1;
+    if (d_dS && !HIP_OK(hipMemcpyAsync(d_dS, plan->d_ds, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice, s)))
+        return -1;
+    return HIP_OK(hipStreamSynchronize(s)) ? 0 : -1;
 }
 
 int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const int *k_off, int Q,

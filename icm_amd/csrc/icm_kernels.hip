@@ -300,6 +300,54 @@ __global__ __launch_bounds__(64) void k_leaf_build_vjp(VjpLeafJob V) {
     for (int e = lane; e < nblk * (B + 1); e += 64) V.dblocks[off + e] = sh[e];
 }
 
+// One player's adjoint at the leaves (A = forward g, Bv = adjoint of P_b,
+// both zero past their lengths; A has B+1 entries, Bv B+2):
+//   Q = P_b / f_j,  W = (dP_b - c_j Q) / f_j   (same division branch)
+//   psi = sum_{m<pk} A[m] Q[m]
+//   h   = sum_m Q[m] (Bv[m] - Bv[m+1]) + sum_m W[m] (A[m] - A[m+1])
+template <int B>
+__device__ __forceinline__ void leaf_vjp_player(const double *P, const double *D, const double *A, const double *Bv,
+                                                double a, double c, int bsize, int pk, double &psi, double &h) {
+    const double bm = 1.0 - a;
+    psi = 0.0;
+    h = 0.0;
+    if (a > 0.5) {
+        const double ia = 1.0 / a, r = -bm * ia;
+        double qv = 0.0, wv = 0.0;
+        for (int m = 0; m < bsize; ++m) {
+            qv = P[m] * ia + r * qv;
+            if (m < pk) psi += A[m] * qv;
+            h += qv * (Bv[m] - Bv[m + 1]);
+            if (m <= bsize - 2) {
+                wv = (D[m] - c * qv) * ia + r * wv;
+                h += wv * (A[m] - A[m + 1]);
+            }
+        }
+    } else if (a > 1e-15) {
+        const double ib = 1.0 / bm, r = -a * ib;
+        double qv = 0.0, wv = 0.0;
+        for (int m = bsize - 1; m >= 0; --m) {
+            qv = (m == bsize - 1) ? P[bsize] * ib : P[m + 1] * ib + r * qv;
+            if (m < pk) psi += A[m] * qv;
+            h += qv * (Bv[m] - Bv[m + 1]);
+            if (m >= 1) {
+                wv = (D[m] - c * qv) * ib + r * wv;     // W[m-1]
+                h += wv * (A[m - 1] - A[m]);
+            }
+        }
+    } else {
+        for (int m = 0; m < bsize; ++m) {
+            const double qv = P[m + 1];
+            if (m < pk) psi += A[m] * qv;
+            h += qv * (Bv[m] - Bv[m + 1]);
+            if (m <= bsize - 2) {
+                const double wv = D[m + 1] - c * (m + 2 <= B ? P[m + 2] : 0.0);
+                h += wv * (A[m] - A[m + 1]);
+            }
+        }
+    }
+}
+
 // Per player j of block b (A = forward g of the block, Bv = adjoint of P_b):
 //   Q = P_b / f_j,  W = (dP_b - c_j Q) / f_j          (same division branch)
 //   Psi = sum_{m<pk} A[m] Q[m]
@@ -351,50 +399,12 @@ __global__ __launch_bounds__(256) void k_leaf_vjp(VjpLeafJob V) {
             const double lv = J.logv[q], wq = J.wq[q];
             const double ivv = exp(-lv);
             const double a = J.a_in ? J.a_in[(long long)q * J.n + j] : a_of(Sj, lv);
-            const double bm = 1.0 - a;
             double pw = wq * Sj * a * ivv;
             if (!isfinite(pw)) pw = 0.0;
             const double c = V.c_in ? V.c_in[(long long)q * J.n + j] : vj * pw;
-            const double *P = shP + lb * (B + 1);
-            const double *D = shD + lb * (B + 1);
-            const double *A = shA + lb * (B + 1);   // A[m] = 0 for m >= G0 (A[B] = 0)
-            const double *Bv = shB + lb * (B + 2);  // Bv[m] = 0 for m >= c0
-            double psi = 0.0, h = 0.0;
-            if (a > 0.5) {
-                const double ia = 1.0 / a, r = -bm * ia;
-                double qv = 0.0, wv = 0.0;
-                for (int m = 0; m < bsize; ++m) {
-                    qv = P[m] * ia + r * qv;
-                    if (m < pk) psi += A[m] * qv;
-                    h += qv * (Bv[m] - Bv[m + 1]);
-                    if (m <= bsize - 2) {
-                        wv = (D[m] - c * qv) * ia + r * wv;
-                        h += wv * (A[m] - A[m + 1]);
-                    }
-                }
-            } else if (a > 1e-15) {
-                const double ib = 1.0 / bm, r = -a * ib;
-                double qv = 0.0, wv = 0.0;
-                for (int m = bsize - 1; m >= 0; --m) {
-                    qv = (m == bsize - 1) ? P[bsize] * ib : P[m + 1] * ib + r * qv;
-                    if (m < pk) psi += A[m] * qv;
-                    h += qv * (Bv[m] - Bv[m + 1]);
-                    if (m >= 1) {
-                        wv = (D[m] - c * qv) * ib + r * wv;     // W[m-1]
-                        h += wv * (A[m - 1] - A[m]);
-                    }
-                }
-            } else {
-                for (int m = 0; m < bsize; ++m) {
-                    const double qv = P[m + 1];
-                    if (m < pk) psi += A[m] * qv;
-                    h += qv * (Bv[m] - Bv[m + 1]);
-                    if (m <= bsize - 2) {
-                        const double wv = D[m + 1] - c * (m + 2 <= B ? P[m + 2] : 0.0);
-                        h += wv * (A[m] - A[m + 1]);
-                    }
-                }
-            }
+            double psi, h;
+            leaf_vjp_player<B>(shP + lb * (B + 1), shD + lb * (B + 1), shA + lb * (B + 1), shB + lb * (B + 2), a,
+                               c, bsize, pk, psi, h);
             if (J.inner_out) J.inner_out[(long long)q * J.n + j] = psi;
             if (V.h_out) V.h_out[(long long)q * J.n + j] = h;
             accE += pw * psi;
@@ -407,6 +417,144 @@ __global__ __launch_bounds__(256) void k_leaf_vjp(VjpLeafJob V) {
     if (valid) {
         if (J.partial) J.partial[(long long)chunk * J.n + j] = accE;
         if (V.partial_ds) V.partial_ds[(long long)chunk * J.n + j] = accD;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ragged-batch adjoint leaves (global block numbering as the forward batch).
+// v: cotangent in sorted concatenated order.
+// ---------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(64) void k_leaf_build_vjp_batch(const BatchInst *__restrict__ inst,
+                                                             const int *__restrict__ blk_inst,
+                                                             const double *__restrict__ S,
+                                                             const double *__restrict__ v, double *blocks,
+                                                             double *dblocks, int NB, int q0) {
+    __shared__ double sh[64 * (B + 1)];
+    const int lane = threadIdx.x;
+    const int qi = blockIdx.y;
+    const int b0 = blockIdx.x * 64;
+    const int gb = b0 + lane;
+    double P[B + 1], D[B + 1];
+    P[0] = 1.0;
+    D[0] = 0.0;
+#pragma unroll
+    for (int m = 1; m <= B; ++m) { P[m] = 0.0; D[m] = 0.0; }
+    if (gb < NB) {
+        const BatchInst &I = inst[blk_inst[gb]];
+        const double lv = I.logv[q0 + qi], wq = I.w[q0 + qi];
+        const double ivv = exp(-lv);
+        const int b = gb - I.boff;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            const int j = b * B + i;
+            double a = 1.0, c = 0.0;
+            if (j < I.n) {
+                const double Sj = S[I.poff + j];
+                a = a_of(Sj, lv);
+                double pw = wq * Sj * a * ivv;
+                if (!isfinite(pw)) pw = 0.0;
+                c = v[I.poff + j] * pw;
+            }
+            const double bb = 1.0 - a;
+#pragma unroll
+            for (int m = i + 1; m >= 1; --m) {
+                D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
+                P[m] = a * P[m] + bb * P[m - 1];
+            }
+            D[0] = a * D[0] + c * P[0];
+            P[0] *= a;
+        }
+    }
+    const int nblk = min(64, NB - b0);
+    const long long off = (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1);
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
+    __syncthreads();
+    for (int e = lane; e < nblk * (B + 1); e += 64) blocks[off + e] = sh[e];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = D[m];
+    __syncthreads();
+    for (int e = lane; e < nblk * (B + 1); e += 64) dblocks[off + e] = sh[e];
+}
+
+template <int B>
+__global__ __launch_bounds__(256) void k_leaf_vjp_batch(const BatchInst *__restrict__ inst,
+                                                        const int *__restrict__ blk_inst,
+                                                        const double *__restrict__ S,
+                                                        const double *__restrict__ v,
+                                                        const double *__restrict__ blocks,
+                                                        const double *__restrict__ dblocks, int NB, int nq, int q0,
+                                                        int q_per_chunk, double *partial, double *partial_ds,
+                                                        int Ntot) {
+    constexpr int BPW = 256 / B;
+    __shared__ double shP[BPW * (B + 1)];
+    __shared__ double shD[BPW * (B + 1)];
+    __shared__ double shA[BPW * (B + 1)];
+    __shared__ double shB[BPW * (B + 2)];
+    const int t = threadIdx.x;
+    const int lb = t / B, li = t % B;
+    const int gfirst = blockIdx.x * BPW;
+    const int gb = gfirst + lb;
+    const int chunk = blockIdx.y;
+    const int qa = chunk * q_per_chunk;
+    const int qe = min(qa + q_per_chunk, nq);
+    const BatchInst *I = (gb < NB) ? &inst[blk_inst[gb]] : nullptr;
+    const int b = I ? gb - I->boff : 0;
+    const int j = b * B + li;
+    const bool valid = I && j < I->n;
+    const int bsize = I ? min(B, I->n - b * B) : 0;
+    const int pk = I ? min(I->G0, bsize) : 0;
+    const double Sj = valid ? S[I->poff + j] : 0.0;
+    const double vj = valid ? v[I->poff + j] : 0.0;
+    const long long bplane = (long long)NB * (B + 1);
+    double accE = 0.0, accD = 0.0;
+    for (int q = qa; q < qe; ++q) {
+        for (int e = t; e < BPW * (B + 1); e += 256) {
+            const int g2 = gfirst + e / (B + 1), m = e % (B + 1);
+            double pv = 0.0, dv = 0.0, av = 0.0;
+            if (g2 < NB) {
+                const long long o = q * bplane + (long long)g2 * (B + 1) + m;
+                pv = blocks[o];
+                dv = dblocks[o];
+                const BatchInst &I2 = inst[blk_inst[g2]];
+                if (m < I2.G0) av = I2.g0.base[q * I2.g0.plane + (long long)(g2 - I2.boff) * I2.g0.stride + m];
+            }
+            shP[e] = pv;
+            shD[e] = dv;
+            shA[e] = av;
+        }
+        for (int e = t; e < BPW * (B + 2); e += 256) {
+            const int g2 = gfirst + e / (B + 2), m = e % (B + 2);
+            double bv = 0.0;
+            if (g2 < NB) {
+                const BatchInst &I2 = inst[blk_inst[g2]];
+                if (I2.b0.base && m < I2.c0)
+                    bv = I2.b0.base[q * I2.b0.plane + (long long)(g2 - I2.boff) * I2.b0.stride + m];
+            }
+            shB[e] = bv;
+        }
+        __syncthreads();
+        if (valid) {
+            const double lv = I->logv[q0 + q], wq = I->w[q0 + q];
+            const double ivv = exp(-lv);
+            const double a = a_of(Sj, lv);
+            double pw = wq * Sj * a * ivv;
+            if (!isfinite(pw)) pw = 0.0;
+            double psi, h;
+            leaf_vjp_player<B>(shP + lb * (B + 1), shD + lb * (B + 1), shA + lb * (B + 1), shB + lb * (B + 2), a,
+                               vj * pw, bsize, pk, psi, h);
+            accE += pw * psi;
+            double dterm = vj * wq * ivv * a * (1.0 + Sj * lv) * psi;
+            if (!isfinite(dterm)) dterm = 0.0;
+            accD += dterm + h * a * lv;
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        partial[(long long)chunk * Ntot + I->poff + j] = accE;
+        partial_ds[(long long)chunk * Ntot + I->poff + j] = accD;
     }
 }
 
@@ -813,22 +961,22 @@ __device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, i
     return V.base[(long long)qi * V.plane + (long long)node * V.stride + m];
 }
 
-template <int F>
-__global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
+// Item maps as the forward kernels, with one parent per item ("pair" = parent).
+template <int F, class Map>
+__device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
+                                                 int twShift) {
     using TL = TileLayout<F>;
-    __shared__ double2 lds[TL::SIZE];
-    constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
-    const int qi = blockIdx.y;
-    const int p0 = blockIdx.x * ITEMS;
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        int qi, pr;
+        M.at(item, qi, pr);
+        const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
     }
     fft_tile<F, false, kLT>(lds, tw, twShift);
@@ -846,7 +994,9 @@ __global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        int qi, pr;
+        M.at(item, qi, pr);
+        const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
     }
     fft_tile<F, false, kLT>(lds, tw, twShift);
@@ -874,7 +1024,8 @@ __global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = p0 + item;
+        int qi, p;
+        M.at(item, qi, p);
         if (p >= J.nr_parent) continue;
         double *pp = J.parent.base + (long long)qi * J.parent.plane + (long long)p * J.parent.stride;
         double *dp = J.tparent.base + (long long)qi * J.tparent.plane + (long long)p * J.tparent.stride;
@@ -907,23 +1058,21 @@ __global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *
     }
 }
 
-template <int F>
-__global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
+template <int F, class Map>
+__device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
+                                                int twShift) {
     using TL = TileLayout<F>;
-    __shared__ double2 lds[TL::SIZE];
-    constexpr int ITEMS = kTile / F;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
-    const int qi = blockIdx.y;
-    const int p0 = blockIdx.x * ITEMS;
     // (1) A + iB of the parent
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = p0 + item;
+        int qi, p;
+        M.at(item, qi, p);
         double2 z = make_double2(0.0, 0.0);
         if (p < J.nr_parent) {
             if (m < J.gpar.count) z.x = view_at(J.gpar, qi, p, m);
@@ -947,7 +1096,9 @@ __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *t
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        int qi, pr;
+        M.at(item, qi, pr);
+        const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
     }
     fft_tile<F, false, kLT>(lds, tw, twShift);
@@ -966,7 +1117,9 @@ __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *t
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = (p0 + item < J.nr_parent) ? p0 + item : -1;
+        int qi, pr;
+        M.at(item, qi, pr);
+        const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
     }
     fft_tile<F, false, kLT>(lds, tw, twShift);
@@ -1001,7 +1154,8 @@ __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *t
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = p0 + item;
+        int qi, p;
+        M.at(item, qi, p);
         if (p < J.nr_parent && m < J.gchild.count) {
             const double2 v = lds[TL::at(item, m)];
             double *g = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
@@ -1023,7 +1177,8 @@ __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *t
     for (int u = 0; u < kLP; ++u) {
         const int e = t + u * kLT;
         const int item = e >> LOGF, m = e & (F - 1);
-        const int p = p0 + item;
+        int qi, p;
+        M.at(item, qi, p);
         if (p < J.nr_parent && m < J.bchild.count) {
             const double2 v = lds[TL::at(item, m)];
             double bl = v.x * invF, br = v.y * invF;
@@ -1045,6 +1200,44 @@ __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *t
             if (2 * p + 1 < J.nr_child) bb[J.bchild.stride + m] = br;
         }
     }
+}
+
+
+template <int F>
+__global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[TileLayout<F>::SIZE];
+    build_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+}
+
+template <int F>
+__global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[TileLayout<F>::SIZE];
+    prop_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+}
+
+// Ragged batches, adjoint levels: work item = (job, first parent item).
+__global__ __launch_bounds__(512) void k_build_lds_t_batch(const LevelJob *__restrict__ jobs,
+                                                           const int2 *__restrict__ items, const double2 *tw,
+                                                           int twShift) {
+    __shared__ double2 lds[kTileMaxSize];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const FlatMap M{it.y, J.nr_parent, J.nq * J.nr_parent};
+#define ICM_BUILD_T_F(FF) build_lds_t_body<FF>(J, lds, M, tw, twShift)
+    ICM_F_CASES(ICM_BUILD_T_F)
+#undef ICM_BUILD_T_F
+}
+
+__global__ __launch_bounds__(512) void k_prop_lds_t_batch(const LevelJob *__restrict__ jobs,
+                                                          const int2 *__restrict__ items, const double2 *tw,
+                                                          int twShift) {
+    __shared__ double2 lds[kTileMaxSize];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const FlatMap M{it.y, J.nr_parent, J.nq * J.nr_parent};
+#define ICM_PROP_T_F(FF) prop_lds_t_body<FF>(J, lds, M, tw, twShift)
+    ICM_F_CASES(ICM_PROP_T_F)
+#undef ICM_PROP_T_F
 }
 
 // ---------------------------------------------------------------------------
@@ -1671,6 +1864,57 @@ hipError_t launch_level_batch(bool build, const LevelJob *jobs, const int2 *item
     else k_prop_lds_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
     hipError_t e = hipGetLastError();
     PF_END(build ? "k_build_lds_batch" : "k_prop_lds_batch", bytes, flops);
+    return e;
+}
+
+hipError_t launch_leaf_build_vjp_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                       const double *v, double *blocks, double *dblocks, int NB, int nq, int q0,
+                                       hipStream_t s, Prof *pf, double bytes) {
+    if (NB < 1 || nq < 1) return hipSuccess;
+    PF_BEGIN();
+    dim3 grid((NB + 63) / 64, nq);
+    switch (B) {
+    case 16: k_leaf_build_vjp_batch<16><<<grid, 64, 0, s>>>(inst, blk_inst, S, v, blocks, dblocks, NB, q0); break;
+    case 32: k_leaf_build_vjp_batch<32><<<grid, 64, 0, s>>>(inst, blk_inst, S, v, blocks, dblocks, NB, q0); break;
+    case 64: k_leaf_build_vjp_batch<64><<<grid, 64, 0, s>>>(inst, blk_inst, S, v, blocks, dblocks, NB, q0); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    PF_END("k_leaf_build_vjp_batch", bytes, 0.0);
+    return e;
+}
+
+hipError_t launch_leaf_vjp_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                 const double *v, const double *blocks, const double *dblocks, int NB, int nq,
+                                 int q0, int nchunk, int q_per_chunk, double *partial, double *partial_ds, int Ntot,
+                                 hipStream_t s, Prof *pf, double bytes) {
+    if (NB < 1 || nq < 1) return hipSuccess;
+    PF_BEGIN();
+    const int bpw = 256 / B;
+    dim3 grid((NB + bpw - 1) / bpw, nchunk);
+#define ICM_LVB(BB)                                                                                          \
+    k_leaf_vjp_batch<BB><<<grid, 256, 0, s>>>(inst, blk_inst, S, v, blocks, dblocks, NB, nq, q0, q_per_chunk, \
+                                              partial, partial_ds, Ntot)
+    switch (B) {
+    case 16: ICM_LVB(16); break;
+    case 32: ICM_LVB(32); break;
+    case 64: ICM_LVB(64); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef ICM_LVB
+    hipError_t e = hipGetLastError();
+    PF_END("k_leaf_vjp_batch", bytes, 0.0);
+    return e;
+}
+
+hipError_t launch_level_t_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, const double2 *tw,
+                                int twShift, hipStream_t s, Prof *pf, double bytes, double flops) {
+    if (nitems < 1) return hipSuccess;
+    PF_BEGIN();
+    if (build) k_build_lds_t_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
+    else k_prop_lds_t_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
+    hipError_t e = hipGetLastError();
+    PF_END(build ? "k_build_lds_t_batch" : "k_prop_lds_t_batch", bytes, flops);
     return e;
 }
 

@@ -93,6 +93,8 @@ struct BatchInst {
     int poff;               // first player in the concatenated sorted stacks
     int boff;               // first block in the global block numbering
     int G0;                 // g entries per block
+    CoefView b0;            // adjoint pass: B adjoint at the leaves (base null when L == 1: zero)
+    int c0;                 // B entries per block (= level-0 coefficient count)
 };
 
 // Optional per-launch timing: when on, every kernel launch is bracketed by
@@ -147,6 +149,17 @@ hipError_t launch_leaf_extract_batch(int B, const BatchInst *inst, const int *bl
                                      double *partial, int Ntot, hipStream_t s, Prof *pf, double bytes);
 hipError_t launch_level_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, const double2 *tw,
                               int twShift, hipStream_t s, Prof *pf, double bytes, double flops);
+// Ragged-batch adjoint: (P, dP) leaves, fused adjoint levels (one parent per
+// tile item), leaf adjoint with equity and dS partials.  v is sorted order.
+hipError_t launch_leaf_build_vjp_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                       const double *v, double *blocks, double *dblocks, int NB, int nq, int q0,
+                                       hipStream_t s, Prof *pf, double bytes);
+hipError_t launch_leaf_vjp_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
+                                 const double *v, const double *blocks, const double *dblocks, int NB, int nq,
+                                 int q0, int nchunk, int q_per_chunk, double *partial, double *partial_ds, int Ntot,
+                                 hipStream_t s, Prof *pf, double bytes);
+hipError_t launch_level_t_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, const double2 *tw,
+                                int twShift, hipStream_t s, Prof *pf, double bytes, double flops);
 
 constexpr int kMaxLdsF = 4096;
 

@@ -190,6 +190,14 @@ int icm_hip_batch_equity(IcmHipBatchPlan *plan, int Q, const double *payout, dou
 /* d_payout[K] and d_equity[N] are device pointers (K = k_off[n_inst], N = n_off[n_inst]). */
 int icm_hip_batch_equity_device(IcmHipBatchPlan *plan, int Q, const double *d_payout, double *d_equity,
                                 IcmHipRunStats *stats);
+/* Adjoint of a ragged batch: equities and dS = d(v.E)/dS per instance for a
+ * cotangent v (flat, same order as S); each instance as icm_hip_vjp_with_plan.
+ * equity / dS may be NULL.  The first call allocates the adjoint buffers;
+ * the first call per Q is recorded as its own HIP graph. */
+int icm_hip_batch_vjp(IcmHipBatchPlan *plan, int Q, const double *payout, const double *v, double *equity,
+                      double *dS, IcmHipRunStats *stats);
+int icm_hip_batch_vjp_device(IcmHipBatchPlan *plan, int Q, const double *d_payout, const double *d_v,
+                             double *d_equity, double *d_dS, IcmHipRunStats *stats);
 /* One-shot: plan, run (no graph), free. */
 int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const int *k_off, int Q,
                          const double *payout, double *equity);
