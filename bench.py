@@ -141,6 +141,38 @@ def batched_field(icm_amd, torch, dev, Q, reps=5, sample=256):
             "device_bytes": summ_b["device_bytes"], "kernels_eager": kern}
 
 
+# full 64 x 64 Jacobians of 512 n = 64 instances: one batched adjoint over
+# 512 x 64 replicas with unit cotangents (graph replay timed)
+def jacobian_field(icm_amd, torch, dev, Q, reps=3):
+    rng = np.random.default_rng(512)
+    n, count = 64, 512
+    stacks = [rng.uniform(1.0, 100.0, n) for _ in range(count)]
+    rep_s = [s for s in stacks for _ in range(n)]
+    bp = icm_amd.BatchPlan(rep_s, [n] * len(rep_s))
+    dP = torch.tensor(np.tile(np.arange(n, 0, -1, dtype=np.float64), len(rep_s)), device=dev)
+    dV = torch.tensor(np.tile(np.eye(n).ravel(), count), dtype=torch.float64, device=dev)
+    dE = torch.empty(len(rep_s) * n, dtype=torch.float64, device=dev)
+    dD = torch.empty_like(dE)
+    bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+    bp.vjp_device(dP.data_ptr(), dV.data_ptr(), dE.data_ptr(), dD.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bp.run_device(dP.data_ptr(), dE.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        bp.vjp_device(dP.data_ptr(), dV.data_ptr(), dE.data_ptr(), dD.data_ptr(), Q)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    bp.close()
+    fwd_ms = (t1 - t0) * 1e3 / reps
+    jac_ms = (t2 - t1) * 1e3 / reps
+    return {"instances": count, "n": n, "replicas": len(rep_s), "jacobians_ms": jac_ms,
+            "forward_same_batch_ms": fwd_ms, "ratio": jac_ms / fwd_ms,
+            "jacobians_per_s": count / (jac_ms * 1e-3)}
+
+
 # forward + adjoint: one pass returns E and dS = d(v.E)/dS; ratio to the forward
 def fwd_vjp_ratio(icm_amd, torch, dev, Q, Sx, px, reps):
     pl = icm_amd.Plan(Sx, len(px))
@@ -269,8 +301,19 @@ def main():
                for k_, v in sorted(kst.items(), key=lambda kv: -kv[1][1])}
     log("kernels (per step): " + json.dumps(kernels, indent=1))
 
-    subset = adjoint = batched = None
+    subset = adjoint = batched = pcie = None
     if args.only == "all":
+        # the same pass through the host-buffer entry (payout upload and equity
+        # download over PCIe inside the call)
+        plan.run(p, Q)
+        t_h = time.perf_counter()
+        for _ in range(2):
+            plan.run(p, Q)
+        host_ms = (time.perf_counter() - t_h) * 1e3 / 2
+        pcie = {"entry": "icm_hip_equity_with_plan (host payout and equity)", "ms_per_step": host_ms,
+                "value": n / (host_ms * 1e-3)}
+        log(f"pcie-inclusive: {pcie}")
+
         # subset variant of the same workload: 64 random target players, pruned tree
         rng = np.random.default_rng(11)
         targets = rng.choice(n, 64, replace=False)
@@ -294,6 +337,7 @@ def main():
     plan.close()
     if args.only == "all":
         adjoint["n1048576_k1048576"] = fwd_vjp_ratio(icm_amd, torch, dev, Q, S, p, 2)
+        adjoint["jacobians_512_n64"] = jacobian_field(icm_amd, torch, dev, Q)
         log(f"adjoint: {adjoint}")
         batched = batched_field(icm_amd, torch, dev, Q)
         log(f"batched: {batched}")
@@ -322,6 +366,7 @@ def main():
         "subset": subset,
         "adjoint": adjoint,
         "batched": batched,
+        "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
