@@ -46,6 +46,41 @@ template <int FS> struct TileLayout {
     __device__ static __forceinline__ int at(int item, int idx) { return item * IS + idx + (idx >> 4); }
 };
 
+// Unpadded, XOR-swizzled tile for the 512-thread (radix-8) kernels, whose
+// accesses are always runs of consecutive elements of one transform or
+// Stockham writes with stride R*Ns.  Measured on gfx950 (LDS probe,
+// SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE): ds_write_b128 is conflict-free
+// when each group of 8 lanes hits 8 distinct 16-byte units mod 8, and
+// ds_read_b128 when each group of 16 lanes hits 16 distinct units mod 16;
+// padding a contiguous run (idx + idx/16) makes the reads conflict 2-way.
+// Element idx of transform it lives at it*FS + (idx ^ ((idx >> 3) & 7) ^ g(it)):
+//   * the low-3-bit XOR by idx bits 3..5 turns the stride-2/4/8 writes of the
+//     first passes into 8 distinct units and permutes runs only inside their
+//     aligned 8-blocks (contiguous reads stay conflict-free);
+//   * g(it) spreads the transforms of FS = 16, 32, 64, where one 8- or
+//     16-lane group spans 2..8 transforms (the passes with FS/R < 16).
+template <int FS> struct SwzLayout {
+    static constexpr int IS = FS;
+    static constexpr int ITEMS = kTile / FS;
+    static constexpr int SIZE = kTile;
+    __device__ static __forceinline__ int g(int item) {
+        if constexpr (FS == 16) {
+            const int b0 = item & 1, b1 = (item >> 1) & 1, b2 = (item >> 2) & 1;
+            return ((b0 ^ b2) << 3) | (b1 << 2) | (b0 << 1);
+        } else if constexpr (FS == 32) {
+            const int b0 = item & 1, b1 = (item >> 1) & 1;
+            return ((b0 ^ b1) << 3) | (b0 << 2);
+        } else if constexpr (FS == 64) {
+            return (item & 1) << 3;
+        } else {
+            return 0;
+        }
+    }
+    __device__ static __forceinline__ int at(int item, int idx) {
+        return item * FS + (idx ^ ((idx >> 3) & 7) ^ g(item));
+    }
+};
+
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
@@ -166,7 +201,7 @@ __device__ __forceinline__ void twiddle(double2 *v, const double2 *tw, int k, in
 // One Stockham pass of radix R over all transforms of size FS in the tile,
 // executed by NT threads (each owning kTile / NT points).
 // Ns = product of the radices of earlier passes.
-template <int FS, int R, bool INV, int NT>
+template <int FS, int R, bool INV, int NT, class TL>
 __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 *tw, int twShift) {
     constexpr int NBI = FS / R;               // butterflies per transform
     constexpr int LOG_NBI = ilog2c(NBI);
@@ -181,7 +216,7 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
         item[u] = bj >> LOG_NBI;
         jj[u] = bj & (NBI - 1);
 #pragma unroll
-        for (int s = 0; s < R; ++s) v[u][s] = lds[TileLayout<FS>::at(item[u], jj[u] + s * NBI)];
+        for (int s = 0; s < R; ++s) v[u][s] = lds[TL::at(item[u], jj[u] + s * NBI)];
     }
     __syncthreads();
     const int twSh = 12 - (logNs + ilog2c(R));   // into the 4096-entry part
@@ -192,7 +227,7 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
         dft<R, INV>(v[u]);
         const int d0 = (jj[u] - k) * R + k;
 #pragma unroll
-        for (int j = 0; j < R; ++j) lds[TileLayout<FS>::at(item[u], d0 + dft_out<R>(j) * Ns)] = v[u][j];
+        for (int j = 0; j < R; ++j) lds[TL::at(item[u], d0 + dft_out<R>(j) * Ns)] = v[u][j];
     }
     __syncthreads();
 }
@@ -202,7 +237,7 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
 // when log2(FS) is not a multiple of the main radix's log.
 // Starts with a barrier so callers may write the tile and call directly;
 // ends with a barrier (the last pass's).
-template <int FS, bool INV, int NT = kThreads>
+template <int FS, bool INV, int NT = kThreads, class TL = TileLayout<FS>>
 __device__ __forceinline__ void fft_tile(double2 *lds, const double2 *tw, int twShift) {
     constexpr int RMAX = kTile / NT;          // 16 or 8
     constexpr int LR = ilog2c(RMAX);
@@ -210,12 +245,12 @@ __device__ __forceinline__ void fft_tile(double2 *lds, const double2 *tw, int tw
     constexpr int REM = LOG % LR;
     __syncthreads();
     int logNs = 0;
-    if constexpr (REM == 1) { fft_pass<FS, 2, INV, NT>(lds, logNs, tw, twShift); logNs += 1; }
-    if constexpr (REM == 2) { fft_pass<FS, 4, INV, NT>(lds, logNs, tw, twShift); logNs += 2; }
-    if constexpr (REM == 3) { fft_pass<FS, 8, INV, NT>(lds, logNs, tw, twShift); logNs += 3; }
+    if constexpr (REM == 1) { fft_pass<FS, 2, INV, NT, TL>(lds, logNs, tw, twShift); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass<FS, 4, INV, NT, TL>(lds, logNs, tw, twShift); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass<FS, 8, INV, NT, TL>(lds, logNs, tw, twShift); logNs += 3; }
 #pragma unroll
     for (int p = 0; p < LOG / LR; ++p) {
-        fft_pass<FS, RMAX, INV, NT>(lds, logNs, tw, twShift);
+        fft_pass<FS, RMAX, INV, NT, TL>(lds, logNs, tw, twShift);
         logNs += LR;
     }
 }

@@ -675,7 +675,7 @@ __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int 
 constexpr int kLT = 512;
 constexpr int kLP = kTile / kLT;
 // Largest padded tile (F = 16): LDS of the kernels that switch over F.
-constexpr int kTileMaxSize = TileLayout<16>::SIZE;
+constexpr int kTileMaxSize = SwzLayout<16>::SIZE;
 
 // Tile item -> (q-plane, pair of parents).
 struct PlainMap {   // one q-plane per workgroup, consecutive pairs
@@ -702,7 +702,7 @@ struct FlatMap {    // v = q * npair + pair over (q-plane, pair), from v0
 template <int F, class Map>
 __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                int twShift) {
-    using TL = TileLayout<F>;
+    using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -719,7 +719,7 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
             M.at(item, qi, pr);
             lds[TL::at(item, m)] = load_child_pair(J, qi, 2 * pr + which, m);
         }
-        fft_tile<F, false, kLT>(lds, tw, twShift);
+        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
 #pragma unroll
         for (int u = 0; u < kLP / 2; ++u) {
             const int ps = t + u * kLT;
@@ -741,7 +741,7 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
         }
         if (which == 0) __syncthreads();
     }
-    fft_tile<F, true, kLT>(lds, tw, twShift);
+    fft_tile<F, true, kLT, TL>(lds, tw, twShift);
     const double invF = 1.0 / (double)F;
     const int ps_ = J.parent.stride, pc = J.parent.count;
 #pragma unroll
@@ -771,7 +771,7 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
 
 template <int F>
 __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[TileLayout<F>::SIZE];
+    __shared__ double2 lds[SwzLayout<F>::SIZE];
     build_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw
 template <int F, class Map>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift) {
-    using TL = TileLayout<F>;
+    using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -808,7 +808,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         }
         lds[TL::at(item, m)] = z;
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 G1[kLP / 2], G2[kLP / 2];
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -836,7 +836,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             M.at(item, qi, pr);
             lds[TL::at(item, m)] = load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
         }
-        fft_tile<F, false, kLT>(lds, tw, twShift);
+        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
 #pragma unroll
         for (int u = 0; u < kLP / 2; ++u) {
             const int ps = t + u * kLT;
@@ -854,7 +854,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
                 lds[ib] = pack_ai(c_conj(A), c_conj(Bv));
             }
         }
-        fft_tile<F, true, kLT>(lds, tw, twShift);
+        fft_tile<F, true, kLT, TL>(lds, tw, twShift);
 #pragma unroll
         for (int u = 0; u < kLP; ++u) {
             const int e = t + u * kLT;
@@ -880,7 +880,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
 
 template <int F>
 __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[TileLayout<F>::SIZE];
+    __shared__ double2 lds[SwzLayout<F>::SIZE];
     prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
@@ -965,7 +965,7 @@ __device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, i
 template <int F, class Map>
 __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                  int twShift) {
-    using TL = TileLayout<F>;
+    using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -979,7 +979,7 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
         const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 XL[kLP / 2], XR[kLP / 2];
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -999,7 +999,7 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
         const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
         const int ps = t + u * kLT;
@@ -1017,7 +1017,7 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
             lds[ib] = pack_ai(c_conj(Pv), c_conj(Dv));
         }
     }
-    fft_tile<F, true, kLT>(lds, tw, twShift);
+    fft_tile<F, true, kLT, TL>(lds, tw, twShift);
     const double invF = 1.0 / (double)F;
     const int pc = J.parent.count, cc = J.child.count;
 #pragma unroll
@@ -1061,7 +1061,7 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
 template <int F, class Map>
 __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                 int twShift) {
-    using TL = TileLayout<F>;
+    using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -1080,7 +1080,7 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
         }
         lds[TL::at(item, m)] = z;
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 GA[kLP / 2], GB[kLP / 2];
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -1101,7 +1101,7 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
         const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.child, qi, p, m, J.nr_child, true);
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 XL[kLP / 2], XR[kLP / 2];
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -1122,7 +1122,7 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
         const int p = (pr < J.nr_parent) ? pr : -1;
         lds[TL::at(item, m)] = load_view_pair(J.tchild, qi, p, m, J.nr_child, false);
     }
-    fft_tile<F, false, kLT>(lds, tw, twShift);
+    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 U2a[kLP / 2], U2b[kLP / 2];   // B_L + i B_R at bins jb and F - jb
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -1148,7 +1148,7 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
             U2b[u] = pack_ai(c_conj(BL), c_conj(BR));
         }
     }
-    fft_tile<F, true, kLT>(lds, tw, twShift);
+    fft_tile<F, true, kLT, TL>(lds, tw, twShift);
     const double invF = 1.0 / (double)F;
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
@@ -1171,7 +1171,7 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
         lds[TL::at(item, jb)] = U2a[u];
         lds[TL::at(item, jb == 0 ? HALF : F - jb)] = U2b[u];
     }
-    fft_tile<F, true, kLT>(lds, tw, twShift);
+    fft_tile<F, true, kLT, TL>(lds, tw, twShift);
     const int cc = J.child.count;
 #pragma unroll
     for (int u = 0; u < kLP; ++u) {
@@ -1205,13 +1205,13 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
 
 template <int F>
 __global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[TileLayout<F>::SIZE];
+    __shared__ double2 lds[SwzLayout<F>::SIZE];
     build_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 template <int F>
 __global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[TileLayout<F>::SIZE];
+    __shared__ double2 lds[SwzLayout<F>::SIZE];
     prop_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
@@ -1383,7 +1383,7 @@ __device__ __forceinline__ int mirror_k2(int k1, int k2) {
 constexpr int kRT = 512;
 constexpr int kRP = kTile / kRT;
 
-using RTL = TileLayout<kF2>;   // one row per tile, T1 starts at RTL::IS
+using RTL = SwzLayout<kF2>;   // one row per tile, T1 starts at RTL::IS
 
 __device__ __forceinline__ int row_at(int k2) { return RTL::at(0, k2); }
 __device__ __forceinline__ int mirror_idx(int k1, int k1m, int k2) {
@@ -1400,8 +1400,8 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
 #pragma unroll
         for (int u = 0; u < kRP; ++u) T[RTL::IS + row_at(t + u * kRT)] = Zb[(long long)k1m * kF2 + t + u * kRT];
     }
-    fft_tile<kF2, false, kRT>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, false, kRT>(T + RTL::IS, tw, twShift);
+    fft_tile<kF2, false, kRT, RTL>(T, tw, twShift);
+    if (k1 != k1m) fft_tile<kF2, false, kRT, RTL>(T + RTL::IS, tw, twShift);
 }
 
 // Inverse-transform the tiles in place and store rows k1 / k1m of V with the
@@ -1410,8 +1410,8 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
                                                    const double2 *tw, int twShift) {
     const int t = threadIdx.x;
     const int F = 1 << logF;
-    fft_tile<kF2, true, kRT>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, true, kRT>(T + RTL::IS, tw, twShift);
+    fft_tile<kF2, true, kRT, RTL>(T, tw, twShift);
+    if (k1 != k1m) fft_tile<kF2, true, kRT, RTL>(T + RTL::IS, tw, twShift);
 #pragma unroll 2
     for (int u = 0; u < kRP; ++u) {
         const int e = t + u * kRT;
