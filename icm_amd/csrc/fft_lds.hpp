@@ -40,6 +40,7 @@ __device__ __forceinline__ double2 tw_big(const double2 *tw, int twShift, int m,
 }
 
 template <int FS> struct TileLayout {
+    static constexpr bool kItemFast = false;        // butterflies of one transform on consecutive lanes
     static constexpr int IS = FS + FS / 16 + 1;     // item stride (odd)
     static constexpr int ITEMS = kTile / FS;
     static constexpr int SIZE = ITEMS * IS;         // complex elements of LDS
@@ -60,6 +61,7 @@ template <int FS> struct TileLayout {
 //   * g(it) spreads the transforms of FS = 16, 32, 64, where one 8- or
 //     16-lane group spans 2..8 transforms (the passes with FS/R < 16).
 template <int FS> struct SwzLayout {
+    static constexpr bool kItemFast = false;
     static constexpr int IS = FS;
     static constexpr int ITEMS = kTile / FS;
     static constexpr int SIZE = kTile;
@@ -79,6 +81,20 @@ template <int FS> struct SwzLayout {
     __device__ static __forceinline__ int at(int item, int idx) {
         return item * FS + (idx ^ ((idx >> 3) & 7) ^ g(item));
     }
+};
+
+// Row-major tile for the column kernels: element idx of transform it at
+// idx * ITEMS + it, and the passes give consecutive lanes consecutive
+// transforms at the same butterfly (kItemFast).  Loads, stores and every
+// pass read / write then touch runs of consecutive elements: no bank
+// conflicts and no padding.  Needs ITEMS >= 8 (FS <= 512).
+template <int FS> struct ColLayout {
+    static constexpr bool kItemFast = true;
+    static constexpr int ITEMS = kTile / FS;
+    static constexpr int IS = 1;
+    static constexpr int SIZE = kTile;
+    static_assert(ITEMS >= 8, "column layout needs 8 or more transforms per tile");
+    __device__ static __forceinline__ int at(int item, int idx) { return idx * ITEMS + item; }
 };
 
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -213,8 +229,14 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int bj = t + u * NT;
-        item[u] = bj >> LOG_NBI;
-        jj[u] = bj & (NBI - 1);
+        if constexpr (TL::kItemFast) {
+            constexpr int LOG_ITEMS = ilog2c(kTile / FS);
+            item[u] = bj & ((kTile / FS) - 1);
+            jj[u] = bj >> LOG_ITEMS;
+        } else {
+            item[u] = bj >> LOG_NBI;
+            jj[u] = bj & (NBI - 1);
+        }
 #pragma unroll
         for (int s = 0; s < R; ++s) v[u][s] = lds[TL::at(item[u], jj[u] + s * NBI)];
     }

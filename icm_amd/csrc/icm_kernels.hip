@@ -1312,7 +1312,7 @@ struct StGChildren {      // array a = slot (propagate)
 template <int F1, class Loader>
 __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
                                                  const double2 *tw, int twShift) {
-    using TL = TileLayout<F1>;
+    using TL = ColLayout<F1>;
     __shared__ double2 lds[TL::SIZE];
     constexpr int TC = kTile / F1;
     constexpr int LOGTC = ilog2c(TC);
@@ -1326,7 +1326,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict_
         const int n = r * kF2 + tile * TC + c;
         lds[TL::at(c, r)] = ld(a, qi, n);
     }
-    fft_tile<F1, false>(lds, tw, twShift);
+    fft_tile<F1, false, kThreads, TL>(lds, tw, twShift);
     double2 *o = out + ((long long)qi * nArr + a) * F;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -1341,7 +1341,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict_
 template <int F1, class Storer>
 __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in, Storer st, int nArr, int logF,
                                                  const double2 *tw, int twShift) {
-    using TL = TileLayout<F1>;
+    using TL = ColLayout<F1>;
     __shared__ double2 lds[TL::SIZE];
     constexpr int TC = kTile / F1;
     constexpr int LOGTC = ilog2c(TC);
@@ -1355,7 +1355,7 @@ __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in,
         const int r = e >> LOGTC, c = e & (TC - 1);
         lds[TL::at(c, r)] = src[r * kF2 + tile * TC + c];
     }
-    fft_tile<F1, true>(lds, tw, twShift);
+    fft_tile<F1, true, kThreads, TL>(lds, tw, twShift);
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int e = t + u * kThreads;
