@@ -102,6 +102,8 @@ _SIGS = [
     ("icm_hip_equity_vjp", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _D, _D]),
     ("icm_hip_node_vjp", ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, _D, _D]),
     ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_set_schoolbook_max", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_get_schoolbook_max", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
@@ -118,6 +120,7 @@ _SIGS = [
     ("icm_hip_batch_summary", ctypes.c_int,
      [ctypes.c_void_p, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
     ("icm_hip_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_batch_set_schoolbook_max", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_batch_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_batch_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
@@ -290,7 +293,7 @@ class Plan:
     """Reusable plan (IcmGpuPlan, src/gpu/icm_gpu.h:79-111): sort, geometry and
     device memory are set up once for (stacks, k); ``run`` takes Q and payout."""
 
-    def __init__(self, stacks, k, block_size=0, q_batch=0, verbose=False):
+    def __init__(self, stacks, k, block_size=0, q_batch=0, verbose=False, schoolbook_max=None):
         _auto_init()
         self._lib = load_library()
         s = _to_double_array(stacks, "stacks")
@@ -304,6 +307,17 @@ class Plan:
         if not self._h:
             raise _err(self._lib, "icm_hip_plan_create")
         self.stats = RunStats()
+        if schoolbook_max is not None:
+            self.set_schoolbook_max(schoolbook_max)
+
+    def set_schoolbook_max(self, cmax):
+        """Levels whose children keep <= cmax coefficients run as direct sums."""
+        if self._lib.icm_hip_plan_set_schoolbook_max(self._h, int(cmax)) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_schoolbook_max")
+
+    @property
+    def schoolbook_max(self):
+        return self._lib.icm_hip_plan_get_schoolbook_max(self._h)
 
     def summary(self):
         sm = PlanSummary()
@@ -508,6 +522,11 @@ class BatchPlan:
         out = {k: v.value for k, v in zip(keys, vals)}
         out["device_bytes"] = nbytes.value
         return out
+
+    def set_schoolbook_max(self, cmax):
+        """Schoolbook / transform crossover of the forward levels."""
+        if self._lib.icm_hip_batch_set_schoolbook_max(self._h, int(cmax)) != 0:
+            raise _err(self._lib, "icm_hip_batch_set_schoolbook_max")
 
     def set_profiling(self, on=True):
         if self._lib.icm_hip_batch_set_profiling(self._h, int(bool(on))) != 0:

@@ -1,0 +1,167 @@
+"""MI355X autotuner: picks the planner's block size B and schoolbook /
+transform crossover per (n, k) bucket from timings on the device, and writes
+the compiled-in tuning table (icm_amd/csrc/tuning_mi355x.h).
+
+It replaces the reference's per-device dispatch calibration
+(tools/calibrate_gpu.cu, tools/calibrate_gpu_best_b.cu ->
+devices/b200/gpu_fft_config.h): instead of fitting a cost model of vendor FFT
+tiers, it times the engine's own launch tiers (schoolbook sums for levels
+whose children keep c <= sb_max coefficients, LDS transforms, four-step
+transforms) end to end.  Acceptance rule (the reference's calibration
+check): with the chosen configurations the pass time must not decrease as n
+or k grows along the grid.
+
+Usage (on a GPU host, after building):
+    python -m icm_amd.autotune                 # full grid, writes the header
+    python -m icm_amd.autotune --quick --dry-run
+Rebuild (python -m icm_amd.build) afterwards to compile the new table in.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+import icm_amd
+
+HEADER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "tuning_mi355x.h")
+SB_CANDIDATES = [0, 17, 33, 65, 129, 257, 512]
+
+
+def workload(n, k, seed=3):
+    rng = np.random.default_rng(seed)
+    S = np.exp(rng.uniform(0.0, math.log(1e9), n))
+    p = np.arange(k, 0, -1, dtype=np.float64)
+    return S, p / p.sum()
+
+
+def time_plan(plan, p, Q, reps):
+    plan.run(p, Q)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.run(p, Q)
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)) * 1e3
+
+
+def k_values(n):
+    ks = sorted({1, 9, max(1, n // 10), n})
+    return [k for k in ks if k <= n]
+
+
+def grid(quick):
+    ns = [1024, 16384, 131072] if quick else [1024, 4096, 16384, 65536, 262144, 1048576]
+    return [(n, k) for n in ns for k in k_values(n)]
+
+
+def tune_shape(n, k, Q, reps, log):
+    S, p = workload(n, k)
+    best = None
+    results = []
+    for B in ([16, 32] if n <= 64 else [32, 64]):
+        plan = icm_amd.Plan(S, k, block_size=B)
+        for sb in SB_CANDIDATES:
+            plan.set_schoolbook_max(sb)
+            ms = time_plan(plan, p, Q, reps)
+            results.append({"B": B, "sb_max": sb, "ms": ms})
+            if best is None or ms < best["ms"]:
+                best = {"B": B, "sb_max": sb, "ms": ms}
+        plan.close()
+    log(f"n={n:8d} k={k:8d}: best B={best['B']} sb_max={best['sb_max']} {best['ms']:.3f} ms "
+        f"(transforms only, B=32: {next(r['ms'] for r in results if r['B'] == 32 and r['sb_max'] == 0):.3f} ms)")
+    return best, results
+
+
+def check_monotone(table):
+    """Pass time must not decrease with n (fixed k fraction) or with k (fixed n)."""
+    bad = []
+    by_n = {}
+    for (n, k), v in table.items():
+        by_n.setdefault(n, []).append((k, v["ms"]))
+    for n, row in by_n.items():
+        row.sort()
+        for (k1, t1), (k2, t2) in zip(row, row[1:]):
+            if t2 < t1 * 0.97:
+                bad.append(f"n={n}: k={k2} ({t2:.3f} ms) faster than k={k1} ({t1:.3f} ms)")
+    ns = sorted(by_n)
+    for n1, n2 in zip(ns, ns[1:]):
+        for frac in ("1", "n"):
+            k1 = 1 if frac == "1" else n1
+            k2 = 1 if frac == "1" else n2
+            t1, t2 = table[(n1, k1)]["ms"], table[(n2, k2)]["ms"]
+            if t2 < t1 * 0.97:
+                bad.append(f"k={frac}: n={n2} ({t2:.3f} ms) faster than n={n1} ({t1:.3f} ms)")
+    return bad
+
+
+def write_header(table, path):
+    ns = sorted({n for n, _ in table})
+    lines = ["// tuning_mi355x.h -- planner table for MI355X (gfx950), written by",
+             "// icm_amd/autotune.py from timings on the device.  First row whose limits",
+             "// cover (n, k) wins.", "#pragma once", "",
+             "struct TuneRow {", "    int n_max, k_max;",
+             "    int B;          // block size (players per leaf block)",
+             "    int sb_max;     // schoolbook for levels with c_{l-1} <= sb_max", "};", "",
+             "static const TuneRow kTuneTable[] = {"]
+    for i, n in enumerate(ns):
+        n_max = 0x7fffffff if i == len(ns) - 1 else (n + ns[i + 1]) // 2
+        ks = sorted(k for nn, k in table if nn == n)
+        for j, k in enumerate(ks):
+            k_max = 0x7fffffff if j == len(ks) - 1 else (k + ks[j + 1]) // 2
+            v = table[(n, k)]
+            lines.append(f"    {{{n_max}, {k_max}, {v['B']}, {v['sb_max']}}},   // tuned at n={n} k={k}: "
+                         f"{v['ms']:.3f} ms")
+    lines += ["};", ""]
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--quick", action="store_true", help="small grid (three n values)")
+    ap.add_argument("--Q", type=int, default=256)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--dry-run", action="store_true", help="print the table, do not write the header")
+    ap.add_argument("--json", default="", help="also write all timings to this file")
+    ap.add_argument("--from-json", default="", help="write the header from a timings file of an earlier run")
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    if args.from_json:
+        with open(args.from_json) as f:
+            best = json.load(f)["best"]
+        table = {tuple(int(x) for x in key.split(",")): v for key, v in best.items()}
+        bad = check_monotone(table)
+        log("acceptance: " + ("PASS" if not bad else f"{len(bad)} violations: {bad}"))
+        write_header(table, HEADER)
+        log(f"wrote {HEADER}")
+        return 0 if not bad else 1
+
+    icm_amd.init()
+    table, every = {}, {}
+    for n, k in grid(args.quick):
+        best, results = tune_shape(n, k, args.Q, args.reps, log)
+        table[(n, k)] = best
+        every[f"{n},{k}"] = results
+    bad = check_monotone(table)
+    for b in bad:
+        log(f"monotonicity: {b}")
+    log("acceptance: " + ("PASS (time monotone in n and k)" if not bad else f"{len(bad)} violations"))
+    if args.json:
+        with open(args.json, "w") as f:
+            json.dump({"best": {f"{n},{k}": v for (n, k), v in table.items()}, "all": every,
+                       "monotone_violations": bad}, f, indent=1)
+    if not args.dry_run:
+        write_header(table, HEADER)
+        log(f"wrote {HEADER}; rebuild with python -m icm_amd.build")
+    return 0 if not bad else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -42,6 +42,7 @@ struct IcmHipBatchPlan {
     int qb = 0;                          // q-planes per pass
     int nchunk = 1;
     int twShift = 12;
+    int sb_max = 0;                      // schoolbook levels: c_{l-1} <= sb_max
     bool use_graph = true;
     bool verbose = false;
     // arena offsets (doubles) per level and instance, sized for qb planes
@@ -68,9 +69,12 @@ struct IcmHipBatchPlan {
         int nq = 0;
         icm::LevelJob *d_jobs = nullptr;
         int2 *d_items = nullptr;
-        // per level l: build items [boff_l, boff_l + bcnt_l), prop items likewise
+        // per level l: build items [b_off[l], b_off[l] + b_cnt[l]), prop items
+        // likewise; transform levels, then schoolbook levels (bs_*, ps_*)
         std::vector<int> b_off, b_cnt, p_off, p_cnt;
         std::vector<double> b_bytes, b_flops, p_bytes, p_flops;
+        std::vector<int> bs_off, bs_cnt, ps_off, ps_cnt;
+        std::vector<double> bs_bytes, bs_flops, ps_bytes, ps_flops;
     };
     std::vector<Pass> passes;            // index 0: nq = qb, 1: the remainder (if any)
     // adjoint pass (allocated on the first VJP call)
@@ -264,15 +268,22 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
     const int Lm = p->Lmax;
     std::vector<icm::LevelJob> jobs;
     std::vector<int2> items;
-    ps.b_off.assign(Lm, 0); ps.b_cnt.assign(Lm, 0); ps.p_off.assign(Lm, 0); ps.p_cnt.assign(Lm, 0);
-    ps.b_bytes.assign(Lm, 0); ps.b_flops.assign(Lm, 0); ps.p_bytes.assign(Lm, 0); ps.p_flops.assign(Lm, 0);
-    auto add_items = [&](int job, int npair, int F) {
-        const long long nv = (long long)nq * npair;
-        const int per = icm::kMaxLdsF / F;
-        for (long long v0 = 0; v0 < nv; v0 += per) items.push_back(make_int2(job, (int)v0));
+    for (auto *v : {&ps.b_off, &ps.b_cnt, &ps.p_off, &ps.p_cnt, &ps.bs_off, &ps.bs_cnt, &ps.ps_off, &ps.ps_cnt})
+        v->assign(Lm, 0);
+    for (auto *v : {&ps.b_bytes, &ps.b_flops, &ps.p_bytes, &ps.p_flops, &ps.bs_bytes, &ps.bs_flops, &ps.ps_bytes,
+                    &ps.ps_flops})
+        v->assign(Lm, 0.0);
+    // work items of one job: tile items over its (q-plane, unit) space, `per` units each
+    auto add_items = [&](std::vector<int2> &dst, int job, int units, int per) {
+        const long long nv = (long long)nq * units;
+        for (long long v0 = 0; v0 < nv; v0 += per) dst.push_back(make_int2(job, (int)v0));
+    };
+    auto sb_ok = [&](const Geometry &g, int l) {
+        const int cc = g.c[l - 1];
+        return cc <= p->sb_max && cc <= icm::kSbMaxC && icm::sb_stride_prop(cc, g.G[l - 1]) <= icm::kSbLds;
     };
     for (int l = 1; l <= Lm - 2; ++l) {
-        ps.b_off[l] = (int)items.size();
+        std::vector<int2> fft, sb;
         for (int i = 0; i < p->ninst; ++i) {
             const Geometry &g = p->geo[i];
             if (l > g.L - 2) continue;
@@ -285,16 +296,28 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
-            const int npair = (g.n_real[l] + 1) / 2;
-            add_items((int)jobs.size(), npair, J.F);
+            const double by = 8.0 * nq * ((double)g.n_real[l - 1] * g.c[l - 1] + (double)g.n_real[l] * g.c[l]);
+            if (sb_ok(g, l)) {
+                add_items(sb, (int)jobs.size(), g.n_real[l], icm::sb_ppw(icm::sb_stride_build(g.c[l - 1])));
+                ps.bs_bytes[l] += by;
+                ps.bs_flops[l] += 2.0 * nq * (double)g.n_real[l] * g.c[l - 1] * g.c[l];
+            } else {
+                const int npair = (g.n_real[l] + 1) / 2;
+                add_items(fft, (int)jobs.size(), npair, icm::kMaxLdsF / J.F);
+                ps.b_bytes[l] += by;
+                ps.b_flops[l] += (double)nq * npair * 3.0 * fft5(J.F);
+            }
             jobs.push_back(J);
-            ps.b_bytes[l] += 8.0 * nq * ((double)g.n_real[l - 1] * g.c[l - 1] + (double)g.n_real[l] * g.c[l]);
-            ps.b_flops[l] += (double)nq * npair * 3.0 * fft5(J.F);
         }
-        ps.b_cnt[l] = (int)items.size() - ps.b_off[l];
+        ps.b_off[l] = (int)items.size();
+        ps.b_cnt[l] = (int)fft.size();
+        items.insert(items.end(), fft.begin(), fft.end());
+        ps.bs_off[l] = (int)items.size();
+        ps.bs_cnt[l] = (int)sb.size();
+        items.insert(items.end(), sb.begin(), sb.end());
     }
     for (int l = Lm - 1; l >= 1; --l) {
-        ps.p_off[l] = (int)items.size();
+        std::vector<int2> fft, sb;
         for (int i = 0; i < p->ninst; ++i) {
             const Geometry &g = p->geo[i];
             if (l > g.L - 1) continue;
@@ -309,14 +332,27 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
-            const int npair = (g.n_real[l] + 1) / 2;
-            add_items((int)jobs.size(), npair, J.F);
-            jobs.push_back(J);
             const double gin = (l == g.L - 1) ? 8.0 * g.k : 8.0 * nq * (double)g.n_real[l] * g.G[l];
-            ps.p_bytes[l] += gin + 8.0 * nq * (double)g.n_real[l - 1] * (g.c[l - 1] + g.G[l - 1]);
-            ps.p_flops[l] += (double)nq * npair * 5.0 * fft5(J.F);
+            const double by = gin + 8.0 * nq * (double)g.n_real[l - 1] * (g.c[l - 1] + g.G[l - 1]);
+            if (sb_ok(g, l)) {
+                add_items(sb, (int)jobs.size(), g.n_real[l],
+                          icm::sb_ppw(icm::sb_stride_prop(g.c[l - 1], g.G[l - 1])));
+                ps.ps_bytes[l] += by;
+                ps.ps_flops[l] += 4.0 * nq * (double)g.n_real[l] * g.c[l - 1] * g.G[l - 1];
+            } else {
+                const int npair = (g.n_real[l] + 1) / 2;
+                add_items(fft, (int)jobs.size(), npair, icm::kMaxLdsF / J.F);
+                ps.p_bytes[l] += by;
+                ps.p_flops[l] += (double)nq * npair * 5.0 * fft5(J.F);
+            }
+            jobs.push_back(J);
         }
-        ps.p_cnt[l] = (int)items.size() - ps.p_off[l];
+        ps.p_off[l] = (int)items.size();
+        ps.p_cnt[l] = (int)fft.size();
+        items.insert(items.end(), fft.begin(), fft.end());
+        ps.ps_off[l] = (int)items.size();
+        ps.ps_cnt[l] = (int)sb.size();
+        items.insert(items.end(), sb.begin(), sb.end());
     }
     if (items.size() > 0x7fffffffULL) { set_err("batch too large: work items exceed 2^31"); return false; }
     if (!balloc(p, &ps.d_jobs, jobs.size()) || !balloc(p, &ps.d_items, items.size())) return false;
@@ -419,6 +455,15 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
     if (forcedQ > 0) qb = std::min(qb, forcedQ);
     p->qb = qb;
+    {
+        // one crossover for the batch: the table entry of its largest instance
+        int nmax = 1, kmax = 1;
+        for (int i = 0; i < n_inst; ++i) {
+            nmax = std::max(nmax, p->geo[i].n);
+            kmax = std::max(kmax, p->geo[i].k);
+        }
+        p->sb_max = tuned_schoolbook_max(nmax, kmax);
+    }
     p->coff.assign(Lm, std::vector<long long>(n_inst, 0));
     p->goff.assign(Lm, std::vector<long long>(n_inst, 0));
     p->coef_elems.assign(Lm, 0);
@@ -565,12 +610,18 @@ bool enqueue(IcmHipBatchPlan *p, int Q) {
         for (int l = 1; l <= p->Lmax - 2; ++l)
             if (!hip_ok(icm::launch_level_batch(true, ps.d_jobs, ps.d_items + ps.b_off[l], ps.b_cnt[l], p->d_tw,
                                                 p->twShift, s, &p->prof, ps.b_bytes[l], ps.b_flops[l]),
-                        "batch build level"))
+                        "batch build level") ||
+                !hip_ok(icm::launch_level_sb_batch(true, ps.d_jobs, ps.d_items + ps.bs_off[l], ps.bs_cnt[l], s,
+                                                   &p->prof, ps.bs_bytes[l], ps.bs_flops[l]),
+                        "batch schoolbook build level"))
                 return false;
         for (int l = p->Lmax - 1; l >= 1; --l)
             if (!hip_ok(icm::launch_level_batch(false, ps.d_jobs, ps.d_items + ps.p_off[l], ps.p_cnt[l], p->d_tw,
                                                 p->twShift, s, &p->prof, ps.p_bytes[l], ps.p_flops[l]),
-                        "batch propagate level"))
+                        "batch propagate level") ||
+                !hip_ok(icm::launch_level_sb_batch(false, ps.d_jobs, ps.d_items + ps.ps_off[l], ps.ps_cnt[l], s,
+                                                   &p->prof, ps.ps_bytes[l], ps.ps_flops[l]),
+                        "batch schoolbook propagate level"))
                 return false;
         const int nchunk = std::min(p->nchunk, nq);
         const int qpc = (nq + nchunk - 1) / nchunk;
@@ -760,6 +811,16 @@ int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const in
     const int r = icm_hip_batch_equity(p, Q, payout, equity, nullptr);
     batch_free(p);
     return r;
+}
+
+int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->sb_max = cmax < 0 ? 0 : cmax;
+    drop_graphs(plan);                  // job tables change with the crossover
+    free_passes(plan->passes);
+    free_passes(plan->vpasses);
+    plan->nodes_Q = 0;
+    return 0;
 }
 
 int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on) {

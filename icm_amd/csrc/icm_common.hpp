@@ -85,11 +85,23 @@ inline bool make_geometry(int n, int k, int B, Geometry &g) {
     return true;
 }
 
+// Schoolbook / transform crossover: levels whose children keep at most this
+// many coefficients run as direct sums.  Chosen by icm_amd/autotune.py on
+// MI355X (tuning table in tuning_mi355x.h).
+#include "tuning_mi355x.h"
+
+inline const TuneRow &tune_row(int n, int k) {
+    for (const TuneRow &r : kTuneTable)
+        if (n <= r.n_max && k <= r.k_max) return r;
+    return kTuneTable[sizeof(kTuneTable) / sizeof(kTuneTable[0]) - 1];
+}
+
+inline int tuned_schoolbook_max(int n, int k) { return tune_row(n, k).sb_max; }
+
 inline int pick_B(int n, int k, int forced) {
     if (forced == 16 || forced == 32 || forced == 64) return forced;
-    (void)k;
-    if (n <= 16) return 16;
-    return 32;
+    const int B = tune_row(n, k).B;
+    return (B == 16 || B == 32 || B == 64) ? B : 32;
 }
 
 // Per-launch event timing, aggregated per kernel name (see icm::Prof).

@@ -37,6 +37,7 @@ struct IcmHipPlan {
     int qb = 0;                         // q-planes per batch
     int nchunk = 1;                     // q-chunks of the leaf extract
     int twShift = 0;
+    int sb_max = 0;                     // levels with c_{l-1} <= sb_max use schoolbook sums
     std::vector<int> perm;              // sorted position -> original index
     std::vector<double> S_sorted;
     double Smax = 0.0;
@@ -173,6 +174,7 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
         }
     }
     p->qb = qb;
+    p->sb_max = tuned_schoolbook_max(n, k);
 
     int maxF = 16;
     for (int l = 1; l < g.L; ++l) maxF = std::max(maxF, g.F[l]);
@@ -231,6 +233,15 @@ bool ensure_nodes(IcmHipPlan *p, int Qa) {
     if (!dalloc(p, &p->d_nodes, (size_t)2 * Qa)) return false;
     p->q_cap = Qa;
     return true;
+}
+
+// Level l (1 .. L-1) runs as schoolbook sums when its children keep few
+// coefficients (c_{l-1} <= sb_max, within the kernels' LDS stage).
+bool use_schoolbook(const IcmHipPlan *p, int l) {
+    const Geometry &g = p->geo;
+    const int cc = g.c[l - 1];
+    return cc <= p->sb_max && cc <= icm::kSbMaxC && icm::sb_stride_build(cc) <= icm::kSbLds &&
+           icm::sb_stride_prop(cc, g.G[l - 1]) <= icm::kSbLds;
 }
 
 // Runs the whole quadrature for the active nodes with device payout; leaves
@@ -302,7 +313,9 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
-            hipError_t e = (J.F <= icm::kMaxLdsF)
+            hipError_t e = use_schoolbook(p, l)
+                               ? icm::launch_build_sb(J, s, &p->prof)
+                           : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
             if (!hip_ok(e, "build level")) return -1;
@@ -332,7 +345,9 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
-            hipError_t e = (J.F <= icm::kMaxLdsF)
+            hipError_t e = use_schoolbook(p, l)
+                               ? icm::launch_prop_sb(J, s, &p->prof)
+                           : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
             if (!hip_ok(e, "propagate level")) return -1;
@@ -882,6 +897,14 @@ int icm_hip_plan_set_targets(IcmHipPlan *plan, const int *targets, int n_targets
     }
     return set_targets(plan, targets, n_targets);
 }
+
+int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->sb_max = cmax < 0 ? 0 : cmax;
+    return 0;
+}
+
+int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan) { return plan ? plan->sb_max : -1; }
 
 int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }

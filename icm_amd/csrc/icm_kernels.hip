@@ -597,6 +597,11 @@ __device__ __forceinline__ double2 pack_ai(double2 A, double2 Bv) {
     return make_double2(A.x - Bv.y, A.y + Bv.x);
 }
 
+// Coefficient m of node `node` in q-plane qi.
+__device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, int m) {
+    return V.base[(long long)qi * V.plane + (long long)node * V.stride + m];
+}
+
 // Parent handled by slot i of a propagate launch (-1: none).
 __device__ __forceinline__ int slot_parent(const LevelJob &J, int i) {
     if (i >= J.np) return -1;
@@ -885,6 +890,168 @@ __global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw,
 }
 
 // ---------------------------------------------------------------------------
+// Schoolbook levels (few coefficients per node).  Direct sums, no transform:
+//   build      P[m]  = sum_t L[t] R[m-t],           m < c_parent
+//   propagate  gL[m] = sum_t R[t] g[m+t], gR[m] = sum_t L[t] g[m+t],  m < G_child
+// A workgroup of 256 threads stages the operands of ppw parents (tile items
+// given by the same maps as the transform kernels, "pair" = parent) in LDS;
+// a thread owns 4 consecutive outputs of one parent and slides a 4-wide
+// register window over R (build) or g (propagate), so each LDS load feeds
+// 4 (build) or 8 (propagate) FMAs and the broadcast coefficient is shared by
+// the lanes of a parent.  Zero padding in LDS replaces bounds tests in the
+// inner loop.  A missing right child is the identity polynomial.
+// ---------------------------------------------------------------------------
+constexpr int kSbT = 256;
+
+template <class Map>
+__device__ __forceinline__ void build_sb_body(const LevelJob &J, double *sh, const Map &M, int ppw) {
+    const int t = threadIdx.x;
+    const int cc = J.child.count, pc = J.parent.count;
+    const int stride = sb_stride_build(cc);     // L[cc] | 4 zeros, R[cc], 6 zeros
+    for (int e = t; e < ppw * stride; e += kSbT) {
+        const int sl = e / stride, x = e - sl * stride;
+        int qi, p;
+        M.at(sl, qi, p);
+        double v = 0.0;
+        if (p < J.nr_parent) {
+            const bool has_r = (2 * p + 1) < J.nr_child;
+            if (x < cc) {
+                v = view_at(J.child, qi, 2 * p, x);
+            } else {
+                const int r = x - cc - 4;          // R index
+                if (r >= 0 && r < cc) v = has_r ? view_at(J.child, qi, 2 * p + 1, r) : (r == 0 ? 1.0 : 0.0);
+            }
+        }
+        sh[e] = v;
+    }
+    __syncthreads();
+    const int nqd = (pc + 3) >> 2;
+    for (int task = t; task < ppw * nqd; task += kSbT) {
+        const int sl = task / nqd, m0 = (task - sl * nqd) * 4;
+        int qi, p;
+        M.at(sl, qi, p);
+        if (p >= J.nr_parent) continue;
+        const double *L = sh + sl * stride;
+        const double *Rz = L + cc + 4;             // Rz[x] = R[x], zero for x in [-4, -1] and [cc, cc+5]
+        const int tlo = max(0, m0 - cc + 1), thi = min(cc - 1, m0 + 3);
+        double r0 = Rz[m0 - tlo], r1 = Rz[m0 + 1 - tlo], r2 = Rz[m0 + 2 - tlo], r3 = Rz[m0 + 3 - tlo];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int tt = tlo; tt <= thi; ++tt) {
+            const double l = L[tt];
+            a0 = fma(l, r0, a0);
+            a1 = fma(l, r1, a1);
+            a2 = fma(l, r2, a2);
+            a3 = fma(l, r3, a3);
+            r3 = r2;
+            r2 = r1;
+            r1 = r0;
+            r0 = Rz[m0 - tt - 1];
+        }
+        double *out = J.parent.base + (long long)qi * J.parent.plane + (long long)p * J.parent.stride;
+        if (m0 < pc) put(&out[m0], a0, J.acc_out);
+        if (m0 + 1 < pc) put(&out[m0 + 1], a1, J.acc_out);
+        if (m0 + 2 < pc) put(&out[m0 + 2], a2, J.acc_out);
+        if (m0 + 3 < pc) put(&out[m0 + 3], a3, J.acc_out);
+    }
+}
+
+template <class Map>
+__device__ __forceinline__ void prop_sb_body(const LevelJob &J, double *sh, const Map &M, int ppw) {
+    const int t = threadIdx.x;
+    const int cc = J.child.count, gc = J.gchild.count, gp = J.gpar.count;
+    const int glen = gc + cc + 8;
+    const int stride = sb_stride_prop(cc, gc);  // L[cc], R[cc], g[gc + cc + 8]
+    for (int e = t; e < ppw * stride; e += kSbT) {
+        const int sl = e / stride, x = e - sl * stride;
+        int qi, pr;
+        M.at(sl, qi, pr);
+        const int p = slot_parent(J, pr);
+        double v = 0.0;
+        if (p >= 0) {
+            const bool has_r = (2 * p + 1) < J.nr_child;
+            if (x < cc) {
+                v = view_at(J.child, qi, 2 * p, x);
+            } else if (x < 2 * cc) {
+                const int r = x - cc;
+                v = has_r ? view_at(J.child, qi, 2 * p + 1, r) : (r == 0 ? 1.0 : 0.0);
+            } else {
+                const int gx = x - 2 * cc;
+                if (gx < gp && gx < glen) v = view_at(J.gpar, qi, p, gx);
+            }
+        }
+        sh[e] = v;
+    }
+    __syncthreads();
+    const int nqd = (gc + 3) >> 2;
+    for (int task = t; task < ppw * nqd; task += kSbT) {
+        const int sl = task / nqd, m0 = (task - sl * nqd) * 4;
+        int qi, pr;
+        M.at(sl, qi, pr);
+        const int p = slot_parent(J, pr);
+        if (p < 0) continue;
+        const double *L = sh + sl * stride;
+        const double *R = L + cc;
+        const double *g = R + cc;
+        double w0 = g[m0], w1 = g[m0 + 1], w2 = g[m0 + 2], w3 = g[m0 + 3];
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        for (int tt = 0; tt < cc; ++tt) {
+            const double rr = R[tt], ll = L[tt];
+            l0 = fma(rr, w0, l0);
+            l1 = fma(rr, w1, l1);
+            l2 = fma(rr, w2, l2);
+            l3 = fma(rr, w3, l3);
+            q0 = fma(ll, w0, q0);
+            q1 = fma(ll, w1, q1);
+            q2 = fma(ll, w2, q2);
+            q3 = fma(ll, w3, q3);
+            w0 = w1;
+            w1 = w2;
+            w2 = w3;
+            w3 = g[m0 + 4 + tt];
+        }
+        double *gl = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        double *gr = gl + J.gchild.stride;
+        const double vl[4] = {l0, l1, l2, l3}, vr[4] = {q0, q1, q2, q3};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (m0 + i < gc) {
+                put(&gl[m0 + i], vl[i], J.acc_out);
+                if (has_r) put(&gr[m0 + i], vr[i], J.acc_out);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_build_sb(LevelJob J, int ppw) {
+    __shared__ double sh[kSbLds];
+    build_sb_body(J, sh, PlainMap{(int)blockIdx.x * ppw, (int)blockIdx.y}, ppw);
+}
+
+__global__ __launch_bounds__(256) void k_prop_sb(LevelJob J, int ppw) {
+    __shared__ double sh[kSbLds];
+    prop_sb_body(J, sh, PlainMap{(int)blockIdx.x * ppw, (int)blockIdx.y}, ppw);
+}
+
+__global__ __launch_bounds__(256) void k_build_sb_batch(const LevelJob *__restrict__ jobs,
+                                                        const int2 *__restrict__ items) {
+    __shared__ double sh[kSbLds];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const int ppw = sb_ppw(sb_stride_build(J.child.count));
+    build_sb_body(J, sh, FlatMap{it.y, J.nr_parent, J.nq * J.nr_parent}, ppw);
+}
+
+__global__ __launch_bounds__(256) void k_prop_sb_batch(const LevelJob *__restrict__ jobs,
+                                                       const int2 *__restrict__ items) {
+    __shared__ double sh[kSbLds];
+    const int2 it = items[blockIdx.x];
+    const LevelJob J = jobs[it.x];
+    const int ppw = sb_ppw(sb_stride_prop(J.child.count, J.gchild.count));
+    prop_sb_body(J, sh, FlatMap{it.y, J.np, J.nq * J.np}, ppw);
+}
+
+// ---------------------------------------------------------------------------
 // Ragged batches: one launch per tree level over every instance that has the
 // level.  Workgroup w takes work item items[w] = (job, v0): instance job's
 // level, tile items v0 .. v0 + 4096/F - 1 over its (q-plane, pair) space, so
@@ -955,10 +1122,6 @@ __device__ __forceinline__ double2 load_view_pair(const CoefView &V, int qi, int
     }
     if (!has_r && m == 0 && missing_is_one) z.y = 1.0;
     return z;
-}
-
-__device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, int m) {
-    return V.base[(long long)qi * V.plane + (long long)node * V.stride + m];
 }
 
 // Item maps as the forward kernels, with one parent per item ("pair" = parent).
@@ -1915,6 +2078,45 @@ hipError_t launch_level_t_batch(bool build, const LevelJob *jobs, const int2 *it
     else k_prop_lds_t_batch<<<nitems, kLT, 0, s>>>(jobs, items, tw, twShift);
     hipError_t e = hipGetLastError();
     PF_END(build ? "k_build_lds_t_batch" : "k_prop_lds_t_batch", bytes, flops);
+    return e;
+}
+
+hipError_t launch_build_sb(const LevelJob &J, hipStream_t s, Prof *pf) {
+    if (J.child.count > kSbMaxC || sb_stride_build(J.child.count) > kSbLds) return hipErrorInvalidValue;
+    PF_BEGIN();
+    const int ppw = sb_ppw(sb_stride_build(J.child.count));
+    dim3 grid((J.nr_parent + ppw - 1) / ppw, J.nq);
+    k_build_sb<<<grid, kSbT, 0, s>>>(J, ppw);
+    hipError_t e = hipGetLastError();
+    const double bytes = level_coef_bytes(J.child, J.nr_child, J.nq) + 8.0 * J.nq * (double)J.nr_parent * J.parent.count;
+    PF_END(kname("k_build_sb<c%d>", J.child.count), bytes,
+           2.0 * J.nq * (double)J.nr_parent * J.child.count * J.parent.count);
+    return e;
+}
+
+hipError_t launch_prop_sb(const LevelJob &J, hipStream_t s, Prof *pf) {
+    if (J.child.count > kSbMaxC || sb_stride_prop(J.child.count, J.gchild.count) > kSbLds)
+        return hipErrorInvalidValue;
+    PF_BEGIN();
+    const int ppw = sb_ppw(sb_stride_prop(J.child.count, J.gchild.count));
+    dim3 grid((J.np + ppw - 1) / ppw, J.nq);
+    k_prop_sb<<<grid, kSbT, 0, s>>>(J, ppw);
+    hipError_t e = hipGetLastError();
+    const double gin = (J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)J.np * J.gpar.count);
+    const double bytes = gin + 8.0 * J.nq * (double)J.np * 2.0 * (J.child.count + J.gchild.count);
+    PF_END(kname("k_prop_sb<c%d>", J.child.count), bytes,
+           4.0 * J.nq * (double)J.np * J.child.count * J.gchild.count);
+    return e;
+}
+
+hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, hipStream_t s,
+                                 Prof *pf, double bytes, double flops) {
+    if (nitems < 1) return hipSuccess;
+    PF_BEGIN();
+    if (build) k_build_sb_batch<<<nitems, kSbT, 0, s>>>(jobs, items);
+    else k_prop_sb_batch<<<nitems, kSbT, 0, s>>>(jobs, items);
+    hipError_t e = hipGetLastError();
+    PF_END(build ? "k_build_sb_batch" : "k_prop_sb_batch", bytes, flops);
     return e;
 }
 

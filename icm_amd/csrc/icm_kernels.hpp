@@ -163,4 +163,22 @@ hipError_t launch_level_t_batch(bool build, const LevelJob *jobs, const int2 *it
 
 constexpr int kMaxLdsF = 4096;
 
+// Schoolbook levels: direct FP64 sums for levels whose children keep few
+// coefficients.  A workgroup stages the operands of `ppw` parents in
+// kSbLds doubles of LDS; the counts below are shared by host and device.
+constexpr int kSbLds = 4096;
+constexpr int kSbMaxC = 512;    // largest child coefficient count the kernels take
+__host__ __device__ inline int sb_stride_build(int cc) { return 2 * cc + 10; }
+__host__ __device__ inline int sb_stride_prop(int cc, int gc) { return 2 * cc + gc + cc + 8; }
+__host__ __device__ inline int sb_ppw(int stride) {
+    const int p = kSbLds / stride;
+    return p < 1 ? 1 : (p > 64 ? 64 : p);
+}
+
+// One tree level by schoolbook sums (children count c_{l-1} <= kSbMaxC).
+hipError_t launch_build_sb(const LevelJob &J, hipStream_t s, Prof *pf);
+hipError_t launch_prop_sb(const LevelJob &J, hipStream_t s, Prof *pf);
+hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, hipStream_t s,
+                                 Prof *pf, double bytes, double flops);
+
 }  // namespace icm

@@ -164,6 +164,13 @@ void icm_hip_make_nodes(int Q, double Smax, double *logv, double *w);
  * stream and clears the aggregates; kernel_stat reports, per kernel name,
  * launches, summed device time and summed compulsory bytes / FLOPs. */
 int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on);
+/* Planner override (the autotuner's knob; replaces the reference's per-device
+ * dispatch calibration, devices/b200/gpu_fft_config.h): levels whose children
+ * keep <= cmax coefficients run as direct (schoolbook) sums instead of
+ * transforms (cmax <= 512 takes effect; 0 = transforms only).  The default
+ * comes from the MI355X tuning table. */
+int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax);
+int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_count(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
                              double *total_ns, double *alg_bytes, double *alg_flops);
@@ -203,6 +210,9 @@ int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const in
                          const double *payout, double *equity);
 int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_players, int *n_blocks,
                           int *max_levels, int *q_batch, size_t *device_bytes);
+/* Schoolbook / transform crossover of the batch's forward levels (as
+ * icm_hip_plan_set_schoolbook_max; the adjoint keeps transforms). */
+int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax);
 /* Per-kernel timing of batch runs (runs eagerly, without the graph, while on). */
 int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on);
 int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan);
