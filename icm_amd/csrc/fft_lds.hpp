@@ -87,13 +87,14 @@ template <int FS> struct SwzLayout {
 // idx * ITEMS + it, and the passes give consecutive lanes consecutive
 // transforms at the same butterfly (kItemFast).  Loads, stores and every
 // pass read / write then touch runs of consecutive elements: no bank
-// conflicts and no padding.  Needs ITEMS >= 8 (FS <= 512).
+// conflicts and no padding while ITEMS >= 8 (FS <= 512); wider columns
+// (FS = 1024 .. 4096, 4 .. 1 per tile) stay correct with 2..8-way write
+// conflicts in their first passes.
 template <int FS> struct ColLayout {
     static constexpr bool kItemFast = true;
     static constexpr int ITEMS = kTile / FS;
     static constexpr int IS = 1;
     static constexpr int SIZE = kTile;
-    static_assert(ITEMS >= 8, "column layout needs 8 or more transforms per tile");
     __device__ static __forceinline__ int at(int item, int idx) { return idx * ITEMS + item; }
 };
 

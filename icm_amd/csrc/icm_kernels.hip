@@ -1908,6 +1908,7 @@ static hipError_t col_fwd(int F1, Loader ld, double2 *out, int nArr, int nq, int
 #define ICM_CF(V) case V: k_col_fwd<V, Loader><<<grid, kThreads, 0, s>>>(ld, out, nArr, logF, tw, twShift); break;
     switch (F1) {
         ICM_CF(2) ICM_CF(4) ICM_CF(8) ICM_CF(16) ICM_CF(32) ICM_CF(64) ICM_CF(128) ICM_CF(256) ICM_CF(512)
+        ICM_CF(1024) ICM_CF(2048) ICM_CF(4096)
     default: return hipErrorInvalidValue;
     }
 #undef ICM_CF
@@ -1921,6 +1922,7 @@ static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq
 #define ICM_CI(V) case V: k_col_inv<V, Storer><<<grid, kThreads, 0, s>>>(in, st, nArr, logF, tw, twShift); break;
     switch (F1) {
         ICM_CI(2) ICM_CI(4) ICM_CI(8) ICM_CI(16) ICM_CI(32) ICM_CI(64) ICM_CI(128) ICM_CI(256) ICM_CI(512)
+        ICM_CI(1024) ICM_CI(2048) ICM_CI(4096)
     default: return hipErrorInvalidValue;
     }
 #undef ICM_CI

@@ -216,3 +216,22 @@ def test_reference_python_semantics():
     np.testing.assert_allclose(icm_amd.equity(S, [1.0]), np.array(S) / sum(S), rtol=1e-10)
     sub = icm_amd.equity_subset([1000, 2000, 3000, 4000], [0.5, 0.3, 0.2], [1, 3])
     assert sub[0] == 0.0 and sub[2] == 0.0
+
+
+def test_wide_column_levels_v1_closed_form():
+    """n = 2^21 + 3, k = n: the top level transforms 2^22 points (1024-point
+    columns); linear payout n - r gives the exact V1 equities."""
+    n = (1 << 21) + 3
+    rng = np.random.default_rng(2021)
+    S = rng.uniform(1.0, 100.0, n)
+    p = np.arange(n, 0, -1, dtype=np.float64)
+    say(f"wide columns n={n} k=n")
+    plan = icm_amd.Plan(S, n)
+    geo = icm_amd.plan_geometry(n, n, plan.summary()["B"])
+    assert max(geo["F"]) >= (1 << 22)
+    e = plan.run(p, Q=256)
+    plan.close()
+    idx = rng.choice(n, 64, replace=False)
+    for i in idx:
+        v1 = 1.0 + np.sum(S[i] / (S[i] + np.delete(S, i)))
+        assert abs(e[i] - v1) / v1 < 1e-9, (i, e[i], v1)
