@@ -1472,8 +1472,13 @@ struct StGChildren {      // array a = slot (propagate)
     }
 };
 
+// Column kernels: 512 threads (8 points each, radix-8 passes), so two 64 KB
+// workgroups keep 16 waves per CU in flight for the HBM-bound passes.
+constexpr int kCT = 512;
+constexpr int kCP = kTile / kCT;
+
 template <int F1, class Loader>
-__global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
+__global__ __launch_bounds__(kCT) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
                                                  const double2 *tw, int twShift) {
     using TL = ColLayout<F1>;
     __shared__ double2 lds[TL::SIZE];
@@ -1483,17 +1488,17 @@ __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict_
     const int t = threadIdx.x;
     const int F = 1 << logF;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kCP; ++u) {
+        const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n = r * kF2 + tile * TC + c;
         lds[TL::at(c, r)] = ld(a, qi, n);
     }
-    fft_tile<F1, false, kThreads, TL>(lds, tw, twShift);
+    fft_tile<F1, false, kCT, TL>(lds, tw, twShift);
     double2 *o = out + ((long long)qi * nArr + a) * F;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kCP; ++u) {
+        const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n2 = tile * TC + c;
         const double2 w = tw_big(tw, twShift, (n2 * r) & (F - 1), logF);
@@ -1502,7 +1507,7 @@ __global__ __launch_bounds__(256) void k_col_fwd(Loader ld, double2 *__restrict_
 }
 
 template <int F1, class Storer>
-__global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in, Storer st, int nArr, int logF,
+__global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in, Storer st, int nArr, int logF,
                                                  const double2 *tw, int twShift) {
     using TL = ColLayout<F1>;
     __shared__ double2 lds[TL::SIZE];
@@ -1513,21 +1518,20 @@ __global__ __launch_bounds__(256) void k_col_inv(const double2 *__restrict__ in,
     const int F = 1 << logF;
     const double2 *src = in + ((long long)qi * nArr + a) * F;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kCP; ++u) {
+        const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
         lds[TL::at(c, r)] = src[r * kF2 + tile * TC + c];
     }
-    fft_tile<F1, true, kThreads, TL>(lds, tw, twShift);
+    fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int e = t + u * kThreads;
+    for (int u = 0; u < kCP; ++u) {
+        const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
         st(a, qi, r * kF2 + tile * TC + c, lds[TL::at(c, r)]);
     }
 }
 
-// ---------------------------------------------------------------------------
 // Fused spectrum stage of a large level: one workgroup owns the row pair
 // {k1, F1 - k1} (rows 0 and F1/2 pair with themselves).  Every bin -K of row
 // k1 lies in row F1 - k1 (position F2-1-k2, or (F2-k2) mod F2 in row 0), so
@@ -1905,7 +1909,7 @@ template <class Loader>
 static hipError_t col_fwd(int F1, Loader ld, double2 *out, int nArr, int nq, int logF, const double2 *tw,
                           int twShift, hipStream_t s) {
     dim3 grid(F1, nArr, nq);  // tiles = F2 / TC = F1
-#define ICM_CF(V) case V: k_col_fwd<V, Loader><<<grid, kThreads, 0, s>>>(ld, out, nArr, logF, tw, twShift); break;
+#define ICM_CF(V) case V: k_col_fwd<V, Loader><<<grid, kCT, 0, s>>>(ld, out, nArr, logF, tw, twShift); break;
     switch (F1) {
         ICM_CF(2) ICM_CF(4) ICM_CF(8) ICM_CF(16) ICM_CF(32) ICM_CF(64) ICM_CF(128) ICM_CF(256) ICM_CF(512)
         ICM_CF(1024) ICM_CF(2048) ICM_CF(4096)
@@ -1919,7 +1923,7 @@ template <class Storer>
 static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq, int logF, const double2 *tw,
                           int twShift, hipStream_t s) {
     dim3 grid(F1, nArr, nq);
-#define ICM_CI(V) case V: k_col_inv<V, Storer><<<grid, kThreads, 0, s>>>(in, st, nArr, logF, tw, twShift); break;
+#define ICM_CI(V) case V: k_col_inv<V, Storer><<<grid, kCT, 0, s>>>(in, st, nArr, logF, tw, twShift); break;
     switch (F1) {
         ICM_CI(2) ICM_CI(4) ICM_CI(8) ICM_CI(16) ICM_CI(32) ICM_CI(64) ICM_CI(128) ICM_CI(256) ICM_CI(512)
         ICM_CI(1024) ICM_CI(2048) ICM_CI(4096)
