@@ -775,7 +775,7 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
 }
 
 template <int F>
-__global__ __launch_bounds__(512) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
+__global__ __launch_bounds__(512, 4) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[SwzLayout<F>::SIZE];
     build_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
@@ -884,7 +884,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
 }
 
 template <int F>
-__global__ __launch_bounds__(512) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
+__global__ __launch_bounds__(512, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[SwzLayout<F>::SIZE];
     prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
