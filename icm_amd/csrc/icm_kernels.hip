@@ -101,8 +101,10 @@ __device__ __forceinline__ double leaf_psi(const double *P, const double *g, dou
 template <int B>
 __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     constexpr int BPW = 256 / B;
-    __shared__ double shP[BPW * (B + 1)];
-    __shared__ double shG[BPW * B];
+    constexpr int NP = BPW * (B + 1);          // 256 + BPW block coefficients per q-plane
+    static_assert(BPW * B == 256, "one g entry per thread");
+    __shared__ double shP[2][NP];
+    __shared__ double shG[2][256];
     const int t = threadIdx.x;
     const int lb = t / B, li = t % B;
     const int sfirst = blockIdx.x * BPW;   // first block slot of this workgroup
@@ -118,28 +120,45 @@ __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     const double Sj = valid ? J.S[j] : 0.0;
     const bool act = valid && (!J.active || J.active[j]);
     const long long bplane = (long long)J.nb * (B + 1);
+    // this thread's share of a q-plane's operands: P elements t and t + 256, g element t
+    const int e1 = t + 256;
+    const bool has1 = e1 < NP;
+    const int bP0 = block_of(sfirst + t / (B + 1)), mP0 = t % (B + 1);
+    const int bP1 = has1 ? block_of(sfirst + e1 / (B + 1)) : -1, mP1 = e1 % (B + 1);
+    const int bG = block_of(sfirst + t / B), mG = t % B;
+    const bool gok = bG >= 0 && mG < J.G0;
+    double p0 = 0.0, p1 = 0.0, g0 = 0.0;
+    auto fetch = [&](int q) {
+        p0 = (bP0 >= 0) ? J.blocks[q * bplane + (long long)bP0 * (B + 1) + mP0] : 0.0;
+        p1 = (bP1 >= 0) ? J.blocks[q * bplane + (long long)bP1 * (B + 1) + mP1] : 0.0;
+        g0 = gok ? J.g[q * J.g_plane + (long long)bG * J.g_stride + mG] : 0.0;
+    };
+    auto stash = [&](int buf) {
+        shP[buf][t] = p0;
+        if (has1) shP[buf][e1] = p1;
+        shG[buf][t] = g0;
+    };
+    if (q0 < q1) {
+        fetch(q0);
+        stash(0);
+    }
+    __syncthreads();
     double acc = 0.0;
+    // double-buffered: plane q+1's loads are in flight while plane q is computed
     for (int q = q0; q < q1; ++q) {
-        for (int e = t; e < BPW * (B + 1); e += 256) {
-            const int bb = block_of(sfirst + e / (B + 1));
-            shP[e] = (bb >= 0) ? J.blocks[q * bplane + (long long)bb * (B + 1) + e % (B + 1)] : 0.0;
-        }
-        for (int e = t; e < BPW * B; e += 256) {
-            const int m = e % B;
-            const int bb = block_of(sfirst + e / B);
-            shG[e] = (bb >= 0 && m < J.G0)
-                         ? J.g[q * J.g_plane + (long long)bb * J.g_stride + m] : 0.0;
-        }
-        __syncthreads();
+        const int cur = (q - q0) & 1;
+        const bool more = q + 1 < q1;
+        if (more) fetch(q + 1);
         if (act) {
             const double lv = J.logv[q];
             const double a = J.a_in ? J.a_in[(long long)q * J.n + j] : a_of(Sj, lv);
-            const double eq = leaf_psi<B>(shP + lb * (B + 1), shG + lb * B, a, bsize, pk);
+            const double eq = leaf_psi<B>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk);
             if (J.inner_out) J.inner_out[(long long)q * J.n + j] = eq;
             double pw = J.wq[q] * Sj * a * exp(-lv);
             if (!isfinite(pw)) pw = 0.0;
             acc += pw * eq;
         }
+        if (more) stash(cur ^ 1);
         __syncthreads();
     }
     if (valid && J.partial) J.partial[(long long)chunk * J.n + j] = act ? acc : 0.0;
@@ -194,8 +213,10 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__r
                                                             const double *__restrict__ blocks, int NB, int nq,
                                                             int q0, int q_per_chunk, double *partial, int Ntot) {
     constexpr int BPW = 256 / B;
-    __shared__ double shP[BPW * (B + 1)];
-    __shared__ double shG[BPW * B];
+    constexpr int NP = BPW * (B + 1);
+    static_assert(BPW * B == 256, "one g entry per thread");
+    __shared__ double shP[2][NP];
+    __shared__ double shG[2][256];
     const int t = threadIdx.x;
     const int lb = t / B, li = t % B;
     const int gfirst = blockIdx.x * BPW;
@@ -211,30 +232,51 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__r
     const int pk = I ? min(I->G0, bsize) : 0;
     const double Sj = valid ? S[I->poff + j] : 0.0;
     const long long bplane = (long long)NB * (B + 1);
+    // this thread's share of a q-plane's operands: P elements t and t + 256, g element t
+    const int e1 = t + 256;
+    const bool has1 = e1 < NP;
+    const int gP0 = gfirst + t / (B + 1), mP0 = t % (B + 1);
+    const int gP1 = has1 ? gfirst + e1 / (B + 1) : NB, mP1 = e1 % (B + 1);
+    const int gG = gfirst + t / B, mG = t % B;
+    const double *gbase = nullptr;
+    long long gplane = 0;
+    if (gG < NB) {
+        const BatchInst &I2 = inst[blk_inst[gG]];
+        if (mG < I2.G0) {
+            gbase = I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG;
+            gplane = I2.g0.plane;
+        }
+    }
+    double p0 = 0.0, p1 = 0.0, g0 = 0.0;
+    auto fetch = [&](int q) {
+        p0 = (gP0 < NB) ? blocks[q * bplane + (long long)gP0 * (B + 1) + mP0] : 0.0;
+        p1 = (gP1 < NB) ? blocks[q * bplane + (long long)gP1 * (B + 1) + mP1] : 0.0;
+        g0 = gbase ? gbase[q * gplane] : 0.0;
+    };
+    auto stash = [&](int buf) {
+        shP[buf][t] = p0;
+        if (has1) shP[buf][e1] = p1;
+        shG[buf][t] = g0;
+    };
+    if (qa < qe) {
+        fetch(qa);
+        stash(0);
+    }
+    __syncthreads();
     double acc = 0.0;
     for (int q = qa; q < qe; ++q) {
-        for (int e = t; e < BPW * (B + 1); e += 256) {
-            const int g2 = gfirst + e / (B + 1);
-            shP[e] = (g2 < NB) ? blocks[q * bplane + (long long)g2 * (B + 1) + e % (B + 1)] : 0.0;
-        }
-        for (int e = t; e < BPW * B; e += 256) {
-            const int g2 = gfirst + e / B, m = e % B;
-            double v = 0.0;
-            if (g2 < NB) {
-                const BatchInst &I2 = inst[blk_inst[g2]];
-                if (m < I2.G0) v = I2.g0.base[q * I2.g0.plane + (long long)(g2 - I2.boff) * I2.g0.stride + m];
-            }
-            shG[e] = v;
-        }
-        __syncthreads();
+        const int cur = (q - qa) & 1;
+        const bool more = q + 1 < qe;
+        if (more) fetch(q + 1);
         if (valid) {
             const double lv = I->logv[q0 + q];
             const double a = a_of(Sj, lv);
-            const double eq = leaf_psi<B>(shP + lb * (B + 1), shG + lb * B, a, bsize, pk);
+            const double eq = leaf_psi<B>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk);
             double pw = I->w[q0 + q] * Sj * a * exp(-lv);
             if (!isfinite(pw)) pw = 0.0;
             acc += pw * eq;
         }
+        if (more) stash(cur ^ 1);
         __syncthreads();
     }
     if (valid) partial[(long long)chunk * Ntot + I->poff + j] = acc;
