@@ -76,6 +76,32 @@ def tune_shape(n, k, Q, reps, log):
     return best, results
 
 
+def tune_batch(Q, reps, log, count=4096):
+    """Block size and crossover of ragged batches, timed on the ragged
+    benchmark mix (icm_amd.workloads.ragged_instances, 4096 instances)."""
+    from icm_amd.workloads import ragged_instances
+    stacks, pays = ragged_instances(count, seed=4)
+    ks = [len(x) for x in pays]
+    flat = np.concatenate(pays)
+    best, results = None, []
+    for B in (32, 64):
+        bp = icm_amd.BatchPlan(stacks, ks, block_size=B)
+        for sb in SB_CANDIDATES:
+            bp.set_schoolbook_max(sb)
+            bp.run_flat(flat, Q)
+            ts = []
+            for _ in range(reps):
+                bp.run_flat(flat, Q)
+                ts.append(bp.stats.total_ns / 1e6)
+            ms = float(np.median(ts))
+            results.append({"B": B, "sb_max": sb, "ms": ms})
+            if best is None or ms < best["ms"]:
+                best = {"B": B, "sb_max": sb, "ms": ms}
+        bp.close()
+    log(f"ragged batch ({count} instances): best B={best['B']} sb_max={best['sb_max']} {best['ms']:.3f} ms")
+    return best, results
+
+
 def check_monotone(table):
     """Pass time must not decrease with n (fixed k fraction) or with k (fixed n)."""
     bad = []
@@ -98,7 +124,7 @@ def check_monotone(table):
     return bad
 
 
-def write_header(table, path):
+def write_header(table, path, batch=None):
     ns = sorted({n for n, _ in table})
     lines = ["// tuning_mi355x.h -- planner table for MI355X (gfx950), written by",
              "// icm_amd/autotune.py from timings on the device.  First row whose limits",
@@ -116,6 +142,10 @@ def write_header(table, path):
             lines.append(f"    {{{n_max}, {k_max}, {v['B']}, {v['sb_max']}}},   // tuned at n={n} k={k}: "
                          f"{v['ms']:.3f} ms")
     lines += ["};", ""]
+    b = batch or {"B": 32, "sb_max": 129, "ms": float("nan")}
+    lines += ["// Ragged batches (one setting per batch), tuned on the 4096-instance ragged mix.",
+              "struct BatchTune {", "    int B, sb_max;", "};",
+              f"static const BatchTune kBatchTune = {{{b['B']}, {b['sb_max']}}};   // {b['ms']:.3f} ms", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))
 
@@ -135,11 +165,11 @@ def main(argv=None):
 
     if args.from_json:
         with open(args.from_json) as f:
-            best = json.load(f)["best"]
-        table = {tuple(int(x) for x in key.split(",")): v for key, v in best.items()}
+            data = json.load(f)
+        table = {tuple(int(x) for x in key.split(",")): v for key, v in data["best"].items()}
         bad = check_monotone(table)
         log("acceptance: " + ("PASS" if not bad else f"{len(bad)} violations: {bad}"))
-        write_header(table, HEADER)
+        write_header(table, HEADER, data.get("batch_best"))
         log(f"wrote {HEADER}")
         return 0 if not bad else 1
 
@@ -153,12 +183,13 @@ def main(argv=None):
     for b in bad:
         log(f"monotonicity: {b}")
     log("acceptance: " + ("PASS (time monotone in n and k)" if not bad else f"{len(bad)} violations"))
+    batch_best, batch_all = tune_batch(args.Q, args.reps, log)
     if args.json:
         with open(args.json, "w") as f:
             json.dump({"best": {f"{n},{k}": v for (n, k), v in table.items()}, "all": every,
-                       "monotone_violations": bad}, f, indent=1)
+                       "monotone_violations": bad, "batch_best": batch_best, "batch_all": batch_all}, f, indent=1)
     if not args.dry_run:
-        write_header(table, HEADER)
+        write_header(table, HEADER, batch_best)
         log(f"wrote {HEADER}; rebuild with python -m icm_amd.build")
     return 0 if not bad else 1
 

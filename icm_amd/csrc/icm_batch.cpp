@@ -389,7 +389,7 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     p->verbose = opts && opts->verbose;
     p->use_graph = !opts || opts->enable_graphs;
     const int forcedB = opts ? opts->force_uncached_fused_levels : 0;
-    p->B = (forcedB == 16 || forcedB == 32 || forcedB == 64) ? forcedB : 32;
+    p->B = (forcedB == 16 || forcedB == 32 || forcedB == 64) ? forcedB : kBatchTune.B;
     const int B = p->B;
     p->geo.resize(n_inst);
     p->boff.assign(n_inst, 0);
@@ -455,15 +455,7 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
     if (forcedQ > 0) qb = std::min(qb, forcedQ);
     p->qb = qb;
-    {
-        // one crossover for the batch: the table entry of its largest instance
-        int nmax = 1, kmax = 1;
-        for (int i = 0; i < n_inst; ++i) {
-            nmax = std::max(nmax, p->geo[i].n);
-            kmax = std::max(kmax, p->geo[i].k);
-        }
-        p->sb_max = tuned_schoolbook_max(nmax, kmax);
-    }
+    p->sb_max = kBatchTune.sb_max;   // one crossover per batch (tuning table)
     p->coff.assign(Lm, std::vector<long long>(n_inst, 0));
     p->goff.assign(Lm, std::vector<long long>(n_inst, 0));
     p->coef_elems.assign(Lm, 0);
