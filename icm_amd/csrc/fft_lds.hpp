@@ -22,6 +22,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace icm {
 
 constexpr int kTile = 4096;   // complex points per workgroup tile
@@ -253,6 +255,92 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
         for (int j = 0; j < R; ++j) lds[TL::at(item[u], d0 + dft_out<R>(j) * Ns)] = v[u][j];
     }
     __syncthreads();
+}
+
+// Transforms whose first pass reads its inputs through a loader (global
+// memory -> registers, no LDS store + load before the first butterflies)
+// and/or whose last pass hands its outputs to a storer (no LDS store + load
+// after the last butterflies).  Ld: double2 operator()(int item, int idx);
+// St: void operator()(int item, int idx, double2 v); NoIO disables either.
+struct NoIO {};
+
+template <int FS, int R, bool INV, int NT, class TL, class Ld, class St>
+__device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const double2 *tw, const Ld &ld, const St &st) {
+    constexpr bool LOAD = !std::is_same<Ld, NoIO>::value;
+    constexpr bool STORE = !std::is_same<St, NoIO>::value;
+    constexpr int NBI = FS / R;
+    constexpr int LOG_NBI = ilog2c(NBI);
+    constexpr int PER = (kTile / NT) / R;
+    const int t = threadIdx.x;
+    const int Ns = 1 << logNs;
+    double2 v[PER][R];
+    int item[PER], jj[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int bj = t + u * NT;
+        if constexpr (TL::kItemFast) {
+            constexpr int LOG_ITEMS = ilog2c(kTile / FS);
+            item[u] = bj & ((kTile / FS) - 1);
+            jj[u] = bj >> LOG_ITEMS;
+        } else {
+            item[u] = bj >> LOG_NBI;
+            jj[u] = bj & (NBI - 1);
+        }
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if constexpr (LOAD) v[u][s] = ld(item[u], jj[u] + s * NBI);
+            else v[u][s] = lds[TL::at(item[u], jj[u] + s * NBI)];
+        }
+    }
+    // loads: the caller's earlier LDS reads must finish before this pass
+    // writes; LDS reads: every read of this pass before anyone writes
+    __syncthreads();
+    const int twSh = 12 - (logNs + ilog2c(R));
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int k = jj[u] & (Ns - 1);
+        if (logNs > 0) twiddle<R, INV>(v[u], tw, k, twSh);
+        dft<R, INV>(v[u]);
+        const int d0 = (jj[u] - k) * R + k;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if constexpr (STORE) st(item[u], d0 + dft_out<R>(j) * Ns, v[u][j]);
+            else lds[TL::at(item[u], d0 + dft_out<R>(j) * Ns)] = v[u][j];
+        }
+    }
+    if constexpr (!STORE) __syncthreads();
+}
+
+// fft_tile with the first pass fed by `ld` and / or the last pass emptied
+// into `st` (NoIO: through LDS as fft_tile).  With a storer the tile is free
+// for writing when the call returns (the last pass's read barrier).
+template <int FS, bool INV, int NT, class TL, class Ld, class St>
+__device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, const Ld &ld, const St &st) {
+    constexpr int RMAX = kTile / NT;
+    constexpr int LR = ilog2c(RMAX);
+    constexpr int LOG = ilog2c(FS);
+    constexpr int REM = LOG % LR;
+    constexpr int NP = (REM ? 1 : 0) + LOG / LR;   // passes
+    static_assert(NP >= 2, "first and last pass must differ");
+    if constexpr (std::is_same<Ld, NoIO>::value) __syncthreads();
+    int logNs = 0;
+    if constexpr (REM == 1) { fft_pass_io<FS, 2, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass_io<FS, 4, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass_io<FS, 8, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 3; }
+#pragma unroll
+    for (int p = 0; p < LOG / LR; ++p) {
+        const bool first = (REM == 0 && p == 0), last = (p == LOG / LR - 1);
+        if (first && last) {
+            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, ld, st);
+        } else if (first) {
+            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, ld, NoIO{});
+        } else if (last) {
+            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, NoIO{}, st);
+        } else {
+            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, NoIO{}, NoIO{});
+        }
+        logNs += LR;
+    }
 }
 
 // In-place transform of every size-FS transform of the tile (TileLayout<FS>) by NT

@@ -751,22 +751,20 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
                                                int twShift) {
     using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
-    constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
+    (void)twShift;
 
     double2 P1[kLP / 2];
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
-#pragma unroll
-        for (int u = 0; u < kLP; ++u) {
-            const int e = t + u * kLT;
-            const int item = e >> LOGF, m = e & (F - 1);
+        // first pass straight from the children (global -> registers)
+        auto ld = [&](int item, int m) {
             int qi, pr;
             M.at(item, qi, pr);
-            lds[TL::at(item, m)] = load_child_pair(J, qi, 2 * pr + which, m);
-        }
-        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
+            return load_child_pair(J, qi, 2 * pr + which, m);
+        };
+        fft_tile_io<F, false, kLT, TL>(lds, tw, ld, NoIO{});
 #pragma unroll
         for (int u = 0; u < kLP / 2; ++u) {
             const int ps = t + u * kLT;
@@ -786,21 +784,16 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
                 lds[ib] = pack_ai(c_conj(P1[u]), c_conj(Pv));
             }
         }
-        if (which == 0) __syncthreads();
     }
-    fft_tile<F, true, kLT, TL>(lds, tw, twShift);
+    // last pass straight to the parents (exact coefficient 0 and top term)
     const double invF = 1.0 / (double)F;
     const int ps_ = J.parent.stride, pc = J.parent.count;
-#pragma unroll
-    for (int u = 0; u < kLP; ++u) {
-        const int e = t + u * kLT;
-        const int item = e >> LOGF, m = e & (F - 1);
+    auto st = [&](int item, int m, double2 v) {
         int qi, pr;
         M.at(item, qi, pr);
         const int p = 2 * pr;
         double *pb = J.parent.base + (long long)qi * J.parent.plane;
         if (m < pc) {
-            const double2 v = lds[TL::at(item, m)];
             if (p < J.nr_parent)
                 put(&pb[(long long)p * ps_ + m], (m == 0) ? child_prod_at(J, qi, p, 0) : v.x * invF, J.acc_out);
             if (p + 1 < J.nr_parent)
@@ -813,7 +806,8 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
             if (p + 1 < J.nr_parent)
                 put(&pb[(long long)(p + 1) * ps_ + F], child_prod_at(J, qi, p + 1, top), J.acc_out);
         }
-    }
+    };
+    fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
 }
 
 template <int F>
@@ -830,7 +824,11 @@ __global__ __launch_bounds__(512, 4) void k_build_lds(LevelJob J, const double2 
 // each parent's children share another; the two outputs of a parent share
 // the inverse (Re = gL, Im = gR).
 // ---------------------------------------------------------------------------
-template <int F, class Map>
+// IO = true: the g and child transforms take their first pass straight from
+// global memory and the inverses store straight to the children's g (two LDS
+// round trips fewer per transform); at 128 VGPRs (two workgroups per CU) the
+// extra live registers spill, so the plain kernel keeps the LDS path.
+template <int F, bool IO, class Map>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift) {
     using TL = SwzLayout<F>;
@@ -840,10 +838,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
     const int t = threadIdx.x;
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
 
-#pragma unroll
-    for (int u = 0; u < kLP; ++u) {
-        const int e = t + u * kLT;
-        const int item = e >> LOGF, m = e & (F - 1);
+    auto ldg = [&](int item, int m) {
         int qi, pr;
         M.at(item, qi, pr);
         const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
@@ -853,9 +848,18 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             if (pa >= 0) z.x = gb[(long long)pa * gs + m];
             if (pb >= 0) z.y = gb[(long long)pb * gs + m];
         }
-        lds[TL::at(item, m)] = z;
+        return z;
+    };
+    if constexpr (IO) {
+        fft_tile_io<F, false, kLT, TL>(lds, tw, ldg, NoIO{});
+    } else {
+#pragma unroll
+        for (int u = 0; u < kLP; ++u) {
+            const int e = t + u * kLT;
+            lds[TL::at(e >> LOGF, e & (F - 1))] = ldg(e >> LOGF, e & (F - 1));
+        }
+        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     }
-    fft_tile<F, false, kLT, TL>(lds, tw, twShift);
     double2 G1[kLP / 2], G2[kLP / 2];
 #pragma unroll
     for (int u = 0; u < kLP / 2; ++u) {
@@ -870,20 +874,26 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             G2[u] = split_y(za, zb);
         }
     }
-    __syncthreads();
+    if constexpr (!IO) __syncthreads();
     const double invF = 1.0 / (double)F;
     const int gcs = J.gchild.stride, gcc = J.gchild.count;
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
-#pragma unroll
-        for (int u = 0; u < kLP; ++u) {
-            const int e = t + u * kLT;
-            const int item = e >> LOGF, m = e & (F - 1);
+        auto ldc = [&](int item, int m) {
             int qi, pr;
             M.at(item, qi, pr);
-            lds[TL::at(item, m)] = load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
+            return load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
+        };
+        if constexpr (IO) {
+            fft_tile_io<F, false, kLT, TL>(lds, tw, ldc, NoIO{});
+        } else {
+#pragma unroll
+            for (int u = 0; u < kLP; ++u) {
+                const int e = t + u * kLT;
+                lds[TL::at(e >> LOGF, e & (F - 1))] = ldc(e >> LOGF, e & (F - 1));
+            }
+            fft_tile<F, false, kLT, TL>(lds, tw, twShift);
         }
-        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
 #pragma unroll
         for (int u = 0; u < kLP / 2; ++u) {
             const int ps = t + u * kLT;
@@ -901,17 +911,12 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
                 lds[ib] = pack_ai(c_conj(A), c_conj(Bv));
             }
         }
-        fft_tile<F, true, kLT, TL>(lds, tw, twShift);
-#pragma unroll
-        for (int u = 0; u < kLP; ++u) {
-            const int e = t + u * kLT;
-            const int item = e >> LOGF, m = e & (F - 1);
+        auto st = [&](int item, int m, double2 v) {
             int qi, pr;
             M.at(item, qi, pr);
             const int p = slot_parent(J, 2 * pr + which);
             if (p >= 0 && m < gcc) {
                 double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
-                const double2 v = lds[TL::at(item, m)];
                 double vl = v.x * invF, vr = v.y * invF;
                 if (J.gfix && m == J.child.count - 1) {
                     vl += prop_wrap_term(J, qi, p, 0);
@@ -920,15 +925,25 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
                 put(&gcb[(long long)(2 * p) * gcs + m], vl, J.acc_out);
                 if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * gcs + m], vr, J.acc_out);
             }
+        };
+        if constexpr (IO) {
+            fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
+        } else {
+            fft_tile<F, true, kLT, TL>(lds, tw, twShift);
+#pragma unroll
+            for (int u = 0; u < kLP; ++u) {
+                const int e = t + u * kLT;
+                st(e >> LOGF, e & (F - 1), lds[TL::at(e >> LOGF, e & (F - 1))]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
 template <int F>
 __global__ __launch_bounds__(512, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[SwzLayout<F>::SIZE];
-    prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+    prop_lds_body<F, false>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -1135,7 +1150,7 @@ __global__ __launch_bounds__(512) void k_prop_lds_batch(const LevelJob *__restri
     const LevelJob J = jobs[it.x];
     const int npair = (J.np + 1) >> 1;
     const FlatMap M{it.y, npair, J.nq * npair};
-#define ICM_PROP_F(FF) prop_lds_body<FF>(J, lds, M, tw, twShift)
+#define ICM_PROP_F(FF) prop_lds_body<FF, true>(J, lds, M, tw, twShift)
     ICM_F_CASES(ICM_PROP_F)
 #undef ICM_PROP_F
 }
