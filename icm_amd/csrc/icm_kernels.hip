@@ -1614,41 +1614,36 @@ __device__ __forceinline__ int mirror_idx(int k1, int k1m, int k2) {
     return (k1 == k1m) ? row_at(mirror_k2(k1, k2)) : RTL::IS + row_at(kF2 - 1 - k2);
 }
 
-// Load rows k1 / k1m of one [k1][n2] array into the tiles and transform them.
+// Load rows k1 / k1m of one [k1][n2] array into the tiles and transform them;
+// the first radix-8 pass reads its points (n2 = t + 512 s, coalesced) straight
+// from global memory.
 __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__restrict__ Zb, int k1, int k1m,
                                                  const double2 *tw, int twShift) {
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = Zb[(long long)k1 * kF2 + t + u * kRT];
+    (void)twShift;
+    const double2 *r0 = Zb + (long long)k1 * kF2;
+    fft_tile_io<kF2, false, kRT, RTL>(T, tw, [&](int, int n2) { return r0[n2]; }, NoIO{});
     if (k1 != k1m) {
-#pragma unroll
-        for (int u = 0; u < kRP; ++u) T[RTL::IS + row_at(t + u * kRT)] = Zb[(long long)k1m * kF2 + t + u * kRT];
+        const double2 *r1 = Zb + (long long)k1m * kF2;
+        fft_tile_io<kF2, false, kRT, RTL>(T + RTL::IS, tw, [&](int, int n2) { return r1[n2]; }, NoIO{});
     }
-    fft_tile<kF2, false, kRT, RTL>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, false, kRT, RTL>(T + RTL::IS, tw, twShift);
 }
 
-// Inverse-transform the tiles in place and store rows k1 / k1m of V with the
-// inverse twiddle w_F^(-n2 row).
+// Inverse-transform the tiles and store rows k1 / k1m of V with the inverse
+// twiddle w_F^(-n2 row); the last pass writes straight to global memory, and
+// the tiles are free for writing on return.
 __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m, int logF,
                                                    const double2 *tw, int twShift) {
-    const int t = threadIdx.x;
     const int F = 1 << logF;
-    fft_tile<kF2, true, kRT, RTL>(T, tw, twShift);
-    if (k1 != k1m) fft_tile<kF2, true, kRT, RTL>(T + RTL::IS, tw, twShift);
-#pragma unroll 2
-    for (int u = 0; u < kRP; ++u) {
-        const int e = t + u * kRT;
-        Vb[(long long)k1 * kF2 + e] = c_mulc(T[row_at(e)], tw_big(tw, twShift, (e * k1) & (F - 1), logF));
-    }
+    double2 *v0 = Vb + (long long)k1 * kF2;
+    fft_tile_io<kF2, true, kRT, RTL>(T, tw, NoIO{}, [&](int, int e, double2 v) {
+        v0[e] = c_mulc(v, tw_big(tw, twShift, (e * k1) & (F - 1), logF));
+    });
     if (k1 != k1m) {
-#pragma unroll 2
-        for (int u = 0; u < kRP; ++u) {
-            const int e = t + u * kRT;
-            Vb[(long long)k1m * kF2 + e] = c_mulc(T[RTL::IS + row_at(e)], tw_big(tw, twShift, (e * k1m) & (F - 1), logF));
-        }
+        double2 *v1 = Vb + (long long)k1m * kF2;
+        fft_tile_io<kF2, true, kRT, RTL>(T + RTL::IS, tw, NoIO{}, [&](int, int e, double2 v) {
+            v1[e] = c_mulc(v, tw_big(tw, twShift, (e * k1m) & (F - 1), logF));
+        });
     }
-    __syncthreads();
 }
 
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
@@ -1711,13 +1706,10 @@ __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict_
         G1[u] = split_x(a, b);
         G2[u] = split_y(a, b);
     }
-#pragma unroll 1
-    for (int which = 0; which < 2; ++which) {
-        const int p = 2 * pair + which;
-        if (p >= nrp) break;  // uniform over the workgroup
+    // one parent: U1 = g conj(R) + i g conj(L) on its child pair's rows
+    auto parent = [&](int p, const double2 (&Gw)[kRP]) {
         __syncthreads();
         rowpair_load_fft(T, Y1 + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
-        const double2 *Gw = which ? G2 : G1;
         if (k1 != k1m) {
             // the thread owns both positions it reads: update in place
 #pragma unroll
@@ -1743,7 +1735,9 @@ __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict_
             for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = U[u];
         }
         rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, logF, tw, twShift);
-    }
+    };
+    parent(2 * pair, G1);
+    if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
 }
 
 // ---------------------------------------------------------------------------
