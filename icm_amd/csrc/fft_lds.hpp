@@ -41,6 +41,18 @@ __device__ __forceinline__ double2 tw_big(const double2 *tw, int twShift, int m,
     return make_double2(c.x * f.x - c.y * f.y, c.x * f.y + c.y * f.x);
 }
 
+// tw_big for transforms larger than the dense table (logF > 12, so the
+// table always has its fine part): no branch, so the two loads of every
+// output issue together.  It costs registers where many outputs are in
+// flight, so callers choose it by measurement.
+__device__ __forceinline__ double2 tw_big2(const double2 *tw, int twShift, int m, int logF) {
+    const int M = m << (twShift - logF);
+    const int lf = twShift - 12;
+    const double2 c = tw[M >> lf];
+    const double2 f = tw[kTile + (M & ((1 << lf) - 1))];
+    return make_double2(c.x * f.x - c.y * f.y, c.x * f.y + c.y * f.x);
+}
+
 template <int FS> struct TileLayout {
     static constexpr bool kItemFast = false;        // butterflies of one transform on consecutive lanes
     static constexpr int IS = FS + FS / 16 + 1;     // item stride (odd)
@@ -264,13 +276,14 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
 // St: void operator()(int item, int idx, double2 v); NoIO disables either.
 struct NoIO {};
 
-template <int FS, int R, bool INV, int NT, class TL, class Ld, class St>
+template <int FS, int R, bool INV, int NT, class TL, int PTS = kTile, class Ld, class St>
 __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const double2 *tw, const Ld &ld, const St &st) {
     constexpr bool LOAD = !std::is_same<Ld, NoIO>::value;
     constexpr bool STORE = !std::is_same<St, NoIO>::value;
     constexpr int NBI = FS / R;
     constexpr int LOG_NBI = ilog2c(NBI);
-    constexpr int PER = (kTile / NT) / R;
+    constexpr int PER = (PTS / NT) / R;
+    static_assert(PTS == kTile || !TL::kItemFast, "item-fast layouts hold one tile");
     const int t = threadIdx.x;
     const int Ns = 1 << logNs;
     double2 v[PER][R];
@@ -314,7 +327,10 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
 // fft_tile with the first pass fed by `ld` and / or the last pass emptied
 // into `st` (NoIO: through LDS as fft_tile).  With a storer the tile is free
 // for writing when the call returns (the last pass's read barrier).
-template <int FS, bool INV, int NT, class TL, class Ld, class St>
+// PTS = points transformed per call: kTile (one tile), or a multiple of it,
+// in which case every thread runs PTS / kTile butterflies per pass at the
+// same index of consecutive tiles (the radix stays kTile / NT).
+template <int FS, bool INV, int NT, class TL, int PTS = kTile, class Ld, class St>
 __device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, const Ld &ld, const St &st) {
     constexpr int RMAX = kTile / NT;
     constexpr int LR = ilog2c(RMAX);
@@ -324,20 +340,20 @@ __device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, con
     static_assert(NP >= 2, "first and last pass must differ");
     if constexpr (std::is_same<Ld, NoIO>::value) __syncthreads();
     int logNs = 0;
-    if constexpr (REM == 1) { fft_pass_io<FS, 2, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 1; }
-    if constexpr (REM == 2) { fft_pass_io<FS, 4, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 2; }
-    if constexpr (REM == 3) { fft_pass_io<FS, 8, INV, NT, TL>(lds, logNs, tw, ld, NoIO{}); logNs += 3; }
+    if constexpr (REM == 1) { fft_pass_io<FS, 2, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass_io<FS, 4, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass_io<FS, 8, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 3; }
 #pragma unroll
     for (int p = 0; p < LOG / LR; ++p) {
         const bool first = (REM == 0 && p == 0), last = (p == LOG / LR - 1);
         if (first && last) {
-            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, ld, st);
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, ld, st);
         } else if (first) {
-            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, ld, NoIO{});
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{});
         } else if (last) {
-            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, NoIO{}, st);
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, NoIO{}, st);
         } else {
-            fft_pass_io<FS, RMAX, INV, NT, TL>(lds, logNs, tw, NoIO{}, NoIO{});
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, NoIO{}, NoIO{});
         }
         logNs += LR;
     }

@@ -1589,12 +1589,18 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
     }
 }
 
-// Fused spectrum stage of a large level: one workgroup owns the row pair
-// {k1, F1 - k1} (rows 0 and F1/2 pair with themselves).  Every bin -K of row
-// k1 lies in row F1 - k1 (position F2-1-k2, or (F2-k2) mod F2 in row 0), so
-// the forward row FFTs, the real-pair split, the pointwise product and the
-// inverse row FFTs of both rows happen without leaving the workgroup.
-// Thread t owns positions k2 = t + 256u of row k1 and their mirrors.
+// Fused spectrum stage of a large level: a workgroup owns a row k1 and its
+// mirror row F1 - k1.  Every bin -K of row k1 lies in row F1 - k1 (position
+// F2-1-k2; rows 0 and F1/2 are their own mirrors, at (F2-k2) mod F2 and
+// F2-1-k2), so the forward row FFTs, the real-pair split, the pointwise
+// product and the inverse row FFTs happen without leaving the workgroup.
+// Both kernels keep two 4096-point rows in two LDS tiles (128 KiB).
+//
+// Propagate (k_rowpair_prop): workgroup k1 = 0 .. F1/2 owns {k1, F1 - k1};
+// the tiles are transformed one after the other, and thread t owns positions
+// k2 = t + 512u of row k1 and their mirrors.  It holds the g spectra of both
+// parents in registers across the child transforms, which leaves no room for
+// transforming both tiles at once.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int mirror_k2(int k1, int k2) {
     return (k1 == 0) ? ((kF2 - k2) & (kF2 - 1)) : (kF2 - 1 - k2);
@@ -1646,46 +1652,133 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
     }
 }
 
+// Build (k_rowpair_build): both tiles are transformed together.
+//
+// Workgroup w of a level owns two rows, one 4096-point LDS tile each:
+// w >= 1 the pair {w, F1 - w}; w = 0 the two self-mirrored rows {0, F1/2}.
+// 512 threads; every thread runs one radix-8 butterfly of each tile per pass
+// at the same index (one barrier per pass for both rows, shared twiddles).
+// For the pointwise stages each thread owns 8 representatives
+// r = t + 512u, each standing for a bin K and its mirror -K
+// (the spectra of real sequences satisfy Z[-K] = conj Z[K], so one value per
+// representative is kept); the representatives cover the 8192 positions of
+// the two tiles exactly once:
+//   w >= 1:           K at tile 0 position r,   -K at tile 1 position F2-1-r;
+//   w = 0, r < 2048:  K at tile 0 position r,   -K at tile 0 position F2-r,
+//                     except r = 0, which holds the two self-mirrored bins
+//                     0 and 2048 of row 0 (their spectra are real, so one
+//                     double2 carries both: .x for bin 0, .y for bin 2048);
+//   w = 0, r >= 2048: K at tile 1 position r-2048, -K at tile 1 F2-1-(r-2048).
+// ---------------------------------------------------------------------------
+constexpr int kBT = 512;
+constexpr int kBU = kTile / kBT;       // representatives per thread
+
+struct RowRep {
+    int ia, ib;      // LDS positions of K and -K (r = 0 of w = 0: bins 0 and 2048)
+    bool fixed;      // the two self-mirrored bins of row 0
+};
+__device__ __forceinline__ RowRep row_rep(int w, int r) {
+    RowRep q;
+    q.fixed = false;
+    if (w != 0) {
+        q.ia = row_at(r);
+        q.ib = RTL::IS + row_at(kF2 - 1 - r);
+    } else if (r < kF2 / 2) {
+        q.ia = row_at(r);
+        q.ib = row_at(r == 0 ? kF2 / 2 : kF2 - r);
+        q.fixed = (r == 0);
+    } else {
+        const int k2 = r - kF2 / 2;
+        q.ia = RTL::IS + row_at(k2);
+        q.ib = RTL::IS + row_at(kF2 - 1 - k2);
+    }
+    return q;
+}
+// Spectra X (real part's) and Y (imaginary part's) of the packed transform at
+// a representative; for the fixed pair, (bin 0, bin 2048) of each, both real.
+__device__ __forceinline__ void row_split(const RowRep &q, double2 a, double2 b, double2 &X, double2 &Y) {
+    if (q.fixed) {
+        X = make_double2(a.x, b.x);
+        Y = make_double2(a.y, b.y);
+    } else {
+        X = split_x(a, b);
+        Y = split_y(a, b);
+    }
+}
+// Write A + iB at K and conj(A) + i conj(B) at -K (fixed pair: A, B real at
+// both bins).
+__device__ __forceinline__ void row_put(double2 *T, const RowRep &q, double2 A, double2 Bv) {
+    if (q.fixed) {
+        T[q.ia] = make_double2(A.x, Bv.x);
+        T[q.ib] = make_double2(A.y, Bv.y);
+    } else {
+        T[q.ia] = pack_ai(A, Bv);
+        T[q.ib] = pack_ai(c_conj(A), c_conj(Bv));
+    }
+}
+
+// Rows of workgroup w: tile 0 and tile 1.
+__device__ __forceinline__ int row_of(int w, int F1, int h) {
+    return (w == 0) ? (h ? F1 / 2 : 0) : (h ? F1 - w : w);
+}
+
+// Load both rows of one [k1][n2] array and transform them; the first radix-8
+// pass reads its points (n2 = t + 512 s, coalesced) straight from global
+// memory.  Ends with the last pass's barrier.
+__device__ __forceinline__ void rowpair2_load_fft(double2 *T, const double2 *__restrict__ Zb, int rA, int rB,
+                                                 const double2 *tw) {
+    const double2 *z0 = Zb + (long long)rA * kF2, *z1 = Zb + (long long)rB * kF2;
+    fft_tile_io<kF2, false, kBT, RTL, 2 * kTile>(T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
+}
+
+// Inverse-transform both tiles and store their rows of V with the inverse
+// twiddle w_F^(-n2 row); the last pass writes straight to global memory, and
+// the tiles are free for writing on return.
+__device__ __forceinline__ void rowpair2_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB, int logF,
+                                                   const double2 *tw, int twShift) {
+    const int F = 1 << logF;
+    fft_tile_io<kF2, true, kBT, RTL, 2 * kTile>(T, tw, NoIO{}, [&](int h, int e, double2 v) {
+        const int row = h ? rB : rA;
+        Vb[(long long)row * kF2 + e] = c_mulc(v, tw_big2(tw, twShift, (e * row) & (F - 1), logF));
+    });
+}
+
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
-__global__ __launch_bounds__(512) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
+__global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
                                                        int nrp, int npairs, int logF,
                                                        const double2 *tw, int twShift) {
     __shared__ double2 T[2 * RTL::IS];
     const int F = 1 << logF, F1 = F >> 12;
-    const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
+    const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
-    double2 P1[kRP];
     const int p = 2 * pair;
-    rowpair_load_fft(T, Y + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
+    double2 P1[kBU], P2[kBU];
+    rowpair2_load_fft(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
-    for (int u = 0; u < kRP; ++u) {
-        const int e = t + u * kRT;
-        const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
-        P1[u] = c_mul(split_x(a, b), split_y(a, b));
+    for (int u = 0; u < kBU; ++u) {
+        const RowRep q = row_rep(w, t + u * kBT);
+        double2 X, Yv;
+        row_split(q, T[q.ia], T[q.ib], X, Yv);
+        P1[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
     }
-    double2 P2[kRP];
     if (p + 1 < nrp) {  // uniform over the workgroup
-        __syncthreads();
-        rowpair_load_fft(T, Y + ((long long)qi * nrp + p + 1) * F, k1, k1m, tw, twShift);
+        rowpair2_load_fft(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
 #pragma unroll
-        for (int u = 0; u < kRP; ++u) {
-            const int e = t + u * kRT;
-            const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
-            P2[u] = c_mul(split_x(a, b), split_y(a, b));
+        for (int u = 0; u < kBU; ++u) {
+            const RowRep q = row_rep(w, t + u * kBT);
+            double2 X, Yv;
+            row_split(q, T[q.ia], T[q.ib], X, Yv);
+            P2[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < kRP; ++u) P2[u] = make_double2(0.0, 0.0);
+        for (int u = 0; u < kBU; ++u) P2[u] = make_double2(0.0, 0.0);
     }
-    __syncthreads();   // self-paired rows read mirrors written below
+    // every thread rewrites only the positions it read
 #pragma unroll
-    for (int u = 0; u < kRP; ++u) {
-        const int e = t + u * kRT;
-        T[row_at(e)] = pack_ai(P1[u], P2[u]);
-        if (k1 != k1m) T[mirror_idx(k1, k1m, e)] = pack_ai(c_conj(P1[u]), c_conj(P2[u]));
-    }
-    rowpair_ifft_store(T, V + ((long long)qi * npairs + pair) * F, k1, k1m, logF, tw, twShift);
+    for (int u = 0; u < kBU; ++u) row_put(T, row_rep(w, t + u * kBT), P1[u], P2[u]);
+    rowpair2_ifft_store(T, V + ((long long)qi * npairs + pair) * F, rA, rB, logF, tw, twShift);
 }
 
 // Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
@@ -1995,7 +2088,7 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(bufA, bufB, nrp, npairs, J.logF, tw, twShift);
+    k_rowpair_build<<<dim3(F1 / 2, npairs, J.nq), kBT, 0, s>>>(bufA, bufB, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
     PF_BEGIN();
