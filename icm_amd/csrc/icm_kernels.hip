@@ -1464,8 +1464,9 @@ __global__ __launch_bounds__(512) void k_prop_lds_t_batch(const LevelJob *__rest
 // Large levels (F >= 8192): four-step transform with F = F1 x F2, F2 = 4096.
 //   forward: column FFTs (size F1, stride F2) + twiddle w_F^(n2 k1) -> [k1][n2]
 //            row FFTs (size F2) -> spectrum in [k1][k2] order (K = k1 + F1 k2)
-//   inverse: row IFFTs of [k1][k2] + twiddle w_F^(-n2 k1) -> [k1][n2]
-//            column IFFTs -> natural order.
+//   inverse: row IFFTs of [k1][k2] -> [k1][n2]
+//            twiddle w_F^(-n2 k1) (on the column kernel's load) + column
+//            IFFTs -> natural order.
 // Spectra stay in [k1][k2] order; the pointwise kernels find -K there.
 // ---------------------------------------------------------------------------
 constexpr int kF2 = kTile;
@@ -1578,7 +1579,9 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
     for (int u = 0; u < kCP; ++u) {
         const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
-        lds[TL::at(c, r)] = src[r * kF2 + tile * TC + c];
+        const int n2 = tile * TC + c;
+        // the inverse four-step twiddle w_F^(-n2 r) of the row stage
+        lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big(tw, twShift, (n2 * r) & (F - 1), logF));
     }
     fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
 #pragma unroll
@@ -1634,20 +1637,20 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
     }
 }
 
-// Inverse-transform the tiles and store rows k1 / k1m of V with the inverse
-// twiddle w_F^(-n2 row); the last pass writes straight to global memory, and
-// the tiles are free for writing on return.
-__device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m, int logF,
-                                                   const double2 *tw, int twShift) {
-    const int F = 1 << logF;
+// Inverse-transform the tiles and store rows k1 / k1m of V (the inverse
+// four-step twiddle is applied by the column inverse as it loads them); the
+// last pass writes straight to global memory, and the tiles are free for
+// writing on return.
+__device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m,
+                                                   const double2 *tw) {
     double2 *v0 = Vb + (long long)k1 * kF2;
     fft_tile_io<kF2, true, kRT, RTL>(T, tw, NoIO{}, [&](int, int e, double2 v) {
-        v0[e] = c_mulc(v, tw_big(tw, twShift, (e * k1) & (F - 1), logF));
+        v0[e] = v;
     });
     if (k1 != k1m) {
         double2 *v1 = Vb + (long long)k1m * kF2;
         fft_tile_io<kF2, true, kRT, RTL>(T + RTL::IS, tw, NoIO{}, [&](int, int e, double2 v) {
-            v1[e] = c_mulc(v, tw_big(tw, twShift, (e * k1m) & (F - 1), logF));
+            v1[e] = v;
         });
     }
 }
@@ -1731,15 +1734,13 @@ __device__ __forceinline__ void rowpair2_load_fft(double2 *T, const double2 *__r
     fft_tile_io<kF2, false, kBT, RTL, 2 * kTile>(T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
 }
 
-// Inverse-transform both tiles and store their rows of V with the inverse
-// twiddle w_F^(-n2 row); the last pass writes straight to global memory, and
-// the tiles are free for writing on return.
-__device__ __forceinline__ void rowpair2_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB, int logF,
-                                                   const double2 *tw, int twShift) {
-    const int F = 1 << logF;
+// Inverse-transform both tiles and store their rows of V (twiddle as above);
+// the last pass writes straight to global memory, and the tiles are free for
+// writing on return.
+__device__ __forceinline__ void rowpair2_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB,
+                                                    const double2 *tw) {
     fft_tile_io<kF2, true, kBT, RTL, 2 * kTile>(T, tw, NoIO{}, [&](int h, int e, double2 v) {
-        const int row = h ? rB : rA;
-        Vb[(long long)row * kF2 + e] = c_mulc(v, tw_big2(tw, twShift, (e * row) & (F - 1), logF));
+        Vb[(long long)(h ? rB : rA) * kF2 + e] = v;
     });
 }
 
@@ -1778,7 +1779,7 @@ __global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict
     // every thread rewrites only the positions it read
 #pragma unroll
     for (int u = 0; u < kBU; ++u) row_put(T, row_rep(w, t + u * kBT), P1[u], P2[u]);
-    rowpair2_ifft_store(T, V + ((long long)qi * npairs + pair) * F, rA, rB, logF, tw, twShift);
+    rowpair2_ifft_store(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 }
 
 // Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
@@ -1827,7 +1828,7 @@ __global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict_
 #pragma unroll
             for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = U[u];
         }
-        rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, logF, tw, twShift);
+        rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, tw);
     };
     parent(2 * pair, G1);
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
