@@ -1581,7 +1581,7 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n2 = tile * TC + c;
         // the inverse four-step twiddle w_F^(-n2 r) of the row stage
-        lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big(tw, twShift, (n2 * r) & (F - 1), logF));
+        lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big2(tw, twShift, (n2 * r) & (F - 1), logF));
     }
     fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
 #pragma unroll
