@@ -104,6 +104,8 @@ _SIGS = [
     ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_set_schoolbook_max", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_schoolbook_max", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_get_spectrum_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
@@ -318,6 +320,16 @@ class Plan:
     @property
     def schoolbook_max(self):
         return self._lib.icm_hip_plan_get_schoolbook_max(self._h)
+
+    def set_spectrum_cache(self, on):
+        """Keep (True) or recompute (False) the build's column transforms of
+        the four-step levels' children for the propagate sweep."""
+        if self._lib.icm_hip_plan_set_spectrum_cache(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_spectrum_cache")
+
+    @property
+    def spectrum_cache(self):
+        return bool(self._lib.icm_hip_plan_get_spectrum_cache(self._h) == 1)
 
     def summary(self):
         sm = PlanSummary()

@@ -55,6 +55,13 @@ struct IcmHipPlan {
     long long g_elems = 0;               // per q-plane
     double2 *d_bufA = nullptr, *d_bufB = nullptr;
     long long big_elems = 0;             // complex per q-plane per buffer
+    // spectrum cache: column transforms of the child pairs of every four-step
+    // build level, kept from the build for the propagate of the same q-batch
+    // (level l at d_spec + spec_off[l], [qb][n_real[l]][F[l]] complex)
+    double2 *d_spec = nullptr;
+    std::vector<long long> spec_off;
+    long long spec_elems = 0;            // complex per q-plane
+    size_t spec_bytes = 0;
     double *d_partial = nullptr;         // [nchunk][n]
     double *d_acc = nullptr;             // [n] sorted order
     double *d_equity = nullptr;          // [n] original order
@@ -100,7 +107,7 @@ void plan_free(IcmHipPlan *p) {
     auto f = [](void *x) { if (x) (void)hipFree(x); };
     f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_blocks);
     for (double *c : p->d_coef) f(c);
-    f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_partial); f(p->d_acc);
+    f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_spec); f(p->d_partial); f(p->d_acc);
     f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
     f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
     f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
@@ -127,6 +134,41 @@ long long plane_bytes(const Geometry &g, long long *g_elems, long long *big_elem
     *g_elems = ge;
     *big_elems = be;
     return bytes;
+}
+
+// Complex elements per q-plane of the spectrum cache (four-step build levels).
+long long spec_plane_elems(const Geometry &g) {
+    long long e = 0;
+    for (int l = 1; l <= g.L - 2; ++l)
+        if (g.F[l] > icm::kMaxLdsF) e += (long long)g.n_real[l] * g.F[l];
+    return e;
+}
+
+bool spec_alloc(IcmHipPlan *p) {
+    const Geometry &g = p->geo;
+    p->spec_off.assign(g.L, -1);
+    long long off = 0;
+    for (int l = 1; l <= g.L - 2; ++l) {
+        if (g.F[l] > icm::kMaxLdsF) {
+            p->spec_off[l] = off;
+            off += (long long)p->qb * g.n_real[l] * g.F[l];
+        }
+    }
+    if (off == 0) return true;
+    if (!dalloc(p, &p->d_spec, (size_t)off)) {
+        (void)hipGetLastError();   // leave no allocation error for later launches to report
+        return false;
+    }
+    p->spec_bytes = sizeof(double2) * (size_t)off;
+    return true;
+}
+
+void spec_release(IcmHipPlan *p) {
+    if (!p->d_spec) return;
+    (void)hipFree(p->d_spec);
+    p->d_spec = nullptr;
+    p->bytes -= p->spec_bytes;
+    p->spec_bytes = 0;
 }
 
 IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts, bool sort_players,
@@ -162,15 +204,36 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { plan_free(p); return nullptr; }
     long long budget = (long long)(0.6 * (double)freeb);
-    int qb = (int)std::max<long long>(1, budget / std::max<long long>(pb, 1));
-    qb = std::min(qb, std::max(1, std::min(max_q, 256)));
-    if (forcedQ > 0) qb = std::min(qb, forcedQ);
-    // launch-shape limits: row passes put (q-plane x array) on grid.y
     for (int l = 1; l < g.L; ++l) {
-        if (g.F[l] > icm::kMaxLdsF) {
-            const long long arrays = (long long)g.n_real[l] + (g.n_real[l] + 1) / 2;
-            while (qb > 1 && arrays * qb > 65535) --qb;
-            if (arrays > 65535) { set_err("level too wide for the row-pass grid"); delete p; return nullptr; }
+        if (g.F[l] > icm::kMaxLdsF && (long long)g.n_real[l] + (g.n_real[l] + 1) / 2 > 65535) {
+            set_err("level too wide for the row-pass grid");
+            delete p;
+            return nullptr;
+        }
+    }
+    auto planes = [&](long long per_plane) {
+        int q = (int)std::max<long long>(1, budget / std::max<long long>(per_plane, 1));
+        q = std::min(q, std::max(1, std::min(max_q, 256)));
+        if (forcedQ > 0) q = std::min(q, forcedQ);
+        // launch-shape limits: row passes put (q-plane x array) on grid.y
+        for (int l = 1; l < g.L; ++l) {
+            if (g.F[l] > icm::kMaxLdsF) {
+                const long long arrays = (long long)g.n_real[l] + (g.n_real[l] + 1) / 2;
+                while (q > 1 && arrays * q > 65535) --q;
+            }
+        }
+        return q;
+    };
+    int qb = planes(pb);
+    // The spectrum cache costs HBM per q-plane; take it while at least
+    // min(qb, 32) q-planes still fit in one batch.
+    p->spec_elems = spec_plane_elems(g);
+    bool spec = false;
+    if (p->spec_elems > 0) {
+        const int qs = planes(pb + 16 * p->spec_elems);
+        if (qs >= std::min(qb, 32)) {
+            qb = qs;
+            spec = true;
         }
     }
     p->qb = qb;
@@ -197,6 +260,7 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     if (ok && be > 0) ok = dalloc(p, &p->d_bufA, (size_t)qb * be) && dalloc(p, &p->d_bufB, (size_t)qb * be);
     p->d_coef.assign(g.L, nullptr);
     for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
+    if (ok && spec) ok = spec_alloc(p);
     if (!ok) { plan_free(p); return nullptr; }
 
     // twiddles, rounded once from long double: 4096-entry coarse table
@@ -317,7 +381,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                                ? icm::launch_build_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
-                               : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
+                               : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
+                                                         p->d_spec ? p->d_spec + p->spec_off[l] : nullptr);
             if (!hip_ok(e, "build level")) return -1;
         }
         // propagate: root g = payout (same for every q-plane)
@@ -349,7 +414,10 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                                ? icm::launch_prop_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
-                               : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof);
+                               : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
+                                                        (p->d_spec && !J.plist && l <= L - 2)
+                                                            ? p->d_spec + p->spec_off[l]
+                                                            : nullptr);
             if (!hip_ok(e, "propagate level")) return -1;
         }
         if (L == 1) {
@@ -905,6 +973,22 @@ int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax) {
 }
 
 int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan) { return plan ? plan->sb_max : -1; }
+
+int icm_hip_plan_set_spectrum_cache(IcmHipPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (!on) {
+        spec_release(plan);
+        return 0;
+    }
+    if (plan->d_spec || plan->spec_elems == 0) return 0;
+    if (!spec_alloc(plan)) {
+        set_err("not enough device memory for the spectrum cache at this plan's q-batch");
+        return -1;
+    }
+    return 0;
+}
+
+int icm_hip_plan_get_spectrum_cache(const IcmHipPlan *plan) { return plan ? (plan->d_spec ? 1 : 0) : -1; }
 
 int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }

@@ -2079,17 +2079,18 @@ static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq
 }
 
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                              hipStream_t s, Prof *pf) {
+                              hipStream_t s, Prof *pf, double2 *spec) {
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    double2 *Y = spec ? spec : bufA;
     PF_BEGIN();
-    ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+    ICM_TRY(col_fwd(F1, LdChildPair{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build<<<dim3(F1 / 2, npairs, J.nq), kBT, 0, s>>>(bufA, bufB, nrp, npairs, J.logF, tw, twShift);
+    k_rowpair_build<<<dim3(F1 / 2, npairs, J.nq), kBT, 0, s>>>(Y, bufB, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
     PF_BEGIN();
@@ -2099,22 +2100,26 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
 }
 
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                             hipStream_t s, Prof *pf) {
+                             hipStream_t s, Prof *pf, const double2 *spec) {
     const int F1 = J.F >> 12;
     const int nrp = J.np, npairs = (nrp + 1) / 2;   // slots (all parents without pruning)
     const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
     const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * nq * nrp * J.gpar.count;
-    PF_BEGIN();
-    ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
-           nq * nrp * colFl);
+    if (spec && J.plist) return hipErrorInvalidValue;   // cached spectra are per parent, not per slot
+    const double2 *Y1 = spec ? spec : bufA;
+    if (!spec) {
+        PF_BEGIN();
+        ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,LdChildPair>", F1),
+               level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
+    }
     PF_BEGIN();
     ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     PF_BEGIN();
-    k_rowpair_prop<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(bufA, bufA + r1, bufB, nrp, npairs, J.logF, tw,
+    k_rowpair_prop<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs, J.logF, tw,
                                                                        twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);

@@ -133,10 +133,14 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
 
 // One tree level through HBM with a three-pass four-step transform (F >= 8192).
 // bufA / bufB: scratch of (n_real[l] + ceil(n_real[l]/2)) * F * nq complex each.
+// spec (optional, nq * n_real[l] * F complex): the build keeps the column
+// transforms of the child pairs there, and the propagate of the same level
+// and q-planes reads them instead of transforming the children again (only
+// without a parent list: slots are then the parents themselves).
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                              double2 *bufB, hipStream_t s, Prof *pf);
+                              double2 *bufB, hipStream_t s, Prof *pf, double2 *spec = nullptr);
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                             double2 *bufB, hipStream_t s, Prof *pf);
+                             double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr);
 
 // Ragged batches.  Leaves: global blocks [q][NB][B+1] for q-planes
 // q0 .. q0 + nq - 1; extract writes partial[chunk][Ntot].  Levels: one

@@ -171,6 +171,15 @@ int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on);
  * comes from the MI355X tuning table. */
 int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax);
 int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan);
+/* Spectrum cache (no counterpart in the reference, whose large levels are
+ * cuFFT plans per level, src/gpu/gpu_api.cu): the column transforms of each
+ * four-step build level's child pairs stay in HBM until the propagate of the
+ * same q-batch, which then does not transform the children again.  Planned on
+ * by default when at least min(q-batch, 32) q-planes fit with it; on = 0
+ * frees it, on = 1 allocates it again (-1 when it does not fit).  Pruned
+ * subset runs keep their own child transforms. */
+int icm_hip_plan_set_spectrum_cache(IcmHipPlan *plan, int on);
+int icm_hip_plan_get_spectrum_cache(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_count(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
                              double *total_ns, double *alg_bytes, double *alg_flops);

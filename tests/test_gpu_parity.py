@@ -296,3 +296,36 @@ def test_benchmark_shape_full_size():
     mask = np.ones(n, bool)
     mask[targets] = False
     assert np.all(sub[mask] == 0.0)
+
+
+@pytest.mark.parametrize("n,k", [(65536, 65536), (65536, 16384), (20000, 20000)])
+def test_spectrum_cache_matches_recompute(n, k):
+    """The propagate reading the build's cached child transforms gives the
+    same bits as transforming the children again; subsets (which recompute)
+    and later runs with another Q agree too."""
+    rng = np.random.default_rng(n + k + 5)
+    S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+    p = np.arange(k, 0, -1, dtype=float)
+    p /= p.sum()
+    plan = icm_amd.Plan(S, k)
+    geo = icm_amd.plan_geometry(n, k, plan.summary()["B"])
+    assert max(geo["F"]) > 4096
+    say(f"spectrum cache n={n} k={k} on={plan.spectrum_cache}")
+    assert plan.spectrum_cache
+    cached = plan.run(p, Q=256)
+    cached64 = plan.run(p, Q=64)
+    targets = rng.choice(n, 16, replace=False)
+    plan.set_targets(targets)
+    sub = plan.run(p, Q=256)
+    plan.set_targets(None)
+    plan.set_spectrum_cache(False)
+    assert not plan.spectrum_cache
+    fresh = plan.run(p, Q=256)
+    fresh64 = plan.run(p, Q=64)
+    plan.set_spectrum_cache(True)
+    again = plan.run(p, Q=256)
+    plan.close()
+    assert np.array_equal(cached, fresh)
+    assert np.array_equal(cached64, fresh64)
+    assert np.array_equal(again, fresh)
+    np.testing.assert_allclose(sub[targets], fresh[targets], rtol=1e-10)
