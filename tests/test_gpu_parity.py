@@ -329,3 +329,25 @@ def test_spectrum_cache_matches_recompute(n, k):
     assert np.array_equal(cached64, fresh64)
     assert np.array_equal(again, fresh)
     np.testing.assert_allclose(sub[targets], fresh[targets], rtol=1e-10)
+
+
+@pytest.mark.parametrize("q_batch,Q", [(1, 3), (2, 5)])
+def test_spectrum_cache_across_q_batches(q_batch, Q):
+    """Runs that span several q-batches rewrite the cache per batch; each
+    batch's propagate must read its own batch's transforms (bit-identical to
+    recomputing them), for a quadratic payout whose result depends on every
+    coefficient, on two plans in a row (recycled device memory)."""
+    n = 65536
+    rng = np.random.default_rng(q_batch * 7 + Q)
+    S = rng.uniform(1.0, 100.0, n)
+    r = np.arange(n, dtype=float)
+    p = (n - r) ** 2
+    for rep in range(2):
+        plan = icm_amd.Plan(S, n, q_batch=q_batch)
+        say(f"cache across q-batches q_batch={q_batch} Q={Q} rep={rep} on={plan.spectrum_cache}")
+        assert plan.spectrum_cache
+        cached = plan.run(p, Q=Q)
+        plan.set_spectrum_cache(False)
+        fresh = plan.run(p, Q=Q)
+        plan.close()
+        assert np.array_equal(cached, fresh)
