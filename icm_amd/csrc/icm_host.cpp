@@ -527,17 +527,22 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         }
         return v;
     };
-    auto level = [&](icm::LevelJob &J, bool build) -> bool {
+    // spec: the level's spectrum cache, for the passes whose child pairs are
+    // the forward products P (built once, read by the A and B propagates)
+    auto level = [&](icm::LevelJob &J, bool build, double2 *spec = nullptr) -> bool {
         hipError_t e;
         if (build)
             e = (J.F <= icm::kMaxLdsF) ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                        : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s,
-                                                                 &p->prof);
+                                                                 &p->prof, spec);
         else
             e = (J.F <= icm::kMaxLdsF) ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                        : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s,
-                                                                &p->prof);
+                                                                &p->prof, spec);
         return hip_ok(e, build ? "vjp build level" : "vjp propagate level");
+    };
+    auto spec_of = [&](int l) -> double2 * {
+        return (p->d_spec && l >= 1 && l <= L - 2 && g.F[l] > icm::kMaxLdsF) ? p->d_spec + p->spec_off[l] : nullptr;
     };
     const int qstep = std::min(p->qb, p->qv);
     for (int q0 = 0; q0 < Qa; q0 += qstep) {
@@ -567,7 +572,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                     return -1;
                 continue;
             }
-            if (!level(J, true)) return -1;                       // P_X = P_L P_R
+            if (!level(J, true, spec_of(l))) return -1;           // P_X = P_L P_R
             J.parent.base = p->d_dcoef[l];
             J.child = view(l - 1, true);
             J.child_r = view(l - 1, false);
@@ -605,13 +610,13 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 cb = nbv;
                 continue;
             }
-            if (!level(J, false)) return -1;                      // A_L = A_X * R
+            if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
             if (!root) {
                 icm::LevelJob K = J;
                 K.gpar = Bpar;
                 K.gchild = Bout;
                 K.gfix = gfix;
-                if (!level(K, false)) return -1;                  // B_L = B_X * R
+                if (!level(K, false, spec_of(l))) return -1;     // B_L = B_X * R
             }
             icm::LevelJob K = J;
             K.gpar = Apar;

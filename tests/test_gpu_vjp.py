@@ -140,3 +140,25 @@ def test_vjp_large_shape_self_consistency():
         fd = (np.dot(v, icm_amd.equity(Sp, p, Q=256)) - np.dot(v, icm_amd.equity(Sm, p, Q=256))) / (2 * h)
         say(f"large n={n} coord {i}: dS={ds[i]:.6e} FD={fd:.6e}")
         assert abs(ds[i] - fd) <= 1e-5 * scale + 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("n,k,q_batch,Q", [(65536, 65536, 0, 64), (20000, 20000, 1, 3)])
+def test_vjp_spectrum_cache_matches_recompute(n, k, q_batch, Q):
+    """The adjoint reads the forward products' cached child transforms in its
+    A and B propagates; equities and dS are bit-identical to recomputing them,
+    within one q-batch and across several."""
+    rng = np.random.default_rng(n + Q)
+    S = rng.uniform(1.0, 100.0, n)
+    r = np.arange(k, dtype=float)
+    p = (k - r) ** 2
+    p /= p.sum()
+    v = rng.normal(size=n)
+    plan = icm_amd.Plan(S, k, q_batch=q_batch)
+    say(f"vjp spectrum cache n={n} k={k} q_batch={q_batch} Q={Q} on={plan.spectrum_cache}")
+    assert plan.spectrum_cache
+    e1, d1 = plan.vjp(p, v, Q=Q)
+    plan.set_spectrum_cache(False)
+    e0, d0 = plan.vjp(p, v, Q=Q)
+    plan.close()
+    assert np.array_equal(e1, e0)
+    assert np.array_equal(d1, d0)
