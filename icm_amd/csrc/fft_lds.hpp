@@ -112,6 +112,19 @@ template <int FS> struct ColLayout {
     __device__ static __forceinline__ int at(int item, int idx) { return idx * ITEMS + item; }
 };
 
+// The thread index through an empty asm statement: every pass then derives
+// its LDS addresses afresh instead of the compiler computing all passes'
+// addresses once and keeping them live (in registers) across the transform,
+// which pushes the 128-VGPR kernels into spills.  No instruction is emitted.
+// KEEP = true leaves the choice to the compiler (kernels that run one
+// workgroup per CU and have registers to spare measure faster that way).
+template <bool KEEP>
+__device__ __forceinline__ int pass_tid() {
+    int t = threadIdx.x;
+    if constexpr (!KEEP) asm volatile("" : "+v"(t));
+    return t;
+}
+
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
@@ -237,7 +250,7 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
     constexpr int NBI = FS / R;               // butterflies per transform
     constexpr int LOG_NBI = ilog2c(NBI);
     constexpr int PER = (kTile / NT) / R;     // butterflies per thread
-    const int t = threadIdx.x;
+    const int t = pass_tid<false>();
     const int Ns = 1 << logNs;
     double2 v[PER][R];
     int item[PER], jj[PER];
@@ -276,7 +289,7 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
 // St: void operator()(int item, int idx, double2 v); NoIO disables either.
 struct NoIO {};
 
-template <int FS, int R, bool INV, int NT, class TL, int PTS = kTile, class Ld, class St>
+template <int FS, int R, bool INV, int NT, class TL, int PTS = kTile, bool KEEP = false, class Ld, class St>
 __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const double2 *tw, const Ld &ld, const St &st) {
     constexpr bool LOAD = !std::is_same<Ld, NoIO>::value;
     constexpr bool STORE = !std::is_same<St, NoIO>::value;
@@ -284,7 +297,7 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
     constexpr int LOG_NBI = ilog2c(NBI);
     constexpr int PER = (PTS / NT) / R;
     static_assert(PTS == kTile || !TL::kItemFast, "item-fast layouts hold one tile");
-    const int t = threadIdx.x;
+    const int t = pass_tid<KEEP>();
     const int Ns = 1 << logNs;
     double2 v[PER][R];
     int item[PER], jj[PER];
@@ -330,7 +343,7 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
 // PTS = points transformed per call: kTile (one tile), or a multiple of it,
 // in which case every thread runs PTS / kTile butterflies per pass at the
 // same index of consecutive tiles (the radix stays kTile / NT).
-template <int FS, bool INV, int NT, class TL, int PTS = kTile, class Ld, class St>
+template <int FS, bool INV, int NT, class TL, int PTS = kTile, bool KEEP = false, class Ld, class St>
 __device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, const Ld &ld, const St &st) {
     constexpr int RMAX = kTile / NT;
     constexpr int LR = ilog2c(RMAX);
@@ -340,20 +353,20 @@ __device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, con
     static_assert(NP >= 2, "first and last pass must differ");
     if constexpr (std::is_same<Ld, NoIO>::value) __syncthreads();
     int logNs = 0;
-    if constexpr (REM == 1) { fft_pass_io<FS, 2, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 1; }
-    if constexpr (REM == 2) { fft_pass_io<FS, 4, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 2; }
-    if constexpr (REM == 3) { fft_pass_io<FS, 8, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{}); logNs += 3; }
+    if constexpr (REM == 1) { fft_pass_io<FS, 2, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, ld, NoIO{}); logNs += 1; }
+    if constexpr (REM == 2) { fft_pass_io<FS, 4, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, ld, NoIO{}); logNs += 2; }
+    if constexpr (REM == 3) { fft_pass_io<FS, 8, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, ld, NoIO{}); logNs += 3; }
 #pragma unroll
     for (int p = 0; p < LOG / LR; ++p) {
         const bool first = (REM == 0 && p == 0), last = (p == LOG / LR - 1);
         if (first && last) {
-            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, ld, st);
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, ld, st);
         } else if (first) {
-            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, ld, NoIO{});
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, ld, NoIO{});
         } else if (last) {
-            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, NoIO{}, st);
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, NoIO{}, st);
         } else {
-            fft_pass_io<FS, RMAX, INV, NT, TL, PTS>(lds, logNs, tw, NoIO{}, NoIO{});
+            fft_pass_io<FS, RMAX, INV, NT, TL, PTS, KEEP>(lds, logNs, tw, NoIO{}, NoIO{});
         }
         logNs += LR;
     }

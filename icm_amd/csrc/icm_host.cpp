@@ -529,7 +529,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
     };
     // spec: the level's spectrum cache, for the passes whose child pairs are
     // the forward products P (built once, read by the A and B propagates)
-    auto level = [&](icm::LevelJob &J, bool build, double2 *spec = nullptr) -> bool {
+    auto level = [&](icm::LevelJob &J, bool build, double2 *spec = nullptr, bool g_ready = false) -> bool {
         hipError_t e;
         if (build)
             e = (J.F <= icm::kMaxLdsF) ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
@@ -538,7 +538,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         else
             e = (J.F <= icm::kMaxLdsF) ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                        : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s,
-                                                                &p->prof, spec);
+                                                                &p->prof, spec, g_ready);
         return hip_ok(e, build ? "vjp build level" : "vjp propagate level");
     };
     auto spec_of = [&](int l) -> double2 * {
@@ -611,21 +611,24 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 continue;
             }
             if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
+            {
+                // same g pair as the pass above: its column transforms are
+                // still in bufA
+                icm::LevelJob K = J;
+                K.child = view(l - 1, true);
+                K.r_missing_zero = 1;
+                K.gchild = Bout;
+                K.gfix = gfix;
+                if (!level(K, false, nullptr, true)) return -1;   // B_L = A_X * dR
+            }
             if (!root) {
                 icm::LevelJob K = J;
                 K.gpar = Bpar;
                 K.gchild = Bout;
                 K.gfix = gfix;
-                if (!level(K, false, spec_of(l))) return -1;     // B_L = B_X * R
+                K.acc_out = 1;
+                if (!level(K, false, spec_of(l))) return -1;     //     + B_X * R
             }
-            icm::LevelJob K = J;
-            K.gpar = Apar;
-            K.child = view(l - 1, true);
-            K.r_missing_zero = 1;
-            K.gchild = Bout;
-            K.gfix = gfix;
-            K.acc_out = root ? 0 : 1;
-            if (!level(K, false)) return -1;                      //     + A_X * dR
             ca = na;
             cb = nbv;
         }

@@ -1630,10 +1630,10 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
                                                  const double2 *tw, int twShift) {
     (void)twShift;
     const double2 *r0 = Zb + (long long)k1 * kF2;
-    fft_tile_io<kF2, false, kRT, RTL>(T, tw, [&](int, int n2) { return r0[n2]; }, NoIO{});
+    fft_tile_io<kF2, false, kRT, RTL, kTile, true>(T, tw, [&](int, int n2) { return r0[n2]; }, NoIO{});
     if (k1 != k1m) {
         const double2 *r1 = Zb + (long long)k1m * kF2;
-        fft_tile_io<kF2, false, kRT, RTL>(T + RTL::IS, tw, [&](int, int n2) { return r1[n2]; }, NoIO{});
+        fft_tile_io<kF2, false, kRT, RTL, kTile, true>(T + RTL::IS, tw, [&](int, int n2) { return r1[n2]; }, NoIO{});
     }
 }
 
@@ -1644,12 +1644,12 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
 __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m,
                                                    const double2 *tw) {
     double2 *v0 = Vb + (long long)k1 * kF2;
-    fft_tile_io<kF2, true, kRT, RTL>(T, tw, NoIO{}, [&](int, int e, double2 v) {
+    fft_tile_io<kF2, true, kRT, RTL, kTile, true>(T, tw, NoIO{}, [&](int, int e, double2 v) {
         v0[e] = v;
     });
     if (k1 != k1m) {
         double2 *v1 = Vb + (long long)k1m * kF2;
-        fft_tile_io<kF2, true, kRT, RTL>(T + RTL::IS, tw, NoIO{}, [&](int, int e, double2 v) {
+        fft_tile_io<kF2, true, kRT, RTL, kTile, true>(T + RTL::IS, tw, NoIO{}, [&](int, int e, double2 v) {
             v1[e] = v;
         });
     }
@@ -1731,7 +1731,7 @@ __device__ __forceinline__ int row_of(int w, int F1, int h) {
 __device__ __forceinline__ void rowpair2_load_fft(double2 *T, const double2 *__restrict__ Zb, int rA, int rB,
                                                  const double2 *tw) {
     const double2 *z0 = Zb + (long long)rA * kF2, *z1 = Zb + (long long)rB * kF2;
-    fft_tile_io<kF2, false, kBT, RTL, 2 * kTile>(T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
+    fft_tile_io<kF2, false, kBT, RTL, 2 * kTile, true>(T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
 }
 
 // Inverse-transform both tiles and store their rows of V (twiddle as above);
@@ -1739,7 +1739,7 @@ __device__ __forceinline__ void rowpair2_load_fft(double2 *T, const double2 *__r
 // writing on return.
 __device__ __forceinline__ void rowpair2_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB,
                                                     const double2 *tw) {
-    fft_tile_io<kF2, true, kBT, RTL, 2 * kTile>(T, tw, NoIO{}, [&](int h, int e, double2 v) {
+    fft_tile_io<kF2, true, kBT, RTL, 2 * kTile, true>(T, tw, NoIO{}, [&](int h, int e, double2 v) {
         Vb[(long long)(h ? rB : rA) * kF2 + e] = v;
     });
 }
@@ -2100,7 +2100,7 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
 }
 
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                             hipStream_t s, Prof *pf, const double2 *spec) {
+                             hipStream_t s, Prof *pf, const double2 *spec, bool g_ready) {
     const int F1 = J.F >> 12;
     const int nrp = J.np, npairs = (nrp + 1) / 2;   // slots (all parents without pruning)
     const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
@@ -2115,9 +2115,11 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,LdChildPair>", F1),
                level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
     }
-    PF_BEGIN();
-    ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    if (!g_ready) {
+        PF_BEGIN();
+        ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    }
     PF_BEGIN();
     k_rowpair_prop<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs, J.logF, tw,
                                                                        twShift);

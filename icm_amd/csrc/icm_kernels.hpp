@@ -139,8 +139,11 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
 // without a parent list: slots are then the parents themselves).
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
                               double2 *bufB, hipStream_t s, Prof *pf, double2 *spec = nullptr);
+// g_ready: bufA already holds this J's g-pair column transforms (the previous
+// launch had the same gpar, nq and parents), so they are not computed again.
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                             double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr);
+                             double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr,
+                             bool g_ready = false);
 
 // Ragged batches.  Leaves: global blocks [q][NB][B+1] for q-planes
 // q0 .. q0 + nq - 1; extract writes partial[chunk][Ntot].  Levels: one
