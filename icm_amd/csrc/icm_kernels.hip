@@ -1592,78 +1592,23 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
     }
 }
 
-// Fused spectrum stage of a large level: a workgroup owns a row k1 and its
-// mirror row F1 - k1.  Every bin -K of row k1 lies in row F1 - k1 (position
-// F2-1-k2; rows 0 and F1/2 are their own mirrors, at (F2-k2) mod F2 and
-// F2-1-k2), so the forward row FFTs, the real-pair split, the pointwise
-// product and the inverse row FFTs happen without leaving the workgroup.
-// Both kernels keep two 4096-point rows in two LDS tiles (128 KiB).
+// Fused spectrum stage of a large level.  Every bin -K of row k1 lies in row
+// F1 - k1 (position F2-1-k2; rows 0 and F1/2 are their own mirrors, at
+// (F2-k2) mod F2 and F2-1-k2), so the forward row FFTs, the real-pair split,
+// the pointwise product and the inverse row FFTs of a row and its mirror row
+// happen without leaving the workgroup.
 //
-// Propagate (k_rowpair_prop): workgroup k1 = 0 .. F1/2 owns {k1, F1 - k1};
-// the tiles are transformed one after the other, and thread t owns positions
-// k2 = t + 512u of row k1 and their mirrors.  It holds the g spectra of both
-// parents in registers across the child transforms, which leaves no room for
-// transforming both tiles at once.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int mirror_k2(int k1, int k2) {
-    return (k1 == 0) ? ((kF2 - k2) & (kF2 - 1)) : (kF2 - 1 - k2);
-}
-
-// The fused row-pair kernels run 512 threads (8 points per thread) and keep
-// both rows' spectra in two LDS tiles (128 KiB): rows k1 -> T0, k1m -> T1.
-// Z[K] of position k2 of row k1 is T0[k2]; Z[-K] is T1[F2-1-k2] (or
-// T0[mirror] for the self-paired rows 0 and F1/2).
-constexpr int kRT = 512;
-constexpr int kRP = kTile / kRT;
-
-using RTL = SwzLayout<kF2>;   // one row per tile, T1 starts at RTL::IS
-
-__device__ __forceinline__ int row_at(int k2) { return RTL::at(0, k2); }
-__device__ __forceinline__ int mirror_idx(int k1, int k1m, int k2) {
-    return (k1 == k1m) ? row_at(mirror_k2(k1, k2)) : RTL::IS + row_at(kF2 - 1 - k2);
-}
-
-// Load rows k1 / k1m of one [k1][n2] array into the tiles and transform them;
-// the first radix-8 pass reads its points (n2 = t + 512 s, coalesced) straight
-// from global memory.
-__device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__restrict__ Zb, int k1, int k1m,
-                                                 const double2 *tw, int twShift) {
-    (void)twShift;
-    const double2 *r0 = Zb + (long long)k1 * kF2;
-    fft_tile_io<kF2, false, kRT, RTL, kTile, true>(T, tw, [&](int, int n2) { return r0[n2]; }, NoIO{});
-    if (k1 != k1m) {
-        const double2 *r1 = Zb + (long long)k1m * kF2;
-        fft_tile_io<kF2, false, kRT, RTL, kTile, true>(T + RTL::IS, tw, [&](int, int n2) { return r1[n2]; }, NoIO{});
-    }
-}
-
-// Inverse-transform the tiles and store rows k1 / k1m of V (the inverse
-// four-step twiddle is applied by the column inverse as it loads them); the
-// last pass writes straight to global memory, and the tiles are free for
-// writing on return.
-__device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int k1, int k1m,
-                                                   const double2 *tw) {
-    double2 *v0 = Vb + (long long)k1 * kF2;
-    fft_tile_io<kF2, true, kRT, RTL, kTile, true>(T, tw, NoIO{}, [&](int, int e, double2 v) {
-        v0[e] = v;
-    });
-    if (k1 != k1m) {
-        double2 *v1 = Vb + (long long)k1m * kF2;
-        fft_tile_io<kF2, true, kRT, RTL, kTile, true>(T + RTL::IS, tw, NoIO{}, [&](int, int e, double2 v) {
-            v1[e] = v;
-        });
-    }
-}
-
-// Build (k_rowpair_build): both tiles are transformed together.
-//
-// Workgroup w of a level owns two rows, one 4096-point LDS tile each:
+// Workgroup w of a level owns two rows, one 4096-point LDS tile each (128 KiB):
 // w >= 1 the pair {w, F1 - w}; w = 0 the two self-mirrored rows {0, F1/2}.
-// 512 threads; every thread runs one radix-8 butterfly of each tile per pass
-// at the same index (one barrier per pass for both rows, shared twiddles).
-// For the pointwise stages each thread owns 8 representatives
-// r = t + 512u, each standing for a bin K and its mirror -K
-// (the spectra of real sequences satisfy Z[-K] = conj Z[K], so one value per
+// Both rows go through every radix-8 pass together (one barrier per pass):
+//   build, 512 threads: each thread runs one butterfly of each tile at the
+//     same index (shared twiddles);
+//   propagate, 1024 threads: threads 0..511 run tile 0 and 512..1023 tile 1
+//     (16 waves per CU; the passes derive their LDS addresses afresh so the
+//     kernel fits 128 VGPRs while it holds a parent's g spectra).
+// For the pointwise stages each thread owns 4096 / threads representatives
+// r = t + threads u, each standing for a bin K and its mirror -K (the spectra
+// of real sequences satisfy Z[-K] = conj Z[K], so one value per
 // representative is kept); the representatives cover the 8192 positions of
 // the two tiles exactly once:
 //   w >= 1:           K at tile 0 position r,   -K at tile 1 position F2-1-r;
@@ -1673,8 +1618,12 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
 //                     double2 carries both: .x for bin 0, .y for bin 2048);
 //   w = 0, r >= 2048: K at tile 1 position r-2048, -K at tile 1 F2-1-(r-2048).
 // ---------------------------------------------------------------------------
-constexpr int kBT = 512;
-constexpr int kBU = kTile / kBT;       // representatives per thread
+constexpr int kBT = 512;    // build threads
+constexpr int kPT = 1024;   // propagate threads
+
+using RTL = SwzLayout<kF2>;   // tile h at h * RTL::IS
+
+__device__ __forceinline__ int row_at(int k2) { return RTL::at(0, k2); }
 
 struct RowRep {
     int ia, ib;      // LDS positions of K and -K (r = 0 of w = 0: bins 0 and 2048)
@@ -1725,48 +1674,65 @@ __device__ __forceinline__ int row_of(int w, int F1, int h) {
     return (w == 0) ? (h ? F1 / 2 : 0) : (h ? F1 - w : w);
 }
 
+// Pass shape of the two-row transforms for a workgroup of NTH threads (the
+// radix stays 8: NT = 512 butterflies per tile and pass).
+template <int NTH> struct RowPass;
+template <> struct RowPass<512> {
+    static constexpr int PTS = 2 * kTile;   // two butterflies per thread
+    static constexpr bool KEEP = true;      // addresses left to the compiler (256 VGPRs available)
+};
+template <> struct RowPass<1024> {
+    static constexpr int PTS = kTile;       // one butterfly per thread, tile = thread / 512
+    static constexpr bool KEEP = false;     // fresh addresses per pass: 128 VGPRs
+};
+
 // Load both rows of one [k1][n2] array and transform them; the first radix-8
 // pass reads its points (n2 = t + 512 s, coalesced) straight from global
 // memory.  Ends with the last pass's barrier.
-__device__ __forceinline__ void rowpair2_load_fft(double2 *T, const double2 *__restrict__ Zb, int rA, int rB,
+template <int NTH>
+__device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__restrict__ Zb, int rA, int rB,
                                                  const double2 *tw) {
     const double2 *z0 = Zb + (long long)rA * kF2, *z1 = Zb + (long long)rB * kF2;
-    fft_tile_io<kF2, false, kBT, RTL, 2 * kTile, true>(T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
+    fft_tile_io<kF2, false, kTile / 8, RTL, RowPass<NTH>::PTS, RowPass<NTH>::KEEP>(
+        T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
 }
 
-// Inverse-transform both tiles and store their rows of V (twiddle as above);
-// the last pass writes straight to global memory, and the tiles are free for
+// Inverse-transform both tiles and store their rows of V (the inverse
+// four-step twiddle is applied by the column inverse as it loads them); the
+// last pass writes straight to global memory, and the tiles are free for
 // writing on return.
-__device__ __forceinline__ void rowpair2_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB,
-                                                    const double2 *tw) {
-    fft_tile_io<kF2, true, kBT, RTL, 2 * kTile, true>(T, tw, NoIO{}, [&](int h, int e, double2 v) {
-        Vb[(long long)(h ? rB : rA) * kF2 + e] = v;
-    });
+template <int NTH>
+__device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB,
+                                                   const double2 *tw) {
+    fft_tile_io<kF2, true, kTile / 8, RTL, RowPass<NTH>::PTS, RowPass<NTH>::KEEP>(
+        T, tw, NoIO{}, [&](int h, int e, double2 v) { Vb[(long long)(h ? rB : rA) * kF2 + e] = v; });
 }
 
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
 __global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
                                                        int nrp, int npairs, int logF,
                                                        const double2 *tw, int twShift) {
+    constexpr int RU = kTile / kBT;   // representatives per thread
     __shared__ double2 T[2 * RTL::IS];
     const int F = 1 << logF, F1 = F >> 12;
     const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
     const int p = 2 * pair;
-    double2 P1[kBU], P2[kBU];
-    rowpair2_load_fft(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
+    (void)twShift;
+    double2 P1[RU], P2[RU];
+    rowpair_load_fft<kBT>(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
-    for (int u = 0; u < kBU; ++u) {
+    for (int u = 0; u < RU; ++u) {
         const RowRep q = row_rep(w, t + u * kBT);
         double2 X, Yv;
         row_split(q, T[q.ia], T[q.ib], X, Yv);
         P1[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
     }
     if (p + 1 < nrp) {  // uniform over the workgroup
-        rowpair2_load_fft(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
+        rowpair_load_fft<kBT>(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
 #pragma unroll
-        for (int u = 0; u < kBU; ++u) {
+        for (int u = 0; u < RU; ++u) {
             const RowRep q = row_rep(w, t + u * kBT);
             double2 X, Yv;
             row_split(q, T[q.ia], T[q.ib], X, Yv);
@@ -1774,61 +1740,46 @@ __global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < kBU; ++u) P2[u] = make_double2(0.0, 0.0);
+        for (int u = 0; u < RU; ++u) P2[u] = make_double2(0.0, 0.0);
     }
     // every thread rewrites only the positions it read
 #pragma unroll
-    for (int u = 0; u < kBU; ++u) row_put(T, row_rep(w, t + u * kBT), P1[u], P2[u]);
-    rowpair2_ifft_store(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    for (int u = 0; u < RU; ++u) row_put(T, row_rep(w, t + u * kBT), P1[u], P2[u]);
+    rowpair_ifft_store<kBT>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 }
 
 // Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
-__global__ __launch_bounds__(512) void k_rowpair_prop(const double2 *__restrict__ Y1, const double2 *__restrict__ Y3,
+__global__ __launch_bounds__(kPT) void k_rowpair_prop(const double2 *__restrict__ Y1, const double2 *__restrict__ Y3,
                                                       double2 *__restrict__ V1, int nrp, int npairs, int logF,
                                                       const double2 *tw, int twShift) {
+    constexpr int RU = kTile / kPT;
     __shared__ double2 T[2 * RTL::IS];
     const int F = 1 << logF, F1 = F >> 12;
-    const int k1 = blockIdx.x, k1m = (k1 == 0) ? 0 : F1 - k1;
+    const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
-    double2 G1[kRP], G2[kRP];
-    rowpair_load_fft(T, Y3 + ((long long)qi * npairs + pair) * F, k1, k1m, tw, twShift);
+    (void)twShift;
+    double2 G1[RU], G2[RU];
+    rowpair_load_fft<kPT>(T, Y3 + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 #pragma unroll
-    for (int u = 0; u < kRP; ++u) {
-        const int e = t + u * kRT;
-        const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
-        G1[u] = split_x(a, b);
-        G2[u] = split_y(a, b);
+    for (int u = 0; u < RU; ++u) {
+        const RowRep q = row_rep(w, t + u * kPT);
+        row_split(q, T[q.ia], T[q.ib], G1[u], G2[u]);
     }
-    // one parent: U1 = g conj(R) + i g conj(L) on its child pair's rows
-    auto parent = [&](int p, const double2 (&Gw)[kRP]) {
-        __syncthreads();
-        rowpair_load_fft(T, Y1 + ((long long)qi * nrp + p) * F, k1, k1m, tw, twShift);
-        if (k1 != k1m) {
-            // the thread owns both positions it reads: update in place
+    // one parent: its child pair's rows, U1 in place, inverse rows
+    auto parent = [&](int p, const double2 (&Gw)[RU]) {
+        rowpair_load_fft<kPT>(T, Y1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
-            for (int u = 0; u < kRP; ++u) {
-                const int e = t + u * kRT;
-                const int ia = row_at(e), ib = mirror_idx(k1, k1m, e);
-                const double2 a = T[ia], b = T[ib];
-                const double2 Ua = c_mulc(Gw[u], split_y(a, b));   // g conj(R)
-                const double2 Ub = c_mulc(Gw[u], split_x(a, b));   // g conj(L)
-                T[ia] = pack_ai(Ua, Ub);
-                T[ib] = pack_ai(c_conj(Ua), c_conj(Ub));
-            }
-        } else {
-            double2 U[kRP];
-#pragma unroll
-            for (int u = 0; u < kRP; ++u) {
-                const int e = t + u * kRT;
-                const double2 a = T[row_at(e)], b = T[mirror_idx(k1, k1m, e)];
-                U[u] = pack_ai(c_mulc(Gw[u], split_y(a, b)), c_mulc(Gw[u], split_x(a, b)));
-            }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < kRP; ++u) T[row_at(t + u * kRT)] = U[u];
+        for (int u = 0; u < RU; ++u) {
+            const RowRep q = row_rep(w, t + u * kPT);
+            double2 L, R;
+            row_split(q, T[q.ia], T[q.ib], L, R);
+            if (q.fixed)
+                row_put(T, q, make_double2(Gw[u].x * R.x, Gw[u].y * R.y), make_double2(Gw[u].x * L.x, Gw[u].y * L.y));
+            else
+                row_put(T, q, c_mulc(Gw[u], R), c_mulc(Gw[u], L));   // g conj(R), g conj(L)
         }
-        rowpair_ifft_store(T, V1 + ((long long)qi * nrp + p) * F, k1, k1m, tw);
+        rowpair_ifft_store<kPT>(T, V1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
     };
     parent(2 * pair, G1);
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
@@ -2121,8 +2072,8 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     }
     PF_BEGIN();
-    k_rowpair_prop<<<dim3(F1 / 2 + 1, npairs, J.nq), kRT, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs, J.logF, tw,
-                                                                       twShift);
+    k_rowpair_prop<<<dim3(F1 / 2, npairs, J.nq), kPT, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs, J.logF, tw,
+                                                               twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     PF_BEGIN();
