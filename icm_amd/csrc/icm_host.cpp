@@ -78,6 +78,8 @@ struct IcmHipPlan {
     double *d_dblocks = nullptr;         // [qv][nb][B+1] tangent block products
     std::vector<double *> d_dcoef;       // level l: [qv][n_real][c] tangent products
     double *d_bv[2] = {nullptr, nullptr};// B adjoint ping-pong, [qv][n_real][c]
+    double2 *d_dspec = nullptr;          // tangent spectrum cache: the dP pairs' row-stage inputs,
+    std::vector<long long> dspec_off;    //   per four-step build level [qv][n_real[l]][F[l]]
     long long bv_elems = 0;
     double *d_v = nullptr;               // cotangent, sorted order
     double *d_partial_ds = nullptr;      // [nchunk][n]
@@ -110,7 +112,7 @@ void plan_free(IcmHipPlan *p) {
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_spec); f(p->d_partial); f(p->d_acc);
     f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
     f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
-    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -469,8 +471,19 @@ bool ensure_vjp(IcmHipPlan *p) {
     per += 2 * be * 8;
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) return false;
-    int qv = (int)std::max<long long>(1, (long long)(0.6 * (double)freeb) / std::max<long long>(per, 1));
+    const long long budget = (long long)(0.6 * (double)freeb);
+    int qv = (int)std::max<long long>(1, budget / std::max<long long>(per, 1));
     qv = std::min(qv, p->qb);
+    // the tangent spectrum cache, beside the forward's, while the q-batch
+    // keeps at least min(qv, 32) planes
+    bool dspec = false;
+    if (p->d_spec && p->spec_elems > 0) {
+        const int qd = std::min<long long>(p->qb, std::max<long long>(1, budget / (per + 16 * p->spec_elems)));
+        if (qd >= std::min(qv, 32)) {
+            qv = qd;
+            dspec = true;
+        }
+    }
     p->bv_elems = be;
     bool ok = dalloc(p, &p->d_dblocks, (size_t)qv * g.nb * (g.B + 1)) && dalloc(p, &p->d_bv[0], (size_t)qv * be) &&
               dalloc(p, &p->d_bv[1], (size_t)qv * be) && dalloc(p, &p->d_v, g.n) &&
@@ -479,6 +492,20 @@ bool ensure_vjp(IcmHipPlan *p) {
     p->d_dcoef.assign(g.L, nullptr);
     for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * g.n_real[l] * g.c[l]);
     if (!ok) return false;
+    if (dspec) {
+        p->dspec_off.assign(g.L, -1);
+        long long off = 0;
+        for (int l = 1; l <= g.L - 2; ++l) {
+            if (g.F[l] > icm::kMaxLdsF) {
+                p->dspec_off[l] = off;
+                off += (long long)qv * g.n_real[l] * g.F[l];
+            }
+        }
+        if (off > 0 && !dalloc(p, &p->d_dspec, (size_t)off)) {
+            (void)hipGetLastError();   // no tangent cache: the three-pass tangent builds
+            p->d_dspec = nullptr;
+        }
+    }
     p->qv = qv;
     return true;
 }
@@ -544,6 +571,9 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
     auto spec_of = [&](int l) -> double2 * {
         return (p->d_spec && l >= 1 && l <= L - 2 && g.F[l] > icm::kMaxLdsF) ? p->d_spec + p->spec_off[l] : nullptr;
     };
+    auto dspec_of = [&](int l) -> double2 * {
+        return (p->d_dspec && spec_of(l)) ? p->d_dspec + p->dspec_off[l] : nullptr;
+    };
     const int qstep = std::min(p->qb, p->qv);
     for (int q0 = 0; q0 < Qa; q0 += qstep) {
         const int nq = std::min(qstep, Qa - q0);
@@ -574,6 +604,14 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             }
             if (!level(J, true, spec_of(l))) return -1;           // P_X = P_L P_R
             J.parent.base = p->d_dcoef[l];
+            if (dspec_of(l)) {                                    // dP_X from both pairs' spectra
+                J.tchild = view(l - 1, true);
+                if (!hip_ok(icm::launch_build_large_t(J, p->d_tw, p->twShift, spec_of(l), dspec_of(l), p->d_bufB, s,
+                                                      &p->prof),
+                            "vjp tangent build level"))
+                    return -1;
+                continue;
+            }
             J.child = view(l - 1, true);
             J.child_r = view(l - 1, false);
             if (!level(J, true)) return -1;                       // dP_X = dP_L P_R
@@ -619,7 +657,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 K.r_missing_zero = 1;
                 K.gchild = Bout;
                 K.gfix = gfix;
-                if (!level(K, false, nullptr, true)) return -1;   // B_L = A_X * dR
+                if (!level(K, false, dspec_of(l), true)) return -1;   // B_L = A_X * dR
             }
             if (!root) {
                 icm::LevelJob K = J;

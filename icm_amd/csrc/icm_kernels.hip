@@ -1512,6 +1512,32 @@ struct StParentPair {     // array a = pair of parents (build)
         }
     }
 };
+struct StParentTangent {  // array a = pair of parents (adjoint tangent build): dP_X into J.parent
+    LevelJob J;              // child = P of the children, tchild = their dP
+    __device__ double exact(int qi, int p, int m) const {   // dP_L[m] P_R[m] + P_L[m] dP_R[m]
+        const double pl = view_at(J.child, qi, 2 * p, m), dl = view_at(J.tchild, qi, 2 * p, m);
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        const double pr = has_r ? view_at(J.child, qi, 2 * p + 1, m) : (m == 0 ? 1.0 : 0.0);
+        const double dr = has_r ? view_at(J.tchild, qi, 2 * p + 1, m) : 0.0;
+        return dl * pr + pl * dr;
+    }
+    __device__ void operator()(int a, int qi, int n, double2 v) const {
+        const double invF = 1.0 / (double)J.F;
+        double *pb = J.parent.base + (long long)qi * J.parent.plane;
+        const int p = 2 * a;
+        const int ps = J.parent.stride, pc = J.parent.count;
+        if (n < pc) {
+            if (p < J.nr_parent) put(&pb[(long long)p * ps + n], (n == 0) ? exact(qi, p, 0) : v.x * invF, J.acc_out);
+            if (p + 1 < J.nr_parent)
+                put(&pb[(long long)(p + 1) * ps + n], (n == 0) ? exact(qi, p + 1, 0) : v.y * invF, J.acc_out);
+        }
+        if (n == 0 && pc > J.F) {
+            const int top = J.child.count - 1;
+            if (p < J.nr_parent) put(&pb[(long long)p * ps + J.F], exact(qi, p, top), J.acc_out);
+            if (p + 1 < J.nr_parent) put(&pb[(long long)(p + 1) * ps + J.F], exact(qi, p + 1, top), J.acc_out);
+        }
+    }
+};
 struct StGChildren {      // array a = slot (propagate)
     LevelJob J;
     __device__ void operator()(int a, int qi, int n, double2 v) const {
@@ -1785,6 +1811,53 @@ __global__ __launch_bounds__(kPT) void k_rowpair_prop(const double2 *__restrict_
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
 }
 
+// Adjoint tangent build: V[pair] rows = inverse rows of
+// U = dP_X[p] + i dP_X[p+1], dP_X = dP_L P_R + P_L dP_R, from the row-stage
+// inputs of the children's P pairs (SP, the spectrum cache) and of their dP
+// pairs (SD); both laid out [qi][nrp][F].
+__global__ __launch_bounds__(kPT) void k_rowpair_build_t(const double2 *__restrict__ SP,
+                                                         const double2 *__restrict__ SD, double2 *__restrict__ V,
+                                                         int nrp, int npairs, int logF, const double2 *tw,
+                                                         int twShift) {
+    constexpr int RU = kTile / kPT;
+    __shared__ double2 T[2 * RTL::IS];
+    const int F = 1 << logF, F1 = F >> 12;
+    const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
+    const int pair = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    (void)twShift;
+    auto tangent = [&](int p, double2 (&D)[RU]) {
+        rowpair_load_fft<kPT>(T, SP + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        double2 PL[RU], PR[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const RowRep q = row_rep(w, t + u * kPT);
+            row_split(q, T[q.ia], T[q.ib], PL[u], PR[u]);
+        }
+        rowpair_load_fft<kPT>(T, SD + ((long long)qi * nrp + p) * F, rA, rB, tw);
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const RowRep q = row_rep(w, t + u * kPT);
+            double2 DL, DR;
+            row_split(q, T[q.ia], T[q.ib], DL, DR);
+            D[u] = q.fixed ? make_double2(DL.x * PR[u].x + PL[u].x * DR.x, DL.y * PR[u].y + PL[u].y * DR.y)
+                           : c_add(c_mul(DL, PR[u]), c_mul(PL[u], DR));
+        }
+    };
+    double2 D1[RU], D2[RU];
+    tangent(2 * pair, D1);
+    if (2 * pair + 1 < nrp) {   // uniform over the workgroup
+        tangent(2 * pair + 1, D2);
+    } else {
+#pragma unroll
+        for (int u = 0; u < RU; ++u) D2[u] = make_double2(0.0, 0.0);
+    }
+    // every thread rewrites only the positions it read
+#pragma unroll
+    for (int u = 0; u < RU; ++u) row_put(T, row_rep(w, t + u * kPT), D1[u], D2[u]);
+    rowpair_ifft_store<kPT>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -2050,6 +2123,31 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     return hipSuccess;
 }
 
+hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShift, const double2 *specP,
+                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf) {
+    if (!specP || !specD) return hipErrorInvalidValue;
+    const int F1 = J.F >> 12;
+    const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
+    const double F = J.F, nq = J.nq;
+    const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    LevelJob K = J;                     // the children's dP pairs, a missing right child 0
+    K.child = J.tchild;
+    K.child_r = CoefView{};
+    K.r_missing_zero = 1;
+    PF_BEGIN();
+    ICM_TRY(col_fwd(F1, LdChildPair{K}, specD, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
+           nq * nrp * colFl);
+    PF_BEGIN();
+    k_rowpair_build_t<<<dim3(F1 / 2, npairs, J.nq), kPT, 0, s>>>(specP, specD, bufB, nrp, npairs, J.logF, tw, twShift);
+    ICM_TRY(hipGetLastError());
+    PF_END("k_rowpair_build_t", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
+    PF_BEGIN();
+    ICM_TRY(col_inv(F1, bufB, StParentTangent{J}, npairs, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_inv<%d,StParentTangent>", F1), 16.0 * nq * npairs * F + 8.0 * nq * nrp * J.parent.count,
+           nq * npairs * kF2 * fft_flops(F1));
+    return hipSuccess;
+}
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
                              hipStream_t s, Prof *pf, const double2 *spec, bool g_ready) {
     const int F1 = J.F >> 12;

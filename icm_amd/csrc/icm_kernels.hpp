@@ -139,6 +139,13 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
 // without a parent list: slots are then the parents themselves).
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
                               double2 *bufB, hipStream_t s, Prof *pf, double2 *spec = nullptr);
+// Adjoint tangent build of a large level: dP_X = dP_L P_R + P_L dP_R into
+// J.parent (J.child = P, J.tchild = dP of the children).  specP holds the
+// level's P-pair row-stage inputs (the spectrum cache, written by the P
+// build); the dP-pair inputs are written to specD (nq * n_real[l] * F
+// complex) for the A * dR propagate of the same q-batch to reuse.
+hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShift, const double2 *specP,
+                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf);
 // g_ready: bufA already holds this J's g-pair column transforms (the previous
 // launch had the same gpar, nq and parents), so they are not computed again.
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
