@@ -53,6 +53,8 @@ __device__ __forceinline__ double2 tw_big2(const double2 *tw, int twShift, int m
     return make_double2(c.x * f.x - c.y * f.y, c.x * f.y + c.y * f.x);
 }
 
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
 template <int FS> struct TileLayout {
     static constexpr bool kItemFast = false;        // butterflies of one transform on consecutive lanes
     static constexpr int IS = FS + FS / 16 + 1;     // item stride (odd)
@@ -94,6 +96,33 @@ template <int FS> struct SwzLayout {
     }
     __device__ static __forceinline__ int at(int item, int idx) {
         return item * FS + (idx ^ ((idx >> 3) & 7) ^ g(item));
+    }
+};
+
+// Unpadded, XOR-swizzled tile for radix-16 passes (256 butterflies per pass)
+// over transforms of FS = 128 .. 4096 points.  Under the gfx950 LDS rules
+// above: a first radix-16 pass writes with stride 16 (idx = 16 l + o over
+// lanes l), which an XOR of index bits 4..6 into the low three bits spreads
+// over 8 distinct 16-byte units per 8-lane group; when log2(FS) is not a
+// multiple of 4, the first pass is a radix-2/4/8 one writing with stride
+// 2/4/8, and the XOR takes bits 3..5 as well.  Later passes write, and every
+// pass reads, runs of consecutive elements, which such XORs only permute
+// inside their aligned 8-blocks.  g(it) flips bit 3 for odd transforms of
+// FS = 128, where one 16-lane read group spans two transforms.
+template <int FS> struct Swz16Layout {
+    static_assert(FS >= 128 && FS <= kTile, "radix-16 tiles hold transforms of 128 .. 4096 points");
+    static constexpr bool kItemFast = false;
+    static constexpr int IS = FS;
+    static constexpr int ITEMS = kTile / FS;
+    static constexpr int SIZE = kTile;
+    static constexpr bool kMixed = (ilog2c(FS) % 4) != 0;
+    __device__ static __forceinline__ int at(int item, int idx) {
+        if constexpr (kMixed) {
+            const int g = (FS == 128) ? ((item & 1) << 3) : 0;
+            return item * FS + (idx ^ (((idx >> 3) ^ (idx >> 4)) & 7) ^ g);
+        } else {
+            return item * FS + (idx ^ ((idx >> 4) & 7));
+        }
     }
 };
 
@@ -213,7 +242,6 @@ template <int R, bool INV> __device__ __forceinline__ void dft(double2 *a) {
     else dft16<INV>(a);
 }
 
-constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
 
 // v[s] *= w^s (s = 1..R-1), w = exp(-+2 pi i k / (Ns R)).  Only w, w^2, w^4
 // and w^8 come from the table (correctly rounded); the other powers are

@@ -718,9 +718,17 @@ __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int 
 // aliases, and it is replaced by the exact L0 R0; coefficient F (if stored) is
 // the exact product of the leading terms.
 // ---------------------------------------------------------------------------
-// LDS-level kernels: 512 threads, 8 tile points per thread (radix-8 passes).
+// LDS-level kernels: LT threads per 4096-point tile, 4096 / LT points per
+// thread: 512 (radix-8 passes) or 256 (radix-16 passes, two workgroups and
+// 256 VGPRs per CU).  The plain build and propagate take 256 threads at
+// F = 128 and 256, where that measured faster (1.1x); at F >= 512, in the
+// fused adjoint kernels (which spill at 256 VGPRs) and in the ragged-batch
+// kernels (which switch over F) 512 threads are faster.
 constexpr int kLT = 512;
 constexpr int kLP = kTile / kLT;
+template <int F, int LT> struct LdsTL { using type = SwzLayout<F>; };
+template <int F> struct LdsTL<F, 256> { using type = Swz16Layout<F>; };
+template <int F> constexpr int lds_threads() { return (F == 128 || F == 256) ? 256 : 512; }
 // Largest padded tile (F = 16): LDS of the kernels that switch over F.
 constexpr int kTileMaxSize = SwzLayout<16>::SIZE;
 
@@ -746,16 +754,17 @@ struct FlatMap {    // v = q * npair + pair over (q-plane, pair), from v0
     }
 };
 
-template <int F, class Map>
+template <int F, class Map, int LT = kLT>
 __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                int twShift) {
-    using TL = SwzLayout<F>;
+    using TL = typename LdsTL<F, LT>::type;
+    constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
     (void)twShift;
 
-    double2 P1[kLP / 2];
+    double2 P1[LP / 2];
 #pragma unroll
     for (int which = 0; which < 2; ++which) {
         // first pass straight from the children (global -> registers)
@@ -764,10 +773,10 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
             M.at(item, qi, pr);
             return load_child_pair(J, qi, 2 * pr + which, m);
         };
-        fft_tile_io<F, false, kLT, TL>(lds, tw, ld, NoIO{});
+        fft_tile_io<F, false, LT, TL>(lds, tw, ld, NoIO{});
 #pragma unroll
-        for (int u = 0; u < kLP / 2; ++u) {
-            const int ps = t + u * kLT;
+        for (int u = 0; u < LP / 2; ++u) {
+            const int ps = t + u * LT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
             const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
             const double2 za = lds[ia], zb = lds[ib];
@@ -807,13 +816,18 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
                 put(&pb[(long long)(p + 1) * ps_ + F], child_prod_at(J, qi, p + 1, top), J.acc_out);
         }
     };
-    fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
+    fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, st);
 }
 
+// Two workgroups per CU (64 KiB of LDS each): 128 VGPRs at 512 threads,
+// 256 at 256 threads.
 template <int F>
-__global__ __launch_bounds__(512, 4) void k_build_lds(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[SwzLayout<F>::SIZE];
-    build_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+__global__ __launch_bounds__(lds_threads<F>(), lds_threads<F>() == 256 ? 2 : 4) void k_build_lds(LevelJob J,
+                                                                                               const double2 *tw,
+                                                                                               int twShift) {
+    __shared__ double2 lds[kTile];
+    build_lds_body<F, PlainMap, lds_threads<F>()>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y},
+                                                  tw, twShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,12 +840,14 @@ __global__ __launch_bounds__(512, 4) void k_build_lds(LevelJob J, const double2 
 // ---------------------------------------------------------------------------
 // IO = true: the g and child transforms take their first pass straight from
 // global memory and the inverses store straight to the children's g (two LDS
-// round trips fewer per transform); at 128 VGPRs (two workgroups per CU) the
-// extra live registers spill, so the plain kernel keeps the LDS path.
-template <int F, bool IO, class Map>
+// round trips fewer per transform); at 128 VGPRs (512 threads, two workgroups
+// per CU) the extra live registers spill, so only the 256-thread kernels
+// (256 VGPRs) take it.
+template <int F, bool IO, class Map, int LT = kLT>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift) {
-    using TL = SwzLayout<F>;
+    using TL = typename LdsTL<F, LT>::type;
+    constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -851,19 +867,19 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         return z;
     };
     if constexpr (IO) {
-        fft_tile_io<F, false, kLT, TL>(lds, tw, ldg, NoIO{});
+        fft_tile_io<F, false, LT, TL>(lds, tw, ldg, NoIO{});
     } else {
 #pragma unroll
-        for (int u = 0; u < kLP; ++u) {
-            const int e = t + u * kLT;
+        for (int u = 0; u < LP; ++u) {
+            const int e = t + u * LT;
             lds[TL::at(e >> LOGF, e & (F - 1))] = ldg(e >> LOGF, e & (F - 1));
         }
-        fft_tile<F, false, kLT, TL>(lds, tw, twShift);
+        fft_tile<F, false, LT, TL>(lds, tw, twShift);
     }
-    double2 G1[kLP / 2], G2[kLP / 2];
+    double2 G1[LP / 2], G2[LP / 2];
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) {
@@ -885,18 +901,18 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             return load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
         };
         if constexpr (IO) {
-            fft_tile_io<F, false, kLT, TL>(lds, tw, ldc, NoIO{});
+            fft_tile_io<F, false, LT, TL>(lds, tw, ldc, NoIO{});
         } else {
 #pragma unroll
-            for (int u = 0; u < kLP; ++u) {
-                const int e = t + u * kLT;
+            for (int u = 0; u < LP; ++u) {
+                const int e = t + u * LT;
                 lds[TL::at(e >> LOGF, e & (F - 1))] = ldc(e >> LOGF, e & (F - 1));
             }
-            fft_tile<F, false, kLT, TL>(lds, tw, twShift);
+            fft_tile<F, false, LT, TL>(lds, tw, twShift);
         }
 #pragma unroll
-        for (int u = 0; u < kLP / 2; ++u) {
-            const int ps = t + u * kLT;
+        for (int u = 0; u < LP / 2; ++u) {
+            const int ps = t + u * LT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
             const double2 g = which == 0 ? G1[u] : G2[u];
             const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
@@ -927,12 +943,12 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             }
         };
         if constexpr (IO) {
-            fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
+            fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, st);
         } else {
-            fft_tile<F, true, kLT, TL>(lds, tw, twShift);
+            fft_tile<F, true, LT, TL>(lds, tw, twShift);
 #pragma unroll
-            for (int u = 0; u < kLP; ++u) {
-                const int e = t + u * kLT;
+            for (int u = 0; u < LP; ++u) {
+                const int e = t + u * LT;
                 st(e >> LOGF, e & (F - 1), lds[TL::at(e >> LOGF, e & (F - 1))]);
             }
             __syncthreads();
@@ -940,10 +956,15 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
     }
 }
 
+// 256 threads: registers for the IO passes (first pass from global memory,
+// last pass to the children's g).
 template <int F>
-__global__ __launch_bounds__(512, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[SwzLayout<F>::SIZE];
-    prop_lds_body<F, false>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+__global__ __launch_bounds__(lds_threads<F>(), lds_threads<F>() == 256 ? 2 : 4) void k_prop_lds(LevelJob J,
+                                                                                              const double2 *tw,
+                                                                                              int twShift) {
+    __shared__ double2 lds[kTile];
+    prop_lds_body<F, lds_threads<F>() == 256, PlainMap, lds_threads<F>()>(
+        J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -1424,14 +1445,14 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
 
 
 template <int F>
-__global__ __launch_bounds__(512) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[SwzLayout<F>::SIZE];
+__global__ __launch_bounds__(kLT) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[kTile];
     build_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 template <int F>
-__global__ __launch_bounds__(512) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
-    __shared__ double2 lds[SwzLayout<F>::SIZE];
+__global__ __launch_bounds__(kLT) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
+    __shared__ double2 lds[kTile];
     prop_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
@@ -1626,12 +1647,9 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
 //
 // Workgroup w of a level owns two rows, one 4096-point LDS tile each (128 KiB):
 // w >= 1 the pair {w, F1 - w}; w = 0 the two self-mirrored rows {0, F1/2}.
-// Both rows go through every radix-8 pass together (one barrier per pass):
-//   build, 512 threads: each thread runs one butterfly of each tile at the
-//     same index (shared twiddles);
-//   propagate, 1024 threads: threads 0..511 run tile 0 and 512..1023 tile 1
-//     (16 waves per CU; the passes derive their LDS addresses afresh so the
-//     kernel fits 128 VGPRs while it holds a parent's g spectra).
+// Both rows go through every pass together (one barrier per pass): threads
+// 0..255 run tile 0 and 256..511 tile 1, radix-16 butterflies of 16 points
+// per thread (8 waves per CU, up to 256 VGPRs each).
 // For the pointwise stages each thread owns 4096 / threads representatives
 // r = t + threads u, each standing for a bin K and its mirror -K (the spectra
 // of real sequences satisfy Z[-K] = conj Z[K], so one value per
@@ -1644,31 +1662,41 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
 //                     double2 carries both: .x for bin 0, .y for bin 2048);
 //   w = 0, r >= 2048: K at tile 1 position r-2048, -K at tile 1 F2-1-(r-2048).
 // ---------------------------------------------------------------------------
-constexpr int kBT = 512;    // build threads
-constexpr int kPT = 1024;   // propagate threads
 
-using RTL = SwzLayout<kF2>;   // tile h at h * RTL::IS
+// Row shape: 512 threads, tile = thread / 256, one radix-16 butterfly per
+// thread and pass (three passes per 4096-point row, two LDS round trips),
+// on the layout that keeps radix-16 passes free of bank conflicts.  KEEP:
+// addresses held by the compiler (build, propagate) or derived afresh per
+// pass (tangent build, whose four live spectra leave no registers for them).
+template <bool K> struct RowR16 {
+    static constexpr int NTH = 512, NT = 256, PTS = kTile;
+    static constexpr bool KEEP = K;
+    using TL = Swz16Layout<kF2>;
+};
+using RowBuild = RowR16<true>;
+using RowProp = RowR16<true>;
+using RowBuildT = RowR16<false>;
 
-__device__ __forceinline__ int row_at(int k2) { return RTL::at(0, k2); }
+template <class TL> __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
 
 struct RowRep {
     int ia, ib;      // LDS positions of K and -K (r = 0 of w = 0: bins 0 and 2048)
     bool fixed;      // the two self-mirrored bins of row 0
 };
-__device__ __forceinline__ RowRep row_rep(int w, int r) {
+template <class TL> __device__ __forceinline__ RowRep row_rep(int w, int r) {
     RowRep q;
     q.fixed = false;
     if (w != 0) {
-        q.ia = row_at(r);
-        q.ib = RTL::IS + row_at(kF2 - 1 - r);
+        q.ia = row_at<TL>(r);
+        q.ib = TL::IS + row_at<TL>(kF2 - 1 - r);
     } else if (r < kF2 / 2) {
-        q.ia = row_at(r);
-        q.ib = row_at(r == 0 ? kF2 / 2 : kF2 - r);
+        q.ia = row_at<TL>(r);
+        q.ib = row_at<TL>(r == 0 ? kF2 / 2 : kF2 - r);
         q.fixed = (r == 0);
     } else {
         const int k2 = r - kF2 / 2;
-        q.ia = RTL::IS + row_at(k2);
-        q.ib = RTL::IS + row_at(kF2 - 1 - k2);
+        q.ia = TL::IS + row_at<TL>(k2);
+        q.ib = TL::IS + row_at<TL>(kF2 - 1 - k2);
     }
     return q;
 }
@@ -1700,26 +1728,14 @@ __device__ __forceinline__ int row_of(int w, int F1, int h) {
     return (w == 0) ? (h ? F1 / 2 : 0) : (h ? F1 - w : w);
 }
 
-// Pass shape of the two-row transforms for a workgroup of NTH threads (the
-// radix stays 8: NT = 512 butterflies per tile and pass).
-template <int NTH> struct RowPass;
-template <> struct RowPass<512> {
-    static constexpr int PTS = 2 * kTile;   // two butterflies per thread
-    static constexpr bool KEEP = true;      // addresses left to the compiler (256 VGPRs available)
-};
-template <> struct RowPass<1024> {
-    static constexpr int PTS = kTile;       // one butterfly per thread, tile = thread / 512
-    static constexpr bool KEEP = false;     // fresh addresses per pass: 128 VGPRs
-};
-
-// Load both rows of one [k1][n2] array and transform them; the first radix-8
-// pass reads its points (n2 = t + 512 s, coalesced) straight from global
+// Load both rows of one [k1][n2] array and transform them; the first radix-16
+// pass reads its points (n2 = (t mod 256) + 256 s, coalesced) straight from global
 // memory.  Ends with the last pass's barrier.
-template <int NTH>
+template <class RS>
 __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__restrict__ Zb, int rA, int rB,
                                                  const double2 *tw) {
     const double2 *z0 = Zb + (long long)rA * kF2, *z1 = Zb + (long long)rB * kF2;
-    fft_tile_io<kF2, false, kTile / 8, RTL, RowPass<NTH>::PTS, RowPass<NTH>::KEEP>(
+    fft_tile_io<kF2, false, RS::NT, typename RS::TL, RS::PTS, RS::KEEP>(
         T, tw, [&](int h, int n2) { return (h ? z1 : z0)[n2]; }, NoIO{});
 }
 
@@ -1727,19 +1743,21 @@ __device__ __forceinline__ void rowpair_load_fft(double2 *T, const double2 *__re
 // four-step twiddle is applied by the column inverse as it loads them); the
 // last pass writes straight to global memory, and the tiles are free for
 // writing on return.
-template <int NTH>
+template <class RS>
 __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restrict__ Vb, int rA, int rB,
                                                    const double2 *tw) {
-    fft_tile_io<kF2, true, kTile / 8, RTL, RowPass<NTH>::PTS, RowPass<NTH>::KEEP>(
+    fft_tile_io<kF2, true, RS::NT, typename RS::TL, RS::PTS, RS::KEEP>(
         T, tw, NoIO{}, [&](int h, int e, double2 v) { Vb[(long long)(h ? rB : rA) * kF2 + e] = v; });
 }
 
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
-__global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
-                                                       int nrp, int npairs, int logF,
-                                                       const double2 *tw, int twShift) {
-    constexpr int RU = kTile / kBT;   // representatives per thread
-    __shared__ double2 T[2 * RTL::IS];
+template <class RS>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
+                                                           int nrp, int npairs, int logF,
+                                                           const double2 *tw, int twShift) {
+    constexpr int NTH = RS::NTH, RU = kTile / NTH;   // representatives per thread
+    using TL = typename RS::TL;
+    __shared__ double2 T[2 * kF2];
     const int F = 1 << logF, F1 = F >> 12;
     const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
@@ -1747,19 +1765,19 @@ __global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict
     const int p = 2 * pair;
     (void)twShift;
     double2 P1[RU], P2[RU];
-    rowpair_load_fft<kBT>(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
+    rowpair_load_fft<RS>(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
-        const RowRep q = row_rep(w, t + u * kBT);
+        const RowRep q = row_rep<TL>(w, t + u * NTH);
         double2 X, Yv;
         row_split(q, T[q.ia], T[q.ib], X, Yv);
         P1[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
     }
     if (p + 1 < nrp) {  // uniform over the workgroup
-        rowpair_load_fft<kBT>(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
+        rowpair_load_fft<RS>(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const RowRep q = row_rep(w, t + u * kBT);
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
             double2 X, Yv;
             row_split(q, T[q.ia], T[q.ib], X, Yv);
             P2[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
@@ -1770,34 +1788,37 @@ __global__ __launch_bounds__(kBT) void k_rowpair_build(const double2 *__restrict
     }
     // every thread rewrites only the positions it read
 #pragma unroll
-    for (int u = 0; u < RU; ++u) row_put(T, row_rep(w, t + u * kBT), P1[u], P2[u]);
-    rowpair_ifft_store<kBT>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    for (int u = 0; u < RU; ++u) row_put(T, row_rep<TL>(w, t + u * NTH), P1[u], P2[u]);
+    rowpair_ifft_store<RS>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 }
 
 // Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
-__global__ __launch_bounds__(kPT) void k_rowpair_prop(const double2 *__restrict__ Y1, const double2 *__restrict__ Y3,
-                                                      double2 *__restrict__ V1, int nrp, int npairs, int logF,
-                                                      const double2 *tw, int twShift) {
-    constexpr int RU = kTile / kPT;
-    __shared__ double2 T[2 * RTL::IS];
+template <class RS>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(const double2 *__restrict__ Y1,
+                                                          const double2 *__restrict__ Y3, double2 *__restrict__ V1,
+                                                          int nrp, int npairs, int logF, const double2 *tw,
+                                                          int twShift) {
+    constexpr int NTH = RS::NTH, RU = kTile / NTH;
+    using TL = typename RS::TL;
+    __shared__ double2 T[2 * kF2];
     const int F = 1 << logF, F1 = F >> 12;
     const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
     (void)twShift;
     double2 G1[RU], G2[RU];
-    rowpair_load_fft<kPT>(T, Y3 + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    rowpair_load_fft<RS>(T, Y3 + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
-        const RowRep q = row_rep(w, t + u * kPT);
+        const RowRep q = row_rep<TL>(w, t + u * NTH);
         row_split(q, T[q.ia], T[q.ib], G1[u], G2[u]);
     }
     // one parent: its child pair's rows, U1 in place, inverse rows
     auto parent = [&](int p, const double2 (&Gw)[RU]) {
-        rowpair_load_fft<kPT>(T, Y1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        rowpair_load_fft<RS>(T, Y1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const RowRep q = row_rep(w, t + u * kPT);
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
             double2 L, R;
             row_split(q, T[q.ia], T[q.ib], L, R);
             if (q.fixed)
@@ -1805,7 +1826,7 @@ __global__ __launch_bounds__(kPT) void k_rowpair_prop(const double2 *__restrict_
             else
                 row_put(T, q, c_mulc(Gw[u], R), c_mulc(Gw[u], L));   // g conj(R), g conj(L)
         }
-        rowpair_ifft_store<kPT>(T, V1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        rowpair_ifft_store<RS>(T, V1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
     };
     parent(2 * pair, G1);
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
@@ -1815,29 +1836,31 @@ __global__ __launch_bounds__(kPT) void k_rowpair_prop(const double2 *__restrict_
 // U = dP_X[p] + i dP_X[p+1], dP_X = dP_L P_R + P_L dP_R, from the row-stage
 // inputs of the children's P pairs (SP, the spectrum cache) and of their dP
 // pairs (SD); both laid out [qi][nrp][F].
-__global__ __launch_bounds__(kPT) void k_rowpair_build_t(const double2 *__restrict__ SP,
+template <class RS>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_build_t(const double2 *__restrict__ SP,
                                                          const double2 *__restrict__ SD, double2 *__restrict__ V,
                                                          int nrp, int npairs, int logF, const double2 *tw,
                                                          int twShift) {
-    constexpr int RU = kTile / kPT;
-    __shared__ double2 T[2 * RTL::IS];
+    constexpr int NTH = RS::NTH, RU = kTile / NTH;
+    using TL = typename RS::TL;
+    __shared__ double2 T[2 * kF2];
     const int F = 1 << logF, F1 = F >> 12;
     const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
     (void)twShift;
     auto tangent = [&](int p, double2 (&D)[RU]) {
-        rowpair_load_fft<kPT>(T, SP + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        rowpair_load_fft<RS>(T, SP + ((long long)qi * nrp + p) * F, rA, rB, tw);
         double2 PL[RU], PR[RU];
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const RowRep q = row_rep(w, t + u * kPT);
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
             row_split(q, T[q.ia], T[q.ib], PL[u], PR[u]);
         }
-        rowpair_load_fft<kPT>(T, SD + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        rowpair_load_fft<RS>(T, SD + ((long long)qi * nrp + p) * F, rA, rB, tw);
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
-            const RowRep q = row_rep(w, t + u * kPT);
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
             double2 DL, DR;
             row_split(q, T[q.ia], T[q.ib], DL, DR);
             D[u] = q.fixed ? make_double2(DL.x * PR[u].x + PL[u].x * DR.x, DL.y * PR[u].y + PL[u].y * DR.y)
@@ -1854,8 +1877,8 @@ __global__ __launch_bounds__(kPT) void k_rowpair_build_t(const double2 *__restri
     }
     // every thread rewrites only the positions it read
 #pragma unroll
-    for (int u = 0; u < RU; ++u) row_put(T, row_rep(w, t + u * kPT), D1[u], D2[u]);
-    rowpair_ifft_store<kPT>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    for (int u = 0; u < RU; ++u) row_put(T, row_rep<TL>(w, t + u * NTH), D1[u], D2[u]);
+    rowpair_ifft_store<RS>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -1966,7 +1989,7 @@ static hipError_t build_lds_f(const LevelJob &J, const double2 *tw, int twShift,
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.nr_parent + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_build_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    k_build_lds<F><<<grid, lds_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 template <int F>
@@ -1974,7 +1997,7 @@ static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, 
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.np + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_prop_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    k_prop_lds<F><<<grid, lds_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
@@ -2114,7 +2137,8 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build<<<dim3(F1 / 2, npairs, J.nq), kBT, 0, s>>>(Y, bufB, nrp, npairs, J.logF, tw, twShift);
+    k_rowpair_build<RowBuild><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(Y, bufB, nrp, npairs, J.logF, tw,
+                                                                                  twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
     PF_BEGIN();
@@ -2139,7 +2163,8 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build_t<<<dim3(F1 / 2, npairs, J.nq), kPT, 0, s>>>(specP, specD, bufB, nrp, npairs, J.logF, tw, twShift);
+    k_rowpair_build_t<RowBuildT><<<dim3(F1 / 2, npairs, J.nq), RowBuildT::NTH, 0, s>>>(specP, specD, bufB, nrp, npairs,
+                                                                                       J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build_t", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     PF_BEGIN();
@@ -2170,8 +2195,8 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     }
     PF_BEGIN();
-    k_rowpair_prop<<<dim3(F1 / 2, npairs, J.nq), kPT, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs, J.logF, tw,
-                                                               twShift);
+    k_rowpair_prop<RowProp><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs,
+                                                                                J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     PF_BEGIN();
