@@ -49,7 +49,8 @@ def time_plan(plan, p, Q, reps):
 
 
 def k_values(n):
-    ks = sorted({1, 9, max(1, n // 10), n})
+    # about one point per decade of k (the crossover moves with k as much as with n)
+    ks = sorted({1, 9, 90, 900, max(1, n // 10), n})
     return [k for k in ks if k <= n]
 
 
@@ -133,11 +134,13 @@ def write_header(table, path, batch=None):
              "    int B;          // block size (players per leaf block)",
              "    int sb_max;     // schoolbook for levels with c_{l-1} <= sb_max", "};", "",
              "static const TuneRow kTuneTable[] = {"]
+    # bucket limits at the geometric means of neighbouring grid points (the
+    # grids are roughly log-spaced)
     for i, n in enumerate(ns):
-        n_max = 0x7fffffff if i == len(ns) - 1 else (n + ns[i + 1]) // 2
+        n_max = 0x7fffffff if i == len(ns) - 1 else math.isqrt(n * ns[i + 1])
         ks = sorted(k for nn, k in table if nn == n)
         for j, k in enumerate(ks):
-            k_max = 0x7fffffff if j == len(ks) - 1 else (k + ks[j + 1]) // 2
+            k_max = 0x7fffffff if j == len(ks) - 1 else math.isqrt(k * ks[j + 1])
             v = table[(n, k)]
             lines.append(f"    {{{n_max}, {k_max}, {v['B']}, {v['sb_max']}}},   // tuned at n={n} k={k}: "
                          f"{v['ms']:.3f} ms")

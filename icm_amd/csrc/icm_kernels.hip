@@ -1601,13 +1601,31 @@ __global__ __launch_bounds__(kCT) void k_col_fwd(Loader ld, double2 *__restrict_
     }
     fft_tile<F1, false, kCT, TL>(lds, tw, twShift);
     double2 *o = out + ((long long)qi * nArr + a) * F;
-#pragma unroll
-    for (int u = 0; u < kCP; ++u) {
-        const int e = t + u * kCT;
-        const int r = e >> LOGTC, c = e & (TC - 1);
+    if constexpr (TC <= kCT) {
+        // A thread's outputs share column n2 at rows r0 + u RS: its twiddles
+        // are w_F^(n2 r0) times powers of w_F^(n2 RS), two table lookups
+        // instead of eight (the lookups gather from scattered table lines;
+        // the product chain adds at most one rounding per step).
+        constexpr int RS = kCT / TC;
+        const int c = t & (TC - 1), r0 = t >> LOGTC;
         const int n2 = tile * TC + c;
-        const double2 w = tw_big(tw, twShift, (n2 * r) & (F - 1), logF);
-        o[r * kF2 + n2] = c_mul(lds[TL::at(c, r)], w);
+        double2 w = tw_big(tw, twShift, (n2 * r0) & (F - 1), logF);
+        const double2 ws = tw_big(tw, twShift, (n2 * RS) & (F - 1), logF);
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int r = r0 + u * RS;
+            o[r * kF2 + n2] = c_mul(lds[TL::at(c, r)], w);
+            w = c_mul(w, ws);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            const int n2 = tile * TC + c;
+            const double2 w = tw_big(tw, twShift, (n2 * r) & (F - 1), logF);
+            o[r * kF2 + n2] = c_mul(lds[TL::at(c, r)], w);
+        }
     }
 }
 
@@ -1622,13 +1640,28 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
     const int t = threadIdx.x;
     const int F = 1 << logF;
     const double2 *src = in + ((long long)qi * nArr + a) * F;
-#pragma unroll
-    for (int u = 0; u < kCP; ++u) {
-        const int e = t + u * kCT;
-        const int r = e >> LOGTC, c = e & (TC - 1);
+    // the inverse four-step twiddle w_F^(-n2 r) of the row stage (powers of
+    // one step per thread where its inputs share a column, as in k_col_fwd)
+    if constexpr (TC <= kCT) {
+        constexpr int RS = kCT / TC;
+        const int c = t & (TC - 1), r0 = t >> LOGTC;
         const int n2 = tile * TC + c;
-        // the inverse four-step twiddle w_F^(-n2 r) of the row stage
-        lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big2(tw, twShift, (n2 * r) & (F - 1), logF));
+        double2 w = tw_big2(tw, twShift, (n2 * r0) & (F - 1), logF);
+        const double2 ws = tw_big2(tw, twShift, (n2 * RS) & (F - 1), logF);
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int r = r0 + u * RS;
+            lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], w);
+            w = c_mul(w, ws);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            const int n2 = tile * TC + c;
+            lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big2(tw, twShift, (n2 * r) & (F - 1), logF));
+        }
     }
     fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
 #pragma unroll
