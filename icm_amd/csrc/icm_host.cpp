@@ -127,7 +127,7 @@ long long plane_bytes(const Geometry &g, long long *g_elems, long long *big_elem
     for (int l = 1; l <= g.L - 2; ++l) bytes += (long long)g.n_real[l] * g.c[l] * 8;
     for (int l = 0; l < g.L; ++l) ge = std::max(ge, (long long)g.n_real[l] * g.G[l]);
     for (int l = 1; l < g.L; ++l) {
-        if (g.F[l] > icm::kMaxLdsF) {
+        if (g.F[l] > icm::kMaxLdsF && !icm::col_fused(g.F[l])) {
             const long long nrp = g.n_real[l], np = (nrp + 1) / 2;
             be = std::max(be, (nrp + np) * (long long)g.F[l]);
         }
@@ -138,11 +138,16 @@ long long plane_bytes(const Geometry &g, long long *g_elems, long long *big_elem
     return bytes;
 }
 
-// Complex elements per q-plane of the spectrum cache (four-step build levels).
+// Four-step build levels with column arrays (the spectrum cache keeps them).
+static bool spec_level(const Geometry &g, int l) {
+    return l >= 1 && l <= g.L - 2 && g.F[l] > icm::kMaxLdsF && !icm::col_fused(g.F[l]);
+}
+
+// Complex elements per q-plane of the spectrum cache.
 long long spec_plane_elems(const Geometry &g) {
     long long e = 0;
     for (int l = 1; l <= g.L - 2; ++l)
-        if (g.F[l] > icm::kMaxLdsF) e += (long long)g.n_real[l] * g.F[l];
+        if (spec_level(g, l)) e += (long long)g.n_real[l] * g.F[l];
     return e;
 }
 
@@ -151,7 +156,7 @@ bool spec_alloc(IcmHipPlan *p) {
     p->spec_off.assign(g.L, -1);
     long long off = 0;
     for (int l = 1; l <= g.L - 2; ++l) {
-        if (g.F[l] > icm::kMaxLdsF) {
+        if (spec_level(g, l)) {
             p->spec_off[l] = off;
             off += (long long)p->qb * g.n_real[l] * g.F[l];
         }
@@ -384,7 +389,9 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
-                                                         p->d_spec ? p->d_spec + p->spec_off[l] : nullptr);
+                                                         (p->d_spec && p->spec_off[l] >= 0)
+                                                             ? p->d_spec + p->spec_off[l]
+                                                             : nullptr);
             if (!hip_ok(e, "build level")) return -1;
         }
         // propagate: root g = payout (same for every q-plane)
@@ -417,7 +424,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
-                                                        (p->d_spec && !J.plist && l <= L - 2)
+                                                        (p->d_spec && !J.plist && l <= L - 2 && p->spec_off[l] >= 0)
                                                             ? p->d_spec + p->spec_off[l]
                                                             : nullptr);
             if (!hip_ok(e, "propagate level")) return -1;
@@ -496,7 +503,7 @@ bool ensure_vjp(IcmHipPlan *p) {
         p->dspec_off.assign(g.L, -1);
         long long off = 0;
         for (int l = 1; l <= g.L - 2; ++l) {
-            if (g.F[l] > icm::kMaxLdsF) {
+            if (spec_level(g, l)) {
                 p->dspec_off[l] = off;
                 off += (long long)qv * g.n_real[l] * g.F[l];
             }
@@ -569,7 +576,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         return hip_ok(e, build ? "vjp build level" : "vjp propagate level");
     };
     auto spec_of = [&](int l) -> double2 * {
-        return (p->d_spec && l >= 1 && l <= L - 2 && g.F[l] > icm::kMaxLdsF) ? p->d_spec + p->spec_off[l] : nullptr;
+        return (p->d_spec && spec_level(g, l)) ? p->d_spec + p->spec_off[l] : nullptr;
     };
     auto dspec_of = [&](int l) -> double2 * {
         return (p->d_dspec && spec_of(l)) ? p->d_dspec + p->dspec_off[l] : nullptr;

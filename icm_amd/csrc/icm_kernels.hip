@@ -1709,6 +1709,7 @@ template <bool K> struct RowR16 {
 using RowBuild = RowR16<true>;
 using RowProp = RowR16<true>;
 using RowBuildT = RowR16<false>;
+using RowPropFused = RowR16<false>;   // F = 8192 propagate: its loaders hold more state (33 spills with KEEP)
 
 template <class TL> __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
 
@@ -1783,10 +1784,67 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
         T, tw, NoIO{}, [&](int h, int e, double2 v) { Vb[(long long)(h ? rB : rA) * kF2 + e] = v; });
 }
 
-// Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y.
-template <class RS>
-__global__ __launch_bounds__(RS::NTH) void k_rowpair_build(const double2 *__restrict__ Y, double2 *__restrict__ V,
-                                                           int nrp, int npairs, int logF,
+// Where the row kernels get array a's rows and put its inverse rows.
+// RowSrc / RowDst: the column kernels' [q][narr][F] arrays in HBM.
+struct RowSrc {
+    const double2 *base;
+    int narr, logF;
+    template <class RS>
+    __device__ __forceinline__ void load_fft(double2 *T, int a, int qi, int rA, int rB, const double2 *tw) const {
+        rowpair_load_fft<RS>(T, base + (((long long)qi * narr + a) << logF), rA, rB, tw);
+    }
+};
+struct RowDst {
+    double2 *base;
+    int narr, logF;
+    template <class RS>
+    __device__ __forceinline__ void ifft_store(double2 *T, int a, int qi, int rA, int rB, const double2 *tw) const {
+        rowpair_ifft_store<RS>(T, base + (((long long)qi * narr + a) << logF), rA, rB, tw);
+    }
+};
+// F = 8192 (F1 = 2): workgroup 0 of a pair owns both rows, so the column
+// stage runs here: rows k1 = 0, 1 of Y are x[n2] +- x[n2 + 4096], the second
+// times w_F^n2, straight from the coefficient loader (Col2Src), and the
+// inverse rows V0, V1 give x[n2] = V0 + V1 w_F^-n2 and x[n2 + 4096] =
+// V0 - V1 w_F^-n2 for the coefficient storer (Col2Dst, which scales by 1/F).
+// No column kernels and no column arrays for such a level.
+template <class Ld> struct Col2Src {
+    Ld ld;
+    int twShift;
+    template <class RS>
+    __device__ __forceinline__ void load_fft(double2 *T, int a, int qi, int, int, const double2 *tw) const {
+        fft_tile_io<kF2, false, RS::NT, typename RS::TL, RS::PTS, RS::KEEP>(
+            T, tw,
+            [&](int h, int n2) {
+                const double2 x0 = ld(a, qi, n2), x1 = ld(a, qi, n2 + kF2);
+                return h ? c_mul(c_sub(x0, x1), tw_big(tw, twShift, n2, 13)) : c_add(x0, x1);
+            },
+            NoIO{});
+    }
+};
+template <class St> struct Col2Dst {
+    St st;
+    int twShift;
+    template <class RS>
+    __device__ __forceinline__ void ifft_store(double2 *T, int a, int qi, int, int, const double2 *tw) const {
+        using TL = typename RS::TL;
+        fft_tile_io<kF2, true, RS::NT, TL, RS::PTS, RS::KEEP>(T, tw, NoIO{}, NoIO{});
+#pragma unroll
+        for (int u = 0; u < kF2 / RS::NTH; ++u) {
+            const int n2 = (int)threadIdx.x + u * RS::NTH;
+            const double2 v0 = T[TL::at(0, n2)];
+            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], tw_big(tw, twShift, n2, 13));
+            st(a, qi, n2, c_add(v0, v1));
+            st(a, qi, n2 + kF2, c_sub(v0, v1));
+        }
+        __syncthreads();   // the tiles are free for writing on return
+    }
+};
+
+// Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y; rows
+// of child pair (slot) p from src, inverse rows of parent pair `pair` to dst.
+template <class RS, class Src, class Dst>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_build(Src src, Dst dst, int nrp, int npairs, int logF,
                                                            const double2 *tw, int twShift) {
     constexpr int NTH = RS::NTH, RU = kTile / NTH;   // representatives per thread
     using TL = typename RS::TL;
@@ -1797,8 +1855,9 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_build(const double2 *__rest
     const int t = threadIdx.x;
     const int p = 2 * pair;
     (void)twShift;
+    (void)npairs;
     double2 P1[RU], P2[RU];
-    rowpair_load_fft<RS>(T, Y + ((long long)qi * nrp + p) * F, rA, rB, tw);
+    src.template load_fft<RS>(T, p, qi, rA, rB, tw);
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
         const RowRep q = row_rep<TL>(w, t + u * NTH);
@@ -1807,7 +1866,7 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_build(const double2 *__rest
         P1[u] = q.fixed ? make_double2(X.x * Yv.x, X.y * Yv.y) : c_mul(X, Yv);
     }
     if (p + 1 < nrp) {  // uniform over the workgroup
-        rowpair_load_fft<RS>(T, Y + ((long long)qi * nrp + p + 1) * F, rA, rB, tw);
+        src.template load_fft<RS>(T, p + 1, qi, rA, rB, tw);
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
             const RowRep q = row_rep<TL>(w, t + u * NTH);
@@ -1822,15 +1881,15 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_build(const double2 *__rest
     // every thread rewrites only the positions it read
 #pragma unroll
     for (int u = 0; u < RU; ++u) row_put(T, row_rep<TL>(w, t + u * NTH), P1[u], P2[u]);
-    rowpair_ifft_store<RS>(T, V + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    dst.template ifft_store<RS>(T, pair, qi, rA, rB, tw);
 }
 
-// Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p).
-template <class RS>
-__global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(const double2 *__restrict__ Y1,
-                                                          const double2 *__restrict__ Y3, double2 *__restrict__ V1,
-                                                          int nrp, int npairs, int logF, const double2 *tw,
-                                                          int twShift) {
+// Propagate: V1[p] rows = inverse rows of U1 = g_p conj(R_p) + i g_p conj(L_p);
+// child pairs (slot p) from srcc, g pairs (pair) from srcg, outputs (slot p)
+// to dst.
+template <class RS, class SrcC, class SrcG, class Dst>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(SrcC srcc, SrcG srcg, Dst dst, int nrp, int npairs,
+                                                          int logF, const double2 *tw, int twShift) {
     constexpr int NTH = RS::NTH, RU = kTile / NTH;
     using TL = typename RS::TL;
     __shared__ double2 T[2 * kF2];
@@ -1839,8 +1898,9 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(const double2 *__restr
     const int pair = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
     (void)twShift;
+    (void)npairs;
     double2 G1[RU], G2[RU];
-    rowpair_load_fft<RS>(T, Y3 + ((long long)qi * npairs + pair) * F, rA, rB, tw);
+    srcg.template load_fft<RS>(T, pair, qi, rA, rB, tw);
 #pragma unroll
     for (int u = 0; u < RU; ++u) {
         const RowRep q = row_rep<TL>(w, t + u * NTH);
@@ -1848,7 +1908,7 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(const double2 *__restr
     }
     // one parent: its child pair's rows, U1 in place, inverse rows
     auto parent = [&](int p, const double2 (&Gw)[RU]) {
-        rowpair_load_fft<RS>(T, Y1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        srcc.template load_fft<RS>(T, p, qi, rA, rB, tw);
 #pragma unroll
         for (int u = 0; u < RU; ++u) {
             const RowRep q = row_rep<TL>(w, t + u * NTH);
@@ -1859,7 +1919,7 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(const double2 *__restr
             else
                 row_put(T, q, c_mulc(Gw[u], R), c_mulc(Gw[u], L));   // g conj(R), g conj(L)
         }
-        rowpair_ifft_store<RS>(T, V1 + ((long long)qi * nrp + p) * F, rA, rB, tw);
+        dst.template ifft_store<RS>(T, p, qi, rA, rB, tw);
     };
     parent(2 * pair, G1);
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
@@ -2164,14 +2224,25 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    if (col_fused(J.F)) {   // column stage inside the row kernel; nothing for the spectrum cache
+        PF_BEGIN();
+        k_rowpair_build<RowBuild, Col2Src<LdChildPair>, Col2Dst<StParentPair>>
+            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<LdChildPair>{LdChildPair{J}, twShift},
+                                                            Col2Dst<StParentPair>{StParentPair{J}, twShift}, nrp,
+                                                            npairs, J.logF, tw, twShift);
+        ICM_TRY(hipGetLastError());
+        PF_END("k_rowpair_build<F=8192>", level_coef_bytes(J.child, J.nr_child, J.nq) + 8.0 * nq * nrp * J.parent.count,
+               nq * (nrp + npairs) * (rowFl + colFl));
+        return hipSuccess;
+    }
     double2 *Y = spec ? spec : bufA;
     PF_BEGIN();
     ICM_TRY(col_fwd(F1, LdChildPair{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build<RowBuild><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(Y, bufB, nrp, npairs, J.logF, tw,
-                                                                                  twShift);
+    k_rowpair_build<RowBuild, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(
+        RowSrc{Y, nrp, J.logF}, RowDst{bufB, npairs, J.logF}, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
     PF_BEGIN();
@@ -2182,7 +2253,7 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
 
 hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShift, const double2 *specP,
                                 double2 *specD, double2 *bufB, hipStream_t s, Prof *pf) {
-    if (!specP || !specD) return hipErrorInvalidValue;
+    if (!specP || !specD || col_fused(J.F)) return hipErrorInvalidValue;
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
@@ -2214,6 +2285,20 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
     const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * nq * nrp * J.gpar.count;
+    if (col_fused(J.F)) {   // column stage inside the row kernel (spec and g_ready do not apply)
+        PF_BEGIN();
+        k_rowpair_prop<RowPropFused, Col2Src<LdChildPair>, Col2Src<LdGPair>, Col2Dst<StGChildren>>
+            <<<dim3(1, npairs, J.nq), RowPropFused::NTH, 0, s>>>(Col2Src<LdChildPair>{LdChildPair{J}, twShift},
+                                                           Col2Src<LdGPair>{LdGPair{J}, twShift},
+                                                           Col2Dst<StGChildren>{StGChildren{J}, twShift}, nrp, npairs,
+                                                           J.logF, tw, twShift);
+        ICM_TRY(hipGetLastError());
+        PF_END("k_rowpair_prop<F=8192>",
+               gin + level_coef_bytes(J.child, J.nr_child, J.nq) * nrp / J.nr_parent +
+                   8.0 * nq * J.nr_child * J.gchild.count * nrp / J.nr_parent,
+               nq * (2 * nrp + npairs) * (rowFl + colFl));
+        return hipSuccess;
+    }
     if (spec && J.plist) return hipErrorInvalidValue;   // cached spectra are per parent, not per slot
     const double2 *Y1 = spec ? spec : bufA;
     if (!spec) {
@@ -2228,8 +2313,9 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     }
     PF_BEGIN();
-    k_rowpair_prop<RowProp><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(Y1, bufA + r1, bufB, nrp, npairs,
-                                                                                J.logF, tw, twShift);
+    k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(
+        RowSrc{Y1, nrp, J.logF}, RowSrc{bufA + r1, npairs, J.logF}, RowDst{bufB, nrp, J.logF}, nrp, npairs, J.logF,
+        tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     PF_BEGIN();

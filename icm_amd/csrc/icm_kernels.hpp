@@ -176,6 +176,9 @@ hipError_t launch_level_t_batch(bool build, const LevelJob *jobs, const int2 *it
                                 int twShift, hipStream_t s, Prof *pf, double bytes, double flops);
 
 constexpr int kMaxLdsF = 4096;
+// Four-step levels whose column stage (F1 = 2) runs inside the row kernels:
+// no column arrays and no spectrum-cache entry.
+__host__ __device__ inline bool col_fused(int F) { return F == 2 * kMaxLdsF; }
 
 // Schoolbook levels: direct FP64 sums for levels whose children keep few
 // coefficients.  A workgroup stages the operands of `ppw` parents in
