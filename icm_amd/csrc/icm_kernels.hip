@@ -88,9 +88,13 @@ __device__ __forceinline__ double leaf_psi(const double *P, const double *g, dou
         const double ib = 1.0 / bm, c = -a * ib;
         double qv = P[bsize] * ib;
         if (bsize - 1 < pk) eq += g[bsize - 1] * qv;
-        for (int m = bsize - 2; m >= 0; --m) {
+        // the quotient runs down from the top; only m < pk meets g (two
+        // branch-free loops instead of a test per step)
+        int m = bsize - 2;
+        for (; m >= pk; --m) qv = c * qv + P[m + 1] * ib;
+        for (; m >= 0; --m) {
             qv = c * qv + P[m + 1] * ib;
-            if (m < pk) eq += g[m] * qv;
+            eq += g[m] * qv;
         }
     } else {
         for (int m = 0; m < pk; ++m) eq += g[m] * P[m + 1];
@@ -353,30 +357,52 @@ __device__ __forceinline__ void leaf_vjp_player(const double *P, const double *D
     const double bm = 1.0 - a;
     psi = 0.0;
     h = 0.0;
+    // Each direction runs its steps in order as a few branch-free loops: the
+    // psi term only for m < pk, the W recurrence only where W exists.
+    using Y = std::true_type;
+    using N = std::false_type;
     if (a > 0.5) {
         const double ia = 1.0 / a, r = -bm * ia;
         double qv = 0.0, wv = 0.0;
-        for (int m = 0; m < bsize; ++m) {
+        auto step = [&](int m, auto psi_on, auto w_on) {
             qv = P[m] * ia + r * qv;
-            if (m < pk) psi += A[m] * qv;
+            if constexpr (decltype(psi_on)::value) psi += A[m] * qv;
             h += qv * (Bv[m] - Bv[m + 1]);
-            if (m <= bsize - 2) {
+            if constexpr (decltype(w_on)::value) {
                 wv = (D[m] - c * qv) * ia + r * wv;
                 h += wv * (A[m] - A[m + 1]);
             }
-        }
+        };
+        int m = 0;
+        for (; m < pk && m <= bsize - 2; ++m) step(m, Y{}, Y{});
+        for (; m < pk; ++m) step(m, Y{}, N{});            // m = bsize - 1 when pk = bsize
+        for (; m <= bsize - 2; ++m) step(m, N{}, Y{});
+        for (; m < bsize; ++m) step(m, N{}, N{});
     } else if (a > 1e-15) {
         const double ib = 1.0 / bm, r = -a * ib;
         double qv = 0.0, wv = 0.0;
-        for (int m = bsize - 1; m >= 0; --m) {
-            qv = (m == bsize - 1) ? P[bsize] * ib : P[m + 1] * ib + r * qv;
-            if (m < pk) psi += A[m] * qv;
+        auto step = [&](int m, auto psi_on, auto w_on) {
+            qv = P[m + 1] * ib + r * qv;
+            if constexpr (decltype(psi_on)::value) psi += A[m] * qv;
             h += qv * (Bv[m] - Bv[m + 1]);
-            if (m >= 1) {
+            if constexpr (decltype(w_on)::value) {
                 wv = (D[m] - c * qv) * ib + r * wv;     // W[m-1]
                 h += wv * (A[m - 1] - A[m]);
             }
+        };
+        int m = bsize - 1;   // the top step starts the quotient: Q[bsize-1] = P[bsize] / (1 - a)
+        qv = P[bsize] * ib;
+        if (m < pk) psi += A[m] * qv;
+        h += qv * (Bv[m] - Bv[m + 1]);
+        if (m >= 1) {
+            wv = (D[m] - c * qv) * ib + r * wv;
+            h += wv * (A[m - 1] - A[m]);
         }
+        --m;
+        for (; m >= pk && m >= 1; --m) step(m, N{}, Y{});
+        for (; m >= pk; --m) step(m, N{}, N{});            // m = 0 when pk = 0
+        for (; m >= 1; --m) step(m, Y{}, Y{});
+        for (; m >= 0; --m) step(m, Y{}, N{});
     } else {
         for (int m = 0; m < bsize; ++m) {
             const double qv = P[m + 1];
