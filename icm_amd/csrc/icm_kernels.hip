@@ -746,15 +746,15 @@ __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int 
 // ---------------------------------------------------------------------------
 // LDS-level kernels: LT threads per 4096-point tile, 4096 / LT points per
 // thread: 512 (radix-8 passes) or 256 (radix-16 passes, two workgroups and
-// 256 VGPRs per CU).  The plain build and propagate take 256 threads at
-// F = 128 and 256, where that measured faster (1.1x); at F >= 512, in the
-// fused adjoint kernels (which spill at 256 VGPRs) and in the ragged-batch
-// kernels (which switch over F) 512 threads are faster.
+// 256 VGPRs per CU).  The plain build takes 256 threads at F = 128 and 256,
+// where that measured faster (1.1-1.2x); every propagate, the build at
+// F >= 512, the fused adjoint kernels (which spill at 256 VGPRs) and the
+// ragged-batch kernels (which switch over F) run 512 threads.
 constexpr int kLT = 512;
 constexpr int kLP = kTile / kLT;
 template <int F, int LT> struct LdsTL { using type = SwzLayout<F>; };
 template <int F> struct LdsTL<F, 256> { using type = Swz16Layout<F>; };
-template <int F> constexpr int lds_threads() { return (F == 128 || F == 256) ? 256 : 512; }
+template <int F> constexpr int lds_build_threads() { return (F == 128 || F == 256) ? 256 : 512; }
 // Largest padded tile (F = 16): LDS of the kernels that switch over F.
 constexpr int kTileMaxSize = SwzLayout<16>::SIZE;
 
@@ -848,12 +848,11 @@ __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, 
 // Two workgroups per CU (64 KiB of LDS each): 128 VGPRs at 512 threads,
 // 256 at 256 threads.
 template <int F>
-__global__ __launch_bounds__(lds_threads<F>(), lds_threads<F>() == 256 ? 2 : 4) void k_build_lds(LevelJob J,
-                                                                                               const double2 *tw,
-                                                                                               int twShift) {
+__global__ __launch_bounds__(lds_build_threads<F>(), lds_build_threads<F>() == 256 ? 2 : 4) void k_build_lds(
+    LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[kTile];
-    build_lds_body<F, PlainMap, lds_threads<F>()>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y},
-                                                  tw, twShift);
+    build_lds_body<F, PlainMap, lds_build_threads<F>()>(
+        J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,16 +863,14 @@ __global__ __launch_bounds__(lds_threads<F>(), lds_threads<F>() == 256 ? 2 : 4) 
 // each parent's children share another; the two outputs of a parent share
 // the inverse (Re = gL, Im = gR).
 // ---------------------------------------------------------------------------
-// IO = true: the g and child transforms take their first pass straight from
-// global memory and the inverses store straight to the children's g (two LDS
-// round trips fewer per transform); at 128 VGPRs (512 threads, two workgroups
-// per CU) the extra live registers spill, so only the 256-thread kernels
-// (256 VGPRs) take it.
-template <int F, bool IO, class Map, int LT = kLT>
+// The g and child transforms take their first pass straight from global
+// memory and the inverses store straight to the children's g (two LDS round
+// trips fewer per transform); with per-pass LDS addresses this fits 128
+// VGPRs at 512 threads (two workgroups per CU) without spills.
+template <int F, class Map>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift) {
-    using TL = typename LdsTL<F, LT>::type;
-    constexpr int LP = kTile / LT;
+    using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGF = ilog2c(F);
     constexpr int LOGH = ilog2c(HALF);
@@ -892,20 +889,11 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         }
         return z;
     };
-    if constexpr (IO) {
-        fft_tile_io<F, false, LT, TL>(lds, tw, ldg, NoIO{});
-    } else {
+    fft_tile_io<F, false, kLT, TL>(lds, tw, ldg, NoIO{});
+    double2 G1[kLP / 2], G2[kLP / 2];
 #pragma unroll
-        for (int u = 0; u < LP; ++u) {
-            const int e = t + u * LT;
-            lds[TL::at(e >> LOGF, e & (F - 1))] = ldg(e >> LOGF, e & (F - 1));
-        }
-        fft_tile<F, false, LT, TL>(lds, tw, twShift);
-    }
-    double2 G1[LP / 2], G2[LP / 2];
-#pragma unroll
-    for (int u = 0; u < LP / 2; ++u) {
-        const int ps = t + u * LT;
+    for (int u = 0; u < kLP / 2; ++u) {
+        const int ps = t + u * kLT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) {
@@ -916,7 +904,6 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             G2[u] = split_y(za, zb);
         }
     }
-    if constexpr (!IO) __syncthreads();
     const double invF = 1.0 / (double)F;
     const int gcs = J.gchild.stride, gcc = J.gchild.count;
 #pragma unroll
@@ -926,19 +913,10 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             M.at(item, qi, pr);
             return load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
         };
-        if constexpr (IO) {
-            fft_tile_io<F, false, LT, TL>(lds, tw, ldc, NoIO{});
-        } else {
+        fft_tile_io<F, false, kLT, TL>(lds, tw, ldc, NoIO{});
 #pragma unroll
-            for (int u = 0; u < LP; ++u) {
-                const int e = t + u * LT;
-                lds[TL::at(e >> LOGF, e & (F - 1))] = ldc(e >> LOGF, e & (F - 1));
-            }
-            fft_tile<F, false, LT, TL>(lds, tw, twShift);
-        }
-#pragma unroll
-        for (int u = 0; u < LP / 2; ++u) {
-            const int ps = t + u * LT;
+        for (int u = 0; u < kLP / 2; ++u) {
+            const int ps = t + u * kLT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
             const double2 g = which == 0 ? G1[u] : G2[u];
             const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
@@ -968,29 +946,14 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
                 if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * gcs + m], vr, J.acc_out);
             }
         };
-        if constexpr (IO) {
-            fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, st);
-        } else {
-            fft_tile<F, true, LT, TL>(lds, tw, twShift);
-#pragma unroll
-            for (int u = 0; u < LP; ++u) {
-                const int e = t + u * LT;
-                st(e >> LOGF, e & (F - 1), lds[TL::at(e >> LOGF, e & (F - 1))]);
-            }
-            __syncthreads();
-        }
+        fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
     }
 }
 
-// 256 threads: registers for the IO passes (first pass from global memory,
-// last pass to the children's g).
 template <int F>
-__global__ __launch_bounds__(lds_threads<F>(), lds_threads<F>() == 256 ? 2 : 4) void k_prop_lds(LevelJob J,
-                                                                                              const double2 *tw,
-                                                                                              int twShift) {
+__global__ __launch_bounds__(kLT, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[kTile];
-    prop_lds_body<F, lds_threads<F>() == 256, PlainMap, lds_threads<F>()>(
-        J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+    prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 // ---------------------------------------------------------------------------
@@ -1197,7 +1160,7 @@ __global__ __launch_bounds__(512) void k_prop_lds_batch(const LevelJob *__restri
     const LevelJob J = jobs[it.x];
     const int npair = (J.np + 1) >> 1;
     const FlatMap M{it.y, npair, J.nq * npair};
-#define ICM_PROP_F(FF) prop_lds_body<FF, true>(J, lds, M, tw, twShift)
+#define ICM_PROP_F(FF) prop_lds_body<FF>(J, lds, M, tw, twShift)
     ICM_F_CASES(ICM_PROP_F)
 #undef ICM_PROP_F
 }
@@ -2108,7 +2071,7 @@ static hipError_t build_lds_f(const LevelJob &J, const double2 *tw, int twShift,
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.nr_parent + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_build_lds<F><<<grid, lds_threads<F>(), 0, s>>>(J, tw, twShift);
+    k_build_lds<F><<<grid, lds_build_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 template <int F>
@@ -2116,7 +2079,7 @@ static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, 
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.np + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_prop_lds<F><<<grid, lds_threads<F>(), 0, s>>>(J, tw, twShift);
+    k_prop_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
