@@ -1797,15 +1797,70 @@ struct RowDst {
 // inverse rows V0, V1 give x[n2] = V0 + V1 w_F^-n2 and x[n2 + 4096] =
 // V0 - V1 w_F^-n2 for the coefficient storer (Col2Dst, which scales by 1/F).
 // No column kernels and no column arrays for such a level.
-template <class Ld> struct Col2Src {
-    Ld ld;
+// Two coefficient rows (x = Re, y = Im) of one transform, resolved once per
+// transform: the loaders' parent lookups and row addresses stay out of the
+// per-point code.  Point m: (x[m], y[m]) below the counts, 0 past them; with
+// no y row, y[0] = `one` (a missing right child as the identity, or 0).
+struct RowPair {
+    const double *x, *y;
+    int cx, cy;
+    double one;
+    __device__ __forceinline__ double2 at(int m) const {
+        double2 z = make_double2(0.0, 0.0);
+        if (m < cx) z.x = x[m];
+        if (m < cy) z.y = y[m];
+        if (m == 0 && one != 0.0) z.y = one;   // only without a y row
+        return z;
+    }
+};
+// Child pair of slot a (load_child_pair's values).
+struct ChildRows {
+    LevelJob J;
+    __device__ __forceinline__ RowPair rows(int a, int qi) const {
+        RowPair r{nullptr, nullptr, 0, 0, 0.0};
+        const int p = slot_parent(J, a);
+        if (p < 0 || p >= J.nr_parent) return r;
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        r.x = J.child.base + (long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride;
+        r.cx = J.child.count;
+        if (has_r) {
+            const CoefView &R = J.child_r.base ? J.child_r : J.child;
+            r.y = R.base + (long long)qi * R.plane + (long long)(2 * p + 1) * R.stride;
+            r.cy = J.child.count;
+        } else if (!J.r_missing_zero) {
+            r.one = 1.0;
+        }
+        return r;
+    }
+};
+// g pair `a` (slots 2a, 2a + 1; LdGPair's values).
+struct GRows {
+    LevelJob J;
+    __device__ __forceinline__ RowPair rows(int a, int qi) const {
+        RowPair r{nullptr, nullptr, 0, 0, 0.0};
+        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+        const int pa = slot_parent(J, 2 * a), pb = slot_parent(J, 2 * a + 1);
+        if (pa >= 0) {
+            r.x = gb + (long long)pa * J.gpar.stride;
+            r.cx = J.gpar.count;
+        }
+        if (pb >= 0) {
+            r.y = gb + (long long)pb * J.gpar.stride;
+            r.cy = J.gpar.count;
+        }
+        return r;
+    }
+};
+template <class Rows> struct Col2Src {
+    Rows src;
     int twShift;
     template <class RS>
     __device__ __forceinline__ void load_fft(double2 *T, int a, int qi, int, int, const double2 *tw) const {
+        const RowPair r = src.rows(a, qi);
         fft_tile_io<kF2, false, RS::NT, typename RS::TL, RS::PTS, RS::KEEP>(
             T, tw,
             [&](int h, int n2) {
-                const double2 x0 = ld(a, qi, n2), x1 = ld(a, qi, n2 + kF2);
+                const double2 x0 = r.at(n2), x1 = r.at(n2 + kF2);
                 return h ? c_mul(c_sub(x0, x1), tw_big(tw, twShift, n2, 13)) : c_add(x0, x1);
             },
             NoIO{});
@@ -2215,8 +2270,8 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
     if (col_fused(J.F)) {   // column stage inside the row kernel; nothing for the spectrum cache
         PF_BEGIN();
-        k_rowpair_build<RowBuild, Col2Src<LdChildPair>, Col2Dst<StParentPair>>
-            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<LdChildPair>{LdChildPair{J}, twShift},
+        k_rowpair_build<RowBuild, Col2Src<ChildRows>, Col2Dst<StParentPair>>
+            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
                                                             Col2Dst<StParentPair>{StParentPair{J}, twShift}, nrp,
                                                             npairs, J.logF, tw, twShift);
         ICM_TRY(hipGetLastError());
@@ -2276,9 +2331,9 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * nq * nrp * J.gpar.count;
     if (col_fused(J.F)) {   // column stage inside the row kernel (spec and g_ready do not apply)
         PF_BEGIN();
-        k_rowpair_prop<RowPropFused, Col2Src<LdChildPair>, Col2Src<LdGPair>, Col2Dst<StGChildren>>
-            <<<dim3(1, npairs, J.nq), RowPropFused::NTH, 0, s>>>(Col2Src<LdChildPair>{LdChildPair{J}, twShift},
-                                                           Col2Src<LdGPair>{LdGPair{J}, twShift},
+        k_rowpair_prop<RowPropFused, Col2Src<ChildRows>, Col2Src<GRows>, Col2Dst<StGChildren>>
+            <<<dim3(1, npairs, J.nq), RowPropFused::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
+                                                                Col2Src<GRows>{GRows{J}, twShift},
                                                            Col2Dst<StGChildren>{StGChildren{J}, twShift}, nrp, npairs,
                                                            J.logF, tw, twShift);
         ICM_TRY(hipGetLastError());
