@@ -1481,44 +1481,87 @@ __global__ __launch_bounds__(512) void k_prop_lds_t_batch(const LevelJob *__rest
 // ---------------------------------------------------------------------------
 constexpr int kF2 = kTile;
 
-struct LdChildPair {      // array a = slot (parent slot_parent(a))
-    LevelJob J;
-    __device__ double2 operator()(int a, int qi, int n) const {
-        return load_child_pair(J, qi, slot_parent(J, a), n);
-    }
-};
-struct LdGPair {          // array a = pair of slots
-    LevelJob J;
-    __device__ double2 operator()(int a, int qi, int n) const {
-        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
-        const int pa = slot_parent(J, 2 * a), pb = slot_parent(J, 2 * a + 1);
+// Two coefficient rows (x = Re, y = Im) of one transform, resolved once per
+// transform: the loaders' parent lookups and row addresses stay out of the
+// per-point code.  Point m: (x[m], y[m]) below the counts, 0 past them; with
+// no y row, y[0] = `one` (a missing right child as the identity, or 0).
+struct RowPair {
+    const double *x, *y;
+    int cx, cy;
+    double one;
+    __device__ __forceinline__ double2 at(int m) const {
         double2 z = make_double2(0.0, 0.0);
-        if (n < J.gpar.count) {
-            if (pa >= 0) z.x = gb[(long long)pa * J.gpar.stride + n];
-            if (pb >= 0) z.y = gb[(long long)pb * J.gpar.stride + n];
-        }
+        if (m < cx) z.x = x[m];
+        if (m < cy) z.y = y[m];
+        if (m == 0 && one != 0.0) z.y = one;   // only without a y row
         return z;
     }
 };
+// Child pair of slot a: left child (x) and right child (y) of its parent, as
+// load_child_pair gives them.
+struct ChildRows {
+    LevelJob J;
+    __device__ __forceinline__ RowPair rows(int a, int qi) const {
+        RowPair r{nullptr, nullptr, 0, 0, 0.0};
+        const int p = slot_parent(J, a);
+        if (p < 0 || p >= J.nr_parent) return r;
+        const bool has_r = (2 * p + 1) < J.nr_child;
+        r.x = J.child.base + (long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride;
+        r.cx = J.child.count;
+        if (has_r) {
+            const CoefView &R = J.child_r.base ? J.child_r : J.child;
+            r.y = R.base + (long long)qi * R.plane + (long long)(2 * p + 1) * R.stride;
+            r.cy = J.child.count;
+        } else if (!J.r_missing_zero) {
+            r.one = 1.0;
+        }
+        return r;
+    }
+};
+// g pair `a`: slots 2a (x) and 2a + 1 (y) of the parents' g.
+struct GRows {
+    LevelJob J;
+    __device__ __forceinline__ RowPair rows(int a, int qi) const {
+        RowPair r{nullptr, nullptr, 0, 0, 0.0};
+        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+        const int pa = slot_parent(J, 2 * a), pb = slot_parent(J, 2 * a + 1);
+        if (pa >= 0) {
+            r.x = gb + (long long)pa * J.gpar.stride;
+            r.cx = J.gpar.count;
+        }
+        if (pb >= 0) {
+            r.y = gb + (long long)pb * J.gpar.stride;
+            r.cy = J.gpar.count;
+        }
+        return r;
+    }
+};
+// Storers of the inverse column stage.  rows(a, qi) resolves array a's
+// output rows once (uniform over a workgroup); store(o, qi, n, v) writes
+// coefficient n of them from v = (x, y) before the 1/F scale.
+struct OutRows {
+    double *x, *y;   // null: no such node
+    int p;           // first node of the array
+};
 struct StParentPair {     // array a = pair of parents (build)
     LevelJob J;
-    __device__ void operator()(int a, int qi, int n, double2 v) const {
-        const double invF = 1.0 / (double)J.F;
+    __device__ OutRows rows(int a, int qi) const {
         double *pb = J.parent.base + (long long)qi * J.parent.plane;
         const int p = 2 * a;
-        const int ps = J.parent.stride, pc = J.parent.count;
+        return OutRows{p < J.nr_parent ? pb + (long long)p * J.parent.stride : nullptr,
+                       p + 1 < J.nr_parent ? pb + (long long)(p + 1) * J.parent.stride : nullptr, p};
+    }
+    __device__ void store(const OutRows &o, int qi, int n, double2 v) const {
+        const double invF = 1.0 / (double)J.F;
+        const int pc = J.parent.count;
         if (n < pc) {
-            if (p < J.nr_parent)
-                put(&pb[(long long)p * ps + n], (n == 0) ? child_prod_at(J, qi, p, 0) : v.x * invF, J.acc_out);
-            if (p + 1 < J.nr_parent)
-                put(&pb[(long long)(p + 1) * ps + n], (n == 0) ? child_prod_at(J, qi, p + 1, 0) : v.y * invF,
-                    J.acc_out);
+            if (o.x) put(&o.x[n], (n == 0) ? child_prod_at(J, qi, o.p, 0) : v.x * invF, J.acc_out);
+            if (o.y) put(&o.y[n], (n == 0) ? child_prod_at(J, qi, o.p + 1, 0) : v.y * invF, J.acc_out);
         }
         if (n == 0 && pc > J.F) {
             const int top = J.child.count - 1;
-            if (p < J.nr_parent) put(&pb[(long long)p * ps + J.F], child_prod_at(J, qi, p, top), J.acc_out);
-            if (p + 1 < J.nr_parent)
-                put(&pb[(long long)(p + 1) * ps + J.F], child_prod_at(J, qi, p + 1, top), J.acc_out);
+            if (o.x) put(&o.x[J.F], child_prod_at(J, qi, o.p, top), J.acc_out);
+            if (o.y) put(&o.y[J.F], child_prod_at(J, qi, o.p + 1, top), J.acc_out);
         }
     }
 };
@@ -1531,38 +1574,44 @@ struct StParentTangent {  // array a = pair of parents (adjoint tangent build): 
         const double dr = has_r ? view_at(J.tchild, qi, 2 * p + 1, m) : 0.0;
         return dl * pr + pl * dr;
     }
-    __device__ void operator()(int a, int qi, int n, double2 v) const {
-        const double invF = 1.0 / (double)J.F;
+    __device__ OutRows rows(int a, int qi) const {
         double *pb = J.parent.base + (long long)qi * J.parent.plane;
         const int p = 2 * a;
-        const int ps = J.parent.stride, pc = J.parent.count;
+        return OutRows{p < J.nr_parent ? pb + (long long)p * J.parent.stride : nullptr,
+                       p + 1 < J.nr_parent ? pb + (long long)(p + 1) * J.parent.stride : nullptr, p};
+    }
+    __device__ void store(const OutRows &o, int qi, int n, double2 v) const {
+        const double invF = 1.0 / (double)J.F;
+        const int pc = J.parent.count;
         if (n < pc) {
-            if (p < J.nr_parent) put(&pb[(long long)p * ps + n], (n == 0) ? exact(qi, p, 0) : v.x * invF, J.acc_out);
-            if (p + 1 < J.nr_parent)
-                put(&pb[(long long)(p + 1) * ps + n], (n == 0) ? exact(qi, p + 1, 0) : v.y * invF, J.acc_out);
+            if (o.x) put(&o.x[n], (n == 0) ? exact(qi, o.p, 0) : v.x * invF, J.acc_out);
+            if (o.y) put(&o.y[n], (n == 0) ? exact(qi, o.p + 1, 0) : v.y * invF, J.acc_out);
         }
         if (n == 0 && pc > J.F) {
             const int top = J.child.count - 1;
-            if (p < J.nr_parent) put(&pb[(long long)p * ps + J.F], exact(qi, p, top), J.acc_out);
-            if (p + 1 < J.nr_parent) put(&pb[(long long)(p + 1) * ps + J.F], exact(qi, p + 1, top), J.acc_out);
+            if (o.x) put(&o.x[J.F], exact(qi, o.p, top), J.acc_out);
+            if (o.y) put(&o.y[J.F], exact(qi, o.p + 1, top), J.acc_out);
         }
     }
 };
-struct StGChildren {      // array a = slot (propagate)
+struct StGChildren {      // array a = slot (propagate): g of the slot's parent's two children
     LevelJob J;
-    __device__ void operator()(int a, int qi, int n, double2 v) const {
-        if (n >= J.gchild.count) return;
-        const double invF = 1.0 / (double)J.F;
-        double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
+    __device__ OutRows rows(int a, int qi) const {
         const int p = slot_parent(J, a);
-        if (p < 0) return;
+        if (p < 0) return OutRows{nullptr, nullptr, -1};
+        double *l = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+        return OutRows{l, (2 * p + 1 < J.nr_child) ? l + J.gchild.stride : nullptr, p};
+    }
+    __device__ void store(const OutRows &o, int qi, int n, double2 v) const {
+        if (!o.x || n >= J.gchild.count) return;
+        const double invF = 1.0 / (double)J.F;
         double vl = v.x * invF, vr = v.y * invF;
         if (J.gfix && n == J.child.count - 1) {
-            vl += prop_wrap_term(J, qi, p, 0);
-            vr += prop_wrap_term(J, qi, p, 1);
+            vl += prop_wrap_term(J, qi, o.p, 0);
+            vr += prop_wrap_term(J, qi, o.p, 1);
         }
-        put(&gcb[(long long)(2 * p) * J.gchild.stride + n], vl, J.acc_out);
-        if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * J.gchild.stride + n], vr, J.acc_out);
+        put(&o.x[n], vl, J.acc_out);
+        if (o.y) put(&o.y[n], vr, J.acc_out);
     }
 };
 
@@ -1571,8 +1620,8 @@ struct StGChildren {      // array a = slot (propagate)
 constexpr int kCT = 512;
 constexpr int kCP = kTile / kCT;
 
-template <int F1, class Loader>
-__global__ __launch_bounds__(kCT) void k_col_fwd(Loader ld, double2 *__restrict__ out, int nArr, int logF,
+template <int F1, class Rows>
+__global__ __launch_bounds__(kCT) void k_col_fwd(Rows src, double2 *__restrict__ out, int nArr, int logF,
                                                  const double2 *tw, int twShift) {
     using TL = ColLayout<F1>;
     __shared__ double2 lds[TL::SIZE];
@@ -1581,12 +1630,13 @@ __global__ __launch_bounds__(kCT) void k_col_fwd(Loader ld, double2 *__restrict_
     const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z;
     const int t = threadIdx.x;
     const int F = 1 << logF;
+    const RowPair rp = src.rows(a, qi);   // uniform over the workgroup
 #pragma unroll
     for (int u = 0; u < kCP; ++u) {
         const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
         const int n = r * kF2 + tile * TC + c;
-        lds[TL::at(c, r)] = ld(a, qi, n);
+        lds[TL::at(c, r)] = rp.at(n);
     }
     fft_tile<F1, false, kCT, TL>(lds, tw, twShift);
     double2 *o = out + ((long long)qi * nArr + a) * F;
@@ -1653,11 +1703,12 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
         }
     }
     fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
+    const OutRows o = st.rows(a, qi);   // uniform over the workgroup
 #pragma unroll
     for (int u = 0; u < kCP; ++u) {
         const int e = t + u * kCT;
         const int r = e >> LOGTC, c = e & (TC - 1);
-        st(a, qi, r * kF2 + tile * TC + c, lds[TL::at(c, r)]);
+        st.store(o, qi, r * kF2 + tile * TC + c, lds[TL::at(c, r)]);
     }
 }
 
@@ -1797,60 +1848,6 @@ struct RowDst {
 // inverse rows V0, V1 give x[n2] = V0 + V1 w_F^-n2 and x[n2 + 4096] =
 // V0 - V1 w_F^-n2 for the coefficient storer (Col2Dst, which scales by 1/F).
 // No column kernels and no column arrays for such a level.
-// Two coefficient rows (x = Re, y = Im) of one transform, resolved once per
-// transform: the loaders' parent lookups and row addresses stay out of the
-// per-point code.  Point m: (x[m], y[m]) below the counts, 0 past them; with
-// no y row, y[0] = `one` (a missing right child as the identity, or 0).
-struct RowPair {
-    const double *x, *y;
-    int cx, cy;
-    double one;
-    __device__ __forceinline__ double2 at(int m) const {
-        double2 z = make_double2(0.0, 0.0);
-        if (m < cx) z.x = x[m];
-        if (m < cy) z.y = y[m];
-        if (m == 0 && one != 0.0) z.y = one;   // only without a y row
-        return z;
-    }
-};
-// Child pair of slot a (load_child_pair's values).
-struct ChildRows {
-    LevelJob J;
-    __device__ __forceinline__ RowPair rows(int a, int qi) const {
-        RowPair r{nullptr, nullptr, 0, 0, 0.0};
-        const int p = slot_parent(J, a);
-        if (p < 0 || p >= J.nr_parent) return r;
-        const bool has_r = (2 * p + 1) < J.nr_child;
-        r.x = J.child.base + (long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride;
-        r.cx = J.child.count;
-        if (has_r) {
-            const CoefView &R = J.child_r.base ? J.child_r : J.child;
-            r.y = R.base + (long long)qi * R.plane + (long long)(2 * p + 1) * R.stride;
-            r.cy = J.child.count;
-        } else if (!J.r_missing_zero) {
-            r.one = 1.0;
-        }
-        return r;
-    }
-};
-// g pair `a` (slots 2a, 2a + 1; LdGPair's values).
-struct GRows {
-    LevelJob J;
-    __device__ __forceinline__ RowPair rows(int a, int qi) const {
-        RowPair r{nullptr, nullptr, 0, 0, 0.0};
-        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
-        const int pa = slot_parent(J, 2 * a), pb = slot_parent(J, 2 * a + 1);
-        if (pa >= 0) {
-            r.x = gb + (long long)pa * J.gpar.stride;
-            r.cx = J.gpar.count;
-        }
-        if (pb >= 0) {
-            r.y = gb + (long long)pb * J.gpar.stride;
-            r.cy = J.gpar.count;
-        }
-        return r;
-    }
-};
 template <class Rows> struct Col2Src {
     Rows src;
     int twShift;
@@ -1873,13 +1870,14 @@ template <class St> struct Col2Dst {
     __device__ __forceinline__ void ifft_store(double2 *T, int a, int qi, int, int, const double2 *tw) const {
         using TL = typename RS::TL;
         fft_tile_io<kF2, true, RS::NT, TL, RS::PTS, RS::KEEP>(T, tw, NoIO{}, NoIO{});
+        const OutRows o = st.rows(a, qi);
 #pragma unroll
         for (int u = 0; u < kF2 / RS::NTH; ++u) {
             const int n2 = (int)threadIdx.x + u * RS::NTH;
             const double2 v0 = T[TL::at(0, n2)];
             const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], tw_big(tw, twShift, n2, 13));
-            st(a, qi, n2, c_add(v0, v1));
-            st(a, qi, n2 + kF2, c_sub(v0, v1));
+            st.store(o, qi, n2, c_add(v0, v1));
+            st.store(o, qi, n2 + kF2, c_sub(v0, v1));
         }
         __syncthreads();   // the tiles are free for writing on return
     }
@@ -2234,11 +2232,11 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
     return e;
 }
 
-template <class Loader>
-static hipError_t col_fwd(int F1, Loader ld, double2 *out, int nArr, int nq, int logF, const double2 *tw,
+template <class Rows>
+static hipError_t col_fwd(int F1, Rows src, double2 *out, int nArr, int nq, int logF, const double2 *tw,
                           int twShift, hipStream_t s) {
     dim3 grid(F1, nArr, nq);  // tiles = F2 / TC = F1
-#define ICM_CF(V) case V: k_col_fwd<V, Loader><<<grid, kCT, 0, s>>>(ld, out, nArr, logF, tw, twShift); break;
+#define ICM_CF(V) case V: k_col_fwd<V, Rows><<<grid, kCT, 0, s>>>(src, out, nArr, logF, tw, twShift); break;
     switch (F1) {
         ICM_CF(2) ICM_CF(4) ICM_CF(8) ICM_CF(16) ICM_CF(32) ICM_CF(64) ICM_CF(128) ICM_CF(256) ICM_CF(512)
         ICM_CF(1024) ICM_CF(2048) ICM_CF(4096)
@@ -2281,8 +2279,8 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     }
     double2 *Y = spec ? spec : bufA;
     PF_BEGIN();
-    ICM_TRY(col_fwd(F1, LdChildPair{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
+    ICM_TRY(col_fwd(F1, ChildRows{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,ChildRows>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
     k_rowpair_build<RowBuild, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(
@@ -2307,8 +2305,8 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
     K.child_r = CoefView{};
     K.r_missing_zero = 1;
     PF_BEGIN();
-    ICM_TRY(col_fwd(F1, LdChildPair{K}, specD, nrp, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,LdChildPair>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
+    ICM_TRY(col_fwd(F1, ChildRows{K}, specD, nrp, J.nq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,ChildRows>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
     k_rowpair_build_t<RowBuildT><<<dim3(F1 / 2, npairs, J.nq), RowBuildT::NTH, 0, s>>>(specP, specD, bufB, nrp, npairs,
@@ -2347,14 +2345,14 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     const double2 *Y1 = spec ? spec : bufA;
     if (!spec) {
         PF_BEGIN();
-        ICM_TRY(col_fwd(F1, LdChildPair{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
-        PF_END(kname("k_col_fwd<%d,LdChildPair>", F1),
+        ICM_TRY(col_fwd(F1, ChildRows{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,ChildRows>", F1),
                level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
     }
     if (!g_ready) {
         PF_BEGIN();
-        ICM_TRY(col_fwd(F1, LdGPair{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
-        PF_END(kname("k_col_fwd<%d,LdGPair>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+        ICM_TRY(col_fwd(F1, GRows{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     }
     PF_BEGIN();
     k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(
