@@ -10,46 +10,46 @@ struct TuneRow {
 };
 
 static const TuneRow kTuneTable[] = {
-    {2048, 3, 32, 512},   // tuned at n=1024 k=1: 0.172 ms
-    {2048, 28, 32, 257},   // tuned at n=1024 k=9: 0.180 ms
-    {2048, 95, 64, 129},   // tuned at n=1024 k=90: 0.216 ms
-    {2048, 302, 64, 512},   // tuned at n=1024 k=102: 0.221 ms
-    {2048, 960, 64, 257},   // tuned at n=1024 k=900: 0.262 ms
-    {2048, 2147483647, 32, 257},   // tuned at n=1024 k=1024: 0.267 ms
-    {8192, 3, 32, 129},   // tuned at n=4096 k=1: 0.174 ms
-    {8192, 28, 64, 512},   // tuned at n=4096 k=9: 0.206 ms
-    {8192, 191, 64, 129},   // tuned at n=4096 k=90: 0.350 ms
-    {8192, 606, 64, 65},   // tuned at n=4096 k=409: 0.471 ms
-    {8192, 1920, 64, 33},   // tuned at n=4096 k=900: 0.487 ms
-    {8192, 2147483647, 64, 65},   // tuned at n=4096 k=4096: 0.501 ms
-    {32768, 3, 32, 512},   // tuned at n=16384 k=1: 0.198 ms
-    {32768, 28, 64, 129},   // tuned at n=16384 k=9: 0.254 ms
-    {32768, 284, 64, 129},   // tuned at n=16384 k=90: 0.517 ms
-    {32768, 1214, 64, 33},   // tuned at n=16384 k=900: 0.740 ms
-    {32768, 5180, 64, 33},   // tuned at n=16384 k=1638: 0.758 ms
-    {32768, 2147483647, 64, 33},   // tuned at n=16384 k=16384: 1.000 ms
-    {131072, 3, 32, 512},   // tuned at n=65536 k=1: 0.356 ms
-    {131072, 28, 64, 257},   // tuned at n=65536 k=9: 0.477 ms
-    {131072, 284, 64, 512},   // tuned at n=65536 k=90: 1.368 ms
-    {131072, 2428, 64, 33},   // tuned at n=65536 k=900: 2.382 ms
-    {131072, 20723, 64, 33},   // tuned at n=65536 k=6553: 4.001 ms
-    {131072, 2147483647, 64, 17},   // tuned at n=65536 k=65536: 4.725 ms
-    {524288, 3, 32, 129},   // tuned at n=262144 k=1: 1.180 ms
-    {524288, 28, 64, 512},   // tuned at n=262144 k=9: 1.595 ms
-    {524288, 284, 64, 17},   // tuned at n=262144 k=90: 4.708 ms
-    {524288, 4857, 64, 17},   // tuned at n=262144 k=900: 8.909 ms
-    {524288, 82896, 64, 33},   // tuned at n=262144 k=26214: 19.213 ms
-    {524288, 2147483647, 64, 0},   // tuned at n=262144 k=262144: 21.897 ms
-    {2147483647, 3, 32, 129},   // tuned at n=1048576 k=1: 4.517 ms
-    {2147483647, 28, 64, 129},   // tuned at n=1048576 k=9: 6.190 ms
-    {2147483647, 284, 64, 33},   // tuned at n=1048576 k=90: 17.961 ms
-    {2147483647, 9714, 64, 17},   // tuned at n=1048576 k=900: 34.784 ms
-    {2147483647, 331587, 64, 33},   // tuned at n=1048576 k=104857: 91.322 ms
-    {2147483647, 2147483647, 64, 33},   // tuned at n=1048576 k=1048576: 103.190 ms
+    {2048, 3, 32, 512},   // tuned at n=1024 k=1: 0.167 ms
+    {2048, 28, 32, 512},   // tuned at n=1024 k=9: 0.177 ms
+    {2048, 95, 32, 257},   // tuned at n=1024 k=90: 0.211 ms
+    {2048, 302, 64, 257},   // tuned at n=1024 k=102: 0.213 ms
+    {2048, 960, 64, 257},   // tuned at n=1024 k=900: 0.256 ms
+    {2048, 2147483647, 64, 257},   // tuned at n=1024 k=1024: 0.254 ms
+    {8192, 3, 32, 65},   // tuned at n=4096 k=1: 0.169 ms
+    {8192, 28, 64, 257},   // tuned at n=4096 k=9: 0.209 ms
+    {8192, 191, 64, 129},   // tuned at n=4096 k=90: 0.340 ms
+    {8192, 606, 64, 33},   // tuned at n=4096 k=409: 0.455 ms
+    {8192, 1920, 64, 65},   // tuned at n=4096 k=900: 0.463 ms
+    {8192, 2147483647, 64, 17},   // tuned at n=4096 k=4096: 0.473 ms
+    {32768, 3, 32, 512},   // tuned at n=16384 k=1: 0.190 ms
+    {32768, 28, 64, 257},   // tuned at n=16384 k=9: 0.241 ms
+    {32768, 284, 64, 512},   // tuned at n=16384 k=90: 0.502 ms
+    {32768, 1214, 64, 33},   // tuned at n=16384 k=900: 0.687 ms
+    {32768, 5180, 64, 33},   // tuned at n=16384 k=1638: 0.737 ms
+    {32768, 2147483647, 64, 33},   // tuned at n=16384 k=16384: 0.908 ms
+    {131072, 3, 32, 512},   // tuned at n=65536 k=1: 0.345 ms
+    {131072, 28, 64, 129},   // tuned at n=65536 k=9: 0.469 ms
+    {131072, 284, 64, 33},   // tuned at n=65536 k=90: 1.313 ms
+    {131072, 2428, 64, 17},   // tuned at n=65536 k=900: 2.229 ms
+    {131072, 20723, 64, 17},   // tuned at n=65536 k=6553: 3.713 ms
+    {131072, 2147483647, 64, 17},   // tuned at n=65536 k=65536: 4.433 ms
+    {524288, 3, 32, 257},   // tuned at n=262144 k=1: 1.175 ms
+    {524288, 28, 64, 257},   // tuned at n=262144 k=9: 1.599 ms
+    {524288, 284, 64, 33},   // tuned at n=262144 k=90: 4.511 ms
+    {524288, 4857, 64, 33},   // tuned at n=262144 k=900: 8.342 ms
+    {524288, 82896, 64, 33},   // tuned at n=262144 k=26214: 18.199 ms
+    {524288, 2147483647, 64, 0},   // tuned at n=262144 k=262144: 20.805 ms
+    {2147483647, 3, 32, 257},   // tuned at n=1048576 k=1: 4.468 ms
+    {2147483647, 28, 64, 65},   // tuned at n=1048576 k=9: 6.043 ms
+    {2147483647, 284, 64, 0},   // tuned at n=1048576 k=90: 17.094 ms
+    {2147483647, 9714, 64, 33},   // tuned at n=1048576 k=900: 32.447 ms
+    {2147483647, 331587, 64, 0},   // tuned at n=1048576 k=104857: 87.216 ms
+    {2147483647, 2147483647, 64, 33},   // tuned at n=1048576 k=1048576: 98.883 ms
 };
 
 // Ragged batches (one setting per batch), tuned on the 4096-instance ragged mix.
 struct BatchTune {
     int B, sb_max;
 };
-static const BatchTune kBatchTune = {64, 0};   // 35.984 ms
+static const BatchTune kBatchTune = {64, 0};   // 35.522 ms
