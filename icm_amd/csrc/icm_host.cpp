@@ -251,10 +251,11 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     maxF = std::max(maxF, icm::kMaxLdsF);
     p->twShift = ilog2i(maxF);
 
-    // leaf-extract q-chunks: enough workgroups to fill the chip
+    // leaf-extract q-chunks: enough workgroups to fill the chip several times
+    // over (about 16k; 4k left a part-filled last round at n = 2^20)
     const int bpw = 256 / B;
     const int wg_blocks = (g.nb + bpw - 1) / bpw;
-    int nchunk = std::max(1, 4096 / std::max(1, wg_blocks));
+    int nchunk = std::max(1, 16384 / std::max(1, wg_blocks));
     p->nchunk = std::min(nchunk, qb);
 
     bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
