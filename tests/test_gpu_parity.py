@@ -351,3 +351,39 @@ def test_spectrum_cache_across_q_batches(q_batch, Q):
         fresh = plan.run(p, Q=Q)
         plan.close()
         assert np.array_equal(cached, fresh)
+
+
+def test_f8192_level_in_row_kernels():
+    """At F = 8192 the row-pair kernels run the radix-2 column stage in their
+    loads and stores (no column kernels, no spectrum-cache entry).  At
+    n = k = 20000 (B = 64) that level holds 3 parents over 5 children, so it
+    has a parent with a missing right child and a pair with one parent:
+    per-player parity with the oracle, targets pruned through the level, and
+    the adjoint against its cache-off recompute."""
+    n = k = 20000
+    rng = np.random.default_rng(8192)
+    S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+    p = np.arange(k, 0, -1, dtype=float)
+    p /= p.sum()
+    plan = icm_amd.Plan(S, k)
+    geo = icm_amd.plan_geometry(n, k, plan.summary()["B"])
+    lvl = [i for i, F in enumerate(geo["F"]) if F == 8192]
+    assert lvl and geo["n_real"][lvl[0]] == 3 and geo["n_real"][lvl[0] - 1] == 5
+    say(f"F=8192 level n={n} k={k}: level {lvl[0]}, F = {[int(F) for F in geo['F']]}")
+    full = plan.run(p, Q=64)
+    cpu = O.equity(S, p, Q=64, fft_min=64)
+    err = rel_err(full, cpu)
+    say(f"  rel err vs oracle {err:.3e}")
+    assert err < 1e-8
+    targets = rng.choice(n, 48, replace=False)
+    plan.set_targets(targets)
+    sub = plan.run(p, Q=64)
+    plan.set_targets(None)
+    np.testing.assert_allclose(sub[targets], full[targets], rtol=1e-10)
+    v = rng.normal(size=n)
+    e1, d1 = plan.vjp(p, v, Q=16)
+    plan.set_spectrum_cache(False)
+    e0, d0 = plan.vjp(p, v, Q=16)
+    plan.close()
+    assert np.max(np.abs(e1 - e0)) / np.max(np.abs(e0)) < 1e-12
+    assert np.max(np.abs(d1 - d0)) / np.max(np.abs(d0)) < 1e-11
