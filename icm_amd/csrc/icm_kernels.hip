@@ -29,9 +29,35 @@ __device__ __forceinline__ double a_of(double S, double lv) {
     return (arg < -700.0) ? 0.0 : exp(arg);
 }
 
+// Coalesced store of the 64 lanes' block products (B + 1 coefficients each;
+// lane i's block at out + i (B + 1)) through an LDS transpose in two halves,
+// coefficients [0, H) and then [H, B + 1): a 64 x H tile is half the LDS of a
+// whole one, which left room for one wave per SIMD only.
+template <int B> constexpr int leaf_half() { return (B + 2) / 2; }
+template <int B>
+__device__ __forceinline__ void store_block_products(const double (&P)[B + 1], double *sh, double *out, int nblk) {
+    constexpr int H = leaf_half<B>(), H2 = B + 1 - H;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < H; ++m) sh[lane * H + m] = P[m];
+    __syncthreads();
+    for (int e = lane; e < nblk * H; e += 64) {
+        const int blk = e / H;
+        out[(long long)blk * (B + 1) + (e - blk * H)] = sh[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < H2; ++m) sh[lane * H2 + m] = P[H + m];
+    __syncthreads();
+    for (int e = lane; e < nblk * H2; e += 64) {
+        const int blk = e / H2;
+        out[(long long)blk * (B + 1) + H + (e - blk * H2)] = sh[e];
+    }
+}
+
 template <int B>
 __global__ __launch_bounds__(64) void k_leaf_build(LeafJob J) {
-    __shared__ double sh[64 * (B + 1)];
+    __shared__ double sh[64 * leaf_half<B>()];
     const int lane = threadIdx.x;
     const int qi = blockIdx.y;
     const int b0 = blockIdx.x * 64;
@@ -54,12 +80,8 @@ __global__ __launch_bounds__(64) void k_leaf_build(LeafJob J) {
             P[0] *= a;
         }
     }
-#pragma unroll
-    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
-    __syncthreads();
-    const int nblk = min(64, J.nb - b0);
-    double *out = J.blocks + (long long)qi * J.nb * (B + 1) + (long long)b0 * (B + 1);
-    for (int e = lane; e < nblk * (B + 1); e += 64) out[e] = sh[e];
+    store_block_products<B>(P, sh, J.blocks + (long long)qi * J.nb * (B + 1) + (long long)b0 * (B + 1),
+                            min(64, J.nb - b0));
 }
 
 // ---------------------------------------------------------------------------
@@ -179,7 +201,7 @@ __global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__rest
                                                          const int *__restrict__ blk_inst,
                                                          const double *__restrict__ S, double *blocks, int NB,
                                                          int q0) {
-    __shared__ double sh[64 * (B + 1)];
+    __shared__ double sh[64 * leaf_half<B>()];
     const int lane = threadIdx.x;
     const int qi = blockIdx.y;
     const int b0 = blockIdx.x * 64;
@@ -202,12 +224,7 @@ __global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__rest
             P[0] *= a;
         }
     }
-#pragma unroll
-    for (int m = 0; m <= B; ++m) sh[lane * (B + 1) + m] = P[m];
-    __syncthreads();
-    const int nblk = min(64, NB - b0);
-    double *out = blocks + (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1);
-    for (int e = lane; e < nblk * (B + 1); e += 64) out[e] = sh[e];
+    store_block_products<B>(P, sh, blocks + (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1), min(64, NB - b0));
 }
 
 template <int B>
