@@ -1,17 +1,13 @@
 // fft_lds.hpp -- workgroup-cooperative FP64 complex FFTs on an LDS tile.
 //
-// One workgroup of NT threads (256 or 512) owns a tile of 4096 complex
-// doubles in LDS.  The tile holds 4096/FS independent transforms of size FS.
-// Every Stockham pass gives each thread 4096/NT points: butterflies of the
-// main radix (16 for 256 threads, 8 for 512), or of a smaller radix for the
-// one mixed pass when log2(FS) is not a multiple of the main radix's log.
-// Data goes LDS -> registers -> butterfly -> LDS, two barriers per pass.
-//
-// LDS layout (TileLayout): element idx of transform it sits at
-// it * IS + idx + idx/16 with the odd item stride IS = FS + FS/16 + 1.  The
-// idx/16 gap spreads the stride-8/16 writes of the Stockham passes over the
-// banks, and the odd IS spreads accesses that step across transforms (the
-// column kernels load one row of many transforms at a time).
+// NT butterflies per pass cover a tile of 4096 complex doubles in LDS
+// (4096/NT points each: radix 16 for NT = 256, 8 for NT = 512), which holds
+// 4096/FS independent transforms of size FS; a smaller radix takes the one
+// mixed pass when log2(FS) is not a multiple of the main radix's log.  Data
+// goes LDS -> registers -> butterfly -> LDS, two barriers per pass; the IO
+// variant feeds the first pass from a loader and drains the last into a
+// storer.  Layouts (below): SwzLayout for radix-8 passes, Swz16Layout for
+// radix-16 passes, ColLayout for the four-step column kernels.
 //
 // Twiddles come from host tables computed from long-double cos/sin (one
 // rounding per entry), no on-device sin/cos.  Table layout (see tw_big):
@@ -27,8 +23,6 @@
 namespace icm {
 
 constexpr int kTile = 4096;   // complex points per workgroup tile
-constexpr int kThreads = 256; // threads per workgroup
-constexpr int kPer = kTile / kThreads;  // 16 points per thread
 
 // w_F^m for F = 2^logF <= FT = 2^twShift and 0 <= m < F: M = m FT / F split
 // into a coarse index (top 12 bits) and a fine index; one complex product.
@@ -54,14 +48,6 @@ __device__ __forceinline__ double2 tw_big2(const double2 *tw, int twShift, int m
 }
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
-
-template <int FS> struct TileLayout {
-    static constexpr bool kItemFast = false;        // butterflies of one transform on consecutive lanes
-    static constexpr int IS = FS + FS / 16 + 1;     // item stride (odd)
-    static constexpr int ITEMS = kTile / FS;
-    static constexpr int SIZE = ITEMS * IS;         // complex elements of LDS
-    __device__ static __forceinline__ int at(int item, int idx) { return item * IS + idx + (idx >> 4); }
-};
 
 // Unpadded, XOR-swizzled tile for the 512-thread (radix-8) kernels, whose
 // accesses are always runs of consecutive elements of one transform or
@@ -400,12 +386,12 @@ __device__ __forceinline__ void fft_tile_io(double2 *lds, const double2 *tw, con
     }
 }
 
-// In-place transform of every size-FS transform of the tile (TileLayout<FS>) by NT
+// In-place transform of every size-FS transform of the tile (layout TL) by NT
 // threads (256: radix-16 passes; 512: radix-8 passes), after one mixed pass
 // when log2(FS) is not a multiple of the main radix's log.
 // Starts with a barrier so callers may write the tile and call directly;
 // ends with a barrier (the last pass's).
-template <int FS, bool INV, int NT = kThreads, class TL = TileLayout<FS>>
+template <int FS, bool INV, int NT, class TL>
 __device__ __forceinline__ void fft_tile(double2 *lds, const double2 *tw, int twShift) {
     constexpr int RMAX = kTile / NT;          // 16 or 8
     constexpr int LR = ilog2c(RMAX);

@@ -133,6 +133,8 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
 
 // One tree level through HBM with a three-pass four-step transform (F >= 8192).
 // bufA / bufB: scratch of (n_real[l] + ceil(n_real[l]/2)) * F * nq complex each.
+// At F = 8192 (col_fused) the column stage runs inside the row kernels, which
+// read and write the coefficients directly: bufA, bufB and spec are unused.
 // spec (optional, nq * n_real[l] * F complex): the build keeps the column
 // transforms of the child pairs there, and the propagate of the same level
 // and q-planes reads them instead of transforming the children again (only
