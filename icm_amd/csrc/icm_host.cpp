@@ -657,16 +657,26 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 continue;
             }
             if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
-            {
-                // same g pair as the pass above: its column transforms are
-                // still in bufA
-                icm::LevelJob K = J;
-                K.child = view(l - 1, true);
-                K.r_missing_zero = 1;
-                K.gchild = Bout;
-                K.gfix = gfix;
-                if (!level(K, false, dspec_of(l), true)) return -1;   // B_L = A_X * dR
+            icm::LevelJob K = J;                                  // B_L = A_X * dR ...
+            K.child = view(l - 1, true);
+            K.r_missing_zero = 1;
+            K.gchild = Bout;
+            K.gfix = gfix;
+            if (!root && dspec_of(l)) {                           // ... + B_X * R in one pass
+                icm::LevelJob KB = J;
+                KB.gpar = Bpar;
+                KB.gchild = Bout;
+                KB.gfix = gfix;
+                if (!hip_ok(icm::launch_prop_large_b(K, KB, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, spec_of(l),
+                                                     dspec_of(l), s, &p->prof),
+                            "vjp propagate level"))
+                    return -1;
+                ca = na;
+                cb = nbv;
+                continue;
             }
+            // same g pair as the pass above: its column transforms are still in bufA
+            if (!level(K, false, dspec_of(l), true)) return -1;
             if (!root) {
                 icm::LevelJob K = J;
                 K.gpar = Bpar;

@@ -1610,6 +1610,24 @@ struct StGChildren {      // array a = slot (propagate): g of the slot's parent'
         if (o.y) put(&o.y[n], vr, J.acc_out);
     }
 };
+// Sum of two propagates into the same children (J1 with its parents' g and
+// children, J2 likewise; J1's outputs, counts and flags): the one wrapped
+// coefficient of each correlation is fixed exactly.
+struct StGChildren2 {
+    LevelJob J1, J2;
+    __device__ OutRows rows(int a, int qi) const { return StGChildren{J1}.rows(a, qi); }
+    __device__ void store(const OutRows &o, int qi, int n, double2 v) const {
+        if (!o.x || n >= J1.gchild.count) return;
+        const double invF = 1.0 / (double)J1.F;
+        double vl = v.x * invF, vr = v.y * invF;
+        if (J1.gfix && n == J1.child.count - 1) {
+            vl += prop_wrap_term(J1, qi, o.p, 0) + prop_wrap_term(J2, qi, o.p, 0);
+            vr += prop_wrap_term(J1, qi, o.p, 1) + prop_wrap_term(J2, qi, o.p, 1);
+        }
+        put(&o.x[n], vl, J1.acc_out);
+        if (o.y) put(&o.y[n], vr, J1.acc_out);
+    }
+};
 
 // Column kernels: 512 threads (8 points each, radix-8 passes), so two 64 KB
 // workgroups keep 16 waves per CU in flight for the HBM-bound passes.
@@ -1746,6 +1764,7 @@ using RowBuild = RowR16<true>;
 using RowProp = RowR16<true>;
 using RowBuildT = RowR16<false>;
 using RowPropFused = RowR16<false>;   // F = 8192 propagate: its loaders hold more state (33 spills with KEEP)
+using RowPropB = RowR16<false>;       // adjoint B propagate: four g spectra live
 
 template <class TL> __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
 
@@ -1961,6 +1980,75 @@ __global__ __launch_bounds__(RS::NTH) void k_rowpair_prop(SrcC srcc, SrcG srcg, 
     };
     parent(2 * pair, G1);
     if (2 * pair + 1 < nrp) parent(2 * pair + 1, G2);   // uniform over the workgroup
+}
+
+// Adjoint B propagate of a four-step level in one pass: per parent
+//   U = A conj(dR) + B conj(R) + i (A conj(dL) + B conj(L))
+// (A, B the parent's two adjoints; L + iR its children's products, dL + i dR
+// their tangent products), one inverse per parent instead of one for each
+// of the two correlations.  srca / srcb: the A and B g pairs, srcp / srcd:
+// the child pairs' P and dP rows.
+template <class RS, class SrcP, class SrcD, class SrcA, class SrcB, class Dst>
+__global__ __launch_bounds__(RS::NTH) void k_rowpair_prop_b(SrcP srcp, SrcD srcd, SrcA srca, SrcB srcb, Dst dst,
+                                                            int nrp, int npairs, int logF, const double2 *tw,
+                                                            int twShift) {
+    constexpr int NTH = RS::NTH, RU = kTile / NTH;
+    using TL = typename RS::TL;
+    __shared__ double2 T[2 * kF2];
+    const int F = 1 << logF, F1 = F >> 12;
+    const int w = blockIdx.x, rA = row_of(w, F1, 0), rB = row_of(w, F1, 1);
+    const int pair = blockIdx.y, qi = blockIdx.z;
+    const int t = threadIdx.x;
+    (void)twShift;
+    (void)npairs;
+    double2 GA1[RU], GA2[RU], GB1[RU], GB2[RU];
+    srca.template load_fft<RS>(T, pair, qi, rA, rB, tw);
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+        const RowRep q = row_rep<TL>(w, t + u * NTH);
+        row_split(q, T[q.ia], T[q.ib], GA1[u], GA2[u]);
+    }
+    srcb.template load_fft<RS>(T, pair, qi, rA, rB, tw);
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {
+        const RowRep q = row_rep<TL>(w, t + u * NTH);
+        row_split(q, T[q.ia], T[q.ib], GB1[u], GB2[u]);
+    }
+    auto parent = [&](int p, const double2 (&GA)[RU], const double2 (&GB)[RU]) {
+        double2 UL[RU], UR[RU];
+        srcd.template load_fft<RS>(T, p, qi, rA, rB, tw);
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
+            double2 dL, dR;
+            row_split(q, T[q.ia], T[q.ib], dL, dR);
+            if (q.fixed) {
+                UL[u] = make_double2(GA[u].x * dR.x, GA[u].y * dR.y);
+                UR[u] = make_double2(GA[u].x * dL.x, GA[u].y * dL.y);
+            } else {
+                UL[u] = c_mulc(GA[u], dR);
+                UR[u] = c_mulc(GA[u], dL);
+            }
+        }
+        srcp.template load_fft<RS>(T, p, qi, rA, rB, tw);
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const RowRep q = row_rep<TL>(w, t + u * NTH);
+            double2 L, R;
+            row_split(q, T[q.ia], T[q.ib], L, R);
+            if (q.fixed) {
+                UL[u] = make_double2(UL[u].x + GB[u].x * R.x, UL[u].y + GB[u].y * R.y);
+                UR[u] = make_double2(UR[u].x + GB[u].x * L.x, UR[u].y + GB[u].y * L.y);
+            } else {
+                UL[u] = c_add(UL[u], c_mulc(GB[u], R));
+                UR[u] = c_add(UR[u], c_mulc(GB[u], L));
+            }
+            row_put(T, q, UL[u], UR[u]);
+        }
+        dst.template ifft_store<RS>(T, p, qi, rA, rB, tw);
+    };
+    parent(2 * pair, GA1, GB1);
+    if (2 * pair + 1 < nrp) parent(2 * pair + 1, GA2, GB2);   // uniform over the workgroup
 }
 
 // Adjoint tangent build: V[pair] rows = inverse rows of
@@ -2359,6 +2447,32 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StGChildren>", F1), 16.0 * nq * nrp * F + 8.0 * nq * J.nr_child * J.gchild.count, nq * nrp * kF2 * fft_flops(F1));
+    return hipSuccess;
+}
+
+hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const double2 *tw, int twShift,
+                               double2 *bufA, double2 *bufB, const double2 *specP, const double2 *specD, hipStream_t s,
+                               Prof *pf) {
+    if (!specP || !specD || JdR.plist || col_fused(JdR.F)) return hipErrorInvalidValue;
+    const int F1 = JdR.F >> 12;
+    const int nrp = JdR.np, npairs = (nrp + 1) / 2;
+    const long long r1 = (long long)JdR.nq * nrp * JdR.F;
+    const double F = JdR.F, nq = JdR.nq;
+    const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    const double gin = JR.gpar.plane == 0 ? 8.0 * JR.gpar.count : 8.0 * nq * nrp * JR.gpar.count;
+    PF_BEGIN();   // the B pairs into the child-pair region of bufA (unused with the caches)
+    ICM_TRY(col_fwd(F1, GRows{JR}, bufA, npairs, JdR.nq, JdR.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    PF_BEGIN();
+    k_rowpair_prop_b<RowPropB, RowSrc, RowSrc, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, JdR.nq), RowPropB::NTH, 0, s>>>(
+        RowSrc{specP, nrp, JdR.logF}, RowSrc{specD, nrp, JdR.logF}, RowSrc{bufA + r1, npairs, JdR.logF},
+        RowSrc{bufA, npairs, JdR.logF}, RowDst{bufB, nrp, JdR.logF}, nrp, npairs, JdR.logF, tw, twShift);
+    ICM_TRY(hipGetLastError());
+    PF_END("k_rowpair_prop_b", 16.0 * nq * (3 * nrp + 2 * npairs) * F, nq * (3 * nrp + 2 * npairs) * rowFl);
+    PF_BEGIN();
+    ICM_TRY(col_inv(F1, bufB, StGChildren2{JdR, JR}, nrp, JdR.nq, JdR.logF, tw, twShift, s));
+    PF_END(kname("k_col_inv<%d,StGChildren2>", F1),
+           16.0 * nq * nrp * F + 8.0 * nq * JdR.nr_child * JdR.gchild.count, nq * nrp * kF2 * fft_flops(F1));
     return hipSuccess;
 }
 

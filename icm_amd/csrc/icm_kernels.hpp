@@ -154,6 +154,16 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
                              double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr,
                              bool g_ready = false);
 
+// Adjoint B propagate of a four-step level in one pass: B children =
+// A * dR + B * R (JdR: A's g pair over the dP children, JR: B's g pair over
+// the P children; JdR's outputs and flags).  Needs both spectrum caches
+// (specP: the P pairs' column transforms, specD: the dP pairs') and bufA
+// holding JdR's g-pair column transforms (the A propagate just ran); B's
+// pairs go to bufA's child-pair region.  No parent lists.
+hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const double2 *tw, int twShift,
+                               double2 *bufA, double2 *bufB, const double2 *specP, const double2 *specD, hipStream_t s,
+                               Prof *pf);
+
 // Ragged batches.  Leaves: global blocks [q][NB][B+1] for q-planes
 // q0 .. q0 + nq - 1; extract writes partial[chunk][Ntot].  Levels: one
 // launch per tree level over nitems work items (job, first tile item);
