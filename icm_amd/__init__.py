@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 __all__ = ["ICMError", "init", "equity", "equity_subset", "equity_vjp", "node_inner", "node_vjp", "Plan",
-           "plan_geometry", "make_nodes", "library_path", "load_library"]
+           "BatchPlan", "plan_geometry", "plan_dry_run", "make_nodes", "library_path", "load_library",
+           "measure_hbm_bandwidth_gbps", "measure_stream_gbps", "measure_fused_pair_ns"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libicm_hip.so"
@@ -34,6 +35,12 @@ class Options(ctypes.Structure):
                 ("force_uncached_fused_levels", ctypes.c_int),
                 ("force_uncached_cufft_levels", ctypes.c_int),
                 ("fast_mode", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class PlanParams(ctypes.Structure):
+    """Layout of IcmHipPlanParams (engine parameters with no IcmGpuOptions field)."""
+    _fields_ = [("block_size", ctypes.c_int), ("q_batch", ctypes.c_int),
+                ("mem_fraction", ctypes.c_double), ("mem_budget_bytes", ctypes.c_size_t)]
 
 
 class RunStats(ctypes.Structure):
@@ -65,6 +72,11 @@ _SIGS = [
     ("icm_hip_last_error", ctypes.c_char_p, []),
     ("icm_hip_release_pooled_memory", ctypes.c_int, []),
     ("icm_hip_plan_create", ctypes.c_void_p, [ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(Options)]),
+    ("icm_hip_plan_create_ex", ctypes.c_void_p,
+     [ctypes.c_int, _D, ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(PlanParams)]),
+    ("icm_hip_plan_dry_run", ctypes.c_int,
+     [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(PlanParams),
+      ctypes.POINTER(PlanSummary), _I]),
     ("icm_hip_plan_destroy", None, [ctypes.c_void_p]),
     ("icm_hip_plan_summary", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PlanSummary)]),
     ("icm_hip_equity_with_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.POINTER(RunStats)]),
@@ -87,6 +99,10 @@ _SIGS = [
     ("icm_gpu_equity_subset", ctypes.c_int,
      [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int,
       ctypes.POINTER(Options), ctypes.POINTER(RunStats)]),
+    ("icm_gpu_measure_hbm_bandwidth_gbps", ctypes.c_int, [_D]),
+    ("icm_gpu_measure_fused_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("icm_gpu_measure_fused_r2c_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("icm_hip_measure_stream_gbps", ctypes.c_int, [ctypes.c_size_t, _D, _D, _D]),
     ("icm_init", None, [ctypes.c_char_p]),
     ("icm_equity", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
     ("icm_equity_subset", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int]),
@@ -110,6 +126,8 @@ _SIGS = [
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
     ("icm_hip_batch_plan_create", ctypes.c_void_p, [ctypes.c_int, _I, _D, _I, ctypes.POINTER(Options)]),
+    ("icm_hip_batch_plan_create_ex", ctypes.c_void_p,
+     [ctypes.c_int, _I, _D, _I, ctypes.POINTER(Options), ctypes.POINTER(PlanParams)]),
     ("icm_hip_batch_plan_destroy", None, [ctypes.c_void_p]),
     ("icm_hip_batch_equity", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, ctypes.POINTER(RunStats)]),
     ("icm_hip_batch_equity_device", ctypes.c_int,
@@ -200,6 +218,26 @@ def init(wisdom_path=None, device_id=0):
 def _auto_init():
     if not _initialized:
         init()
+
+
+def _options(verbose=False, graphs=True, memory_strategy=0):
+    """IcmHipOptions with the reference's defaults (-1 = auto for the level counts)."""
+    o = Options()
+    o.enable_graphs = int(bool(graphs))
+    o.memory_strategy = int(memory_strategy)
+    o.force_uncached_fused_levels = -1
+    o.force_uncached_cufft_levels = -1
+    o.verbose = int(bool(verbose))
+    return o
+
+
+def _params(block_size=0, q_batch=0, mem_fraction=0.0, mem_budget_bytes=0):
+    p = PlanParams()
+    p.block_size = int(block_size)
+    p.q_batch = int(q_batch)
+    p.mem_fraction = float(mem_fraction)
+    p.mem_budget_bytes = int(mem_budget_bytes)
+    return p
 
 
 def _validate(n, k, Q):
@@ -295,17 +333,16 @@ class Plan:
     """Reusable plan (IcmGpuPlan, src/gpu/icm_gpu.h:79-111): sort, geometry and
     device memory are set up once for (stacks, k); ``run`` takes Q and payout."""
 
-    def __init__(self, stacks, k, block_size=0, q_batch=0, verbose=False, schoolbook_max=None):
+    def __init__(self, stacks, k, block_size=0, q_batch=0, verbose=False, schoolbook_max=None,
+                 mem_fraction=0.0, mem_budget_bytes=0, memory_strategy=0):
         _auto_init()
         self._lib = load_library()
         s = _to_double_array(stacks, "stacks")
         self.n, self.k = len(s), int(k)
         _validate(self.n, self.k, 1)
-        opts = Options()
-        opts.force_uncached_fused_levels = int(block_size)
-        opts.force_uncached_cufft_levels = int(q_batch)
-        opts.verbose = int(bool(verbose))
-        self._h = self._lib.icm_hip_plan_create(self.n, _ptr(s), self.k, ctypes.byref(opts))
+        opts = _options(verbose=verbose, memory_strategy=memory_strategy)
+        prm = _params(block_size, q_batch, mem_fraction, mem_budget_bytes)
+        self._h = self._lib.icm_hip_plan_create_ex(self.n, _ptr(s), self.k, ctypes.byref(opts), ctypes.byref(prm))
         if not self._h:
             raise _err(self._lib, "icm_hip_plan_create")
         self.stats = RunStats()
@@ -441,7 +478,8 @@ class BatchPlan:
     instance; the first run for a Q is recorded as a HIP graph and replayed
     by later runs."""
 
-    def __init__(self, stacks_list, k_list, block_size=0, q_batch=0, graphs=True, verbose=False):
+    def __init__(self, stacks_list, k_list, block_size=0, q_batch=0, graphs=True, verbose=False,
+                 mem_fraction=0.0, mem_budget_bytes=0):
         _auto_init()
         self._lib = load_library()
         if len(stacks_list) != len(k_list) or len(stacks_list) == 0:
@@ -454,13 +492,11 @@ class BatchPlan:
         self.n_off[1:] = np.cumsum([len(s) for s in ss])
         self.k_off[1:] = np.cumsum([int(k) for k in k_list])
         S = np.concatenate(ss)
-        opts = Options()
-        opts.enable_graphs = int(bool(graphs))
-        opts.force_uncached_fused_levels = int(block_size)
-        opts.force_uncached_cufft_levels = int(q_batch)
-        opts.verbose = int(bool(verbose))
-        self._h = self._lib.icm_hip_batch_plan_create(len(ss), _ptr(self.n_off, ctypes.c_int), _ptr(S),
-                                                      _ptr(self.k_off, ctypes.c_int), ctypes.byref(opts))
+        opts = _options(verbose=verbose, graphs=graphs)
+        prm = _params(block_size, q_batch, mem_fraction, mem_budget_bytes)
+        self._h = self._lib.icm_hip_batch_plan_create_ex(len(ss), _ptr(self.n_off, ctypes.c_int), _ptr(S),
+                                                         _ptr(self.k_off, ctypes.c_int), ctypes.byref(opts),
+                                                         ctypes.byref(prm))
         if not self._h:
             raise _err(self._lib, "icm_hip_batch_plan_create")
         self.n_inst = len(ss)
@@ -632,3 +668,48 @@ def make_nodes(Q, Smax):
     w = np.zeros(Q)
     lib.icm_hip_make_nodes(int(Q), float(Smax), _ptr(logv), _ptr(w))
     return logv, w
+
+
+def plan_dry_run(n, k, mem_budget_bytes, block_size=0, q_batch=0, mem_fraction=0.0, memory_strategy=0):
+    """Host-only planning against a simulated device with mem_budget_bytes
+    free: (summary dict, q-batch halvings).  Raises ICMError when one q-plane
+    does not fit."""
+    lib = load_library()
+    sm = PlanSummary()
+    retries = ctypes.c_int(0)
+    if lib.icm_hip_plan_dry_run(int(n), int(k), ctypes.byref(_options(memory_strategy=memory_strategy)),
+                                ctypes.byref(_params(block_size, q_batch, mem_fraction, mem_budget_bytes)),
+                                ctypes.byref(sm), ctypes.byref(retries)) != 0:
+        raise _err(lib, "icm_hip_plan_dry_run")
+    return {f: getattr(sm, f) for f, _ in PlanSummary._fields_}, retries.value
+
+
+def measure_hbm_bandwidth_gbps():
+    """Device-copy rate of the reference probe (bytes copied per second)."""
+    _auto_init()
+    lib = load_library()
+    v = ctypes.c_double(0)
+    if lib.icm_gpu_measure_hbm_bandwidth_gbps(ctypes.byref(v)) != 1:
+        raise _err(lib, "icm_gpu_measure_hbm_bandwidth_gbps")
+    return v.value
+
+
+def measure_stream_gbps(nbytes=1 << 30):
+    """(read, write, copy) HBM rates of this library's streaming kernels."""
+    _auto_init()
+    lib = load_library()
+    r, w, c = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+    if lib.icm_hip_measure_stream_gbps(int(nbytes), ctypes.byref(r), ctypes.byref(w), ctypes.byref(c)) != 1:
+        raise _err(lib, "icm_hip_measure_stream_gbps")
+    return r.value, w.value, c.value
+
+
+def measure_fused_pair_ns(fft_n, batch, quick=False):
+    """(build, propagate) device ns per parent of one LDS-level launch."""
+    _auto_init()
+    lib = load_library()
+    b, c = ctypes.c_double(0), ctypes.c_double(0)
+    if lib.icm_gpu_measure_fused_pair_ns(int(fft_n), int(batch), int(bool(quick)), ctypes.byref(b),
+                                         ctypes.byref(c)) != 1:
+        raise _err(lib, "icm_gpu_measure_fused_pair_ns")
+    return b.value, c.value

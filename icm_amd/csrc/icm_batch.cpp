@@ -56,6 +56,7 @@ struct IcmHipBatchPlan {
     double *d_payout = nullptr;
     double *d_nodes = nullptr;           // [ninst][2][Q]
     int nodes_Q = 0;
+    int nodes_alloc_Q = 0;               // Q that d_nodes was sized for
     icm::BatchInst *d_inst = nullptr;
     int *d_blk_inst = nullptr;
     double *d_blocks = nullptr;          // [qb][NB][B+1]
@@ -69,6 +70,7 @@ struct IcmHipBatchPlan {
         int nq = 0;
         icm::LevelJob *d_jobs = nullptr;
         int2 *d_items = nullptr;
+        size_t dev_bytes = 0;
         // per level l: build items [b_off[l], b_off[l] + b_cnt[l]), prop items
         // likewise; transform levels, then schoolbook levels (bs_*, ps_*)
         std::vector<int> b_off, b_cnt, p_off, p_cnt;
@@ -93,7 +95,8 @@ struct IcmHipBatchPlan {
     int graph_Q[2] = {0, 0};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t bytes = 0;
+    DevMem mem;                          // device bytes held (cap accounting)
+    int retries = 0;                     // q-pass halvings at creation (out-of-memory policy)
     KernelProf prof;
 };
 
@@ -101,11 +104,7 @@ namespace {
 
 template <class T>
 bool balloc(IcmHipBatchPlan *p, T **ptr, size_t count) {
-    if (count == 0) count = 1;
-    const size_t b = count * sizeof(T);
-    if (!hip_ok(hipMalloc((void **)ptr, b), "hipMalloc")) return false;
-    p->bytes += b;
-    return true;
+    return p->mem.alloc(ptr, count);
 }
 
 void drop_graph(IcmHipBatchPlan *p, int mode) {
@@ -121,25 +120,35 @@ void drop_graphs(IcmHipBatchPlan *p) {
     drop_graph(p, 1);
 }
 
-void free_passes(std::vector<IcmHipBatchPlan::Pass> &v) {
+void free_passes(IcmHipBatchPlan *p, std::vector<IcmHipBatchPlan::Pass> &v) {
     for (auto &ps : v) {
         if (ps.d_jobs) (void)hipFree(ps.d_jobs);
         if (ps.d_items) (void)hipFree(ps.d_items);
+        p->mem.bytes -= std::min(p->mem.bytes, ps.dev_bytes);
     }
     v.clear();
 }
 
+// Frees the plan's device buffers (forward, adjoint, node and job tables)
+// and clears the byte count; graphs, stream and events stay.
+void free_buffers(IcmHipBatchPlan *p) {
+    drop_graphs(p);
+    free_passes(p, p->passes);
+    free_passes(p, p->vpasses);
+    auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
+    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_inst); f(p->d_blk_inst);
+    f(p->d_blocks); for (double *&c : p->d_coef) f(c);
+    f(p->d_g[0]); f(p->d_g[1]); f(p->d_partial); f(p->d_acc); f(p->d_equity);
+    f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_vin); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    p->nodes_Q = 0;
+    p->vjp_ready = false;
+    p->mem.bytes = 0;
+}
+
 void batch_free(IcmHipBatchPlan *p) {
     if (!p) return;
-    drop_graphs(p);
-    auto f = [](void *x) { if (x) (void)hipFree(x); };
-    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_inst); f(p->d_blk_inst);
-    f(p->d_blocks); for (double *c : p->d_coef) f(c);
-    f(p->d_g[0]); f(p->d_g[1]); f(p->d_partial); f(p->d_acc); f(p->d_equity);
-    f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
-    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_vin); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
-    free_passes(p->passes);
-    free_passes(p->vpasses);
+    free_buffers(p);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -255,7 +264,9 @@ bool make_vjp_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
         ps.p_cnt[l] = (int)items.size() - ps.p_off[l];
     }
     if (items.size() > 0x7fffffffULL) { set_err("batch too large: work items exceed 2^31"); return false; }
+    const size_t before = p->mem.bytes;
     if (!balloc(p, &ps.d_jobs, jobs.size()) || !balloc(p, &ps.d_items, items.size())) return false;
+    ps.dev_bytes = p->mem.bytes - before;
     return (jobs.empty() || HIP_OK(hipMemcpy(ps.d_jobs, jobs.data(), sizeof(icm::LevelJob) * jobs.size(),
                                              hipMemcpyHostToDevice))) &&
            (items.empty() || HIP_OK(hipMemcpy(ps.d_items, items.data(), sizeof(int2) * items.size(),
@@ -355,7 +366,9 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
         items.insert(items.end(), sb.begin(), sb.end());
     }
     if (items.size() > 0x7fffffffULL) { set_err("batch too large: work items exceed 2^31"); return false; }
+    const size_t before = p->mem.bytes;
     if (!balloc(p, &ps.d_jobs, jobs.size()) || !balloc(p, &ps.d_items, items.size())) return false;
+    ps.dev_bytes = p->mem.bytes - before;
     return (jobs.empty() || HIP_OK(hipMemcpy(ps.d_jobs, jobs.data(), sizeof(icm::LevelJob) * jobs.size(),
                                              hipMemcpyHostToDevice))) &&
            (items.empty() || HIP_OK(hipMemcpy(ps.d_items, items.data(), sizeof(int2) * items.size(),
@@ -363,7 +376,7 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
 }
 
 IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, const int *k_off,
-                              const IcmHipOptions *opts) {
+                              const IcmHipOptions *opts, const IcmHipPlanParams *params = nullptr) {
     if (n_inst < 1 || !n_off || !S || !k_off) { set_err("invalid arguments: need n_inst >= 1 and offsets"); return nullptr; }
     if (n_off[0] != 0 || k_off[0] != 0) { set_err("offset arrays must start at 0"); return nullptr; }
     for (int i = 0; i < n_inst; ++i) {
@@ -378,6 +391,8 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     const int Ntot = n_off[n_inst];
     for (int j = 0; j < Ntot; ++j)
         if (!(S[j] > 0.0) || !std::isfinite(S[j])) { set_err("stacks must be positive and finite"); return nullptr; }
+    PlanCfg cfg;
+    if (!plan_config(opts, params, cfg)) return nullptr;
     if (!ensure_device()) return nullptr;
 
     IcmHipBatchPlan *p = new IcmHipBatchPlan();
@@ -386,10 +401,9 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     p->Ktot = k_off[n_inst];
     p->n_off.assign(n_off, n_off + n_inst + 1);
     p->k_off.assign(k_off, k_off + n_inst + 1);
-    p->verbose = opts && opts->verbose;
-    p->use_graph = !opts || opts->enable_graphs;
-    const int forcedB = opts ? opts->force_uncached_fused_levels : 0;
-    p->B = (forcedB == 16 || forcedB == 32 || forcedB == 64) ? forcedB : kBatchTune.B;
+    p->verbose = cfg.verbose;
+    p->use_graph = cfg.graphs;
+    p->B = cfg.B ? cfg.B : kBatchTune.B;
     const int B = p->B;
     p->geo.resize(n_inst);
     p->boff.assign(n_inst, 0);
@@ -449,41 +463,65 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     per_plane += gmax[0] + gmax[1];
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { delete p; return nullptr; }
-    const long long budget = (long long)(0.6 * (double)freeb);
+    if (cfg.cap) freeb = std::min(freeb, cfg.cap);
+    p->mem.cap = cfg.cap;
+    const long long budget = (long long)(cfg.frac * (double)freeb);
     int qb = (int)std::max<long long>(1, budget / std::max<long long>(8 * per_plane, 1));
     qb = std::min(qb, 256);
-    const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
-    if (forcedQ > 0) qb = std::min(qb, forcedQ);
-    p->qb = qb;
+    if (cfg.qb > 0) qb = cfg.qb;
     p->sb_max = kBatchTune.sb_max;   // one crossover per batch (tuning table)
-    p->coff.assign(Lm, std::vector<long long>(n_inst, 0));
-    p->goff.assign(Lm, std::vector<long long>(n_inst, 0));
-    p->coef_elems.assign(Lm, 0);
-    for (int l = 0; l < Lm; ++l) {
-        long long co = 0, go = 0;
-        for (int i = 0; i < n_inst; ++i) {
-            const Geometry &g = p->geo[i];
-            if (l >= 1 && l <= g.L - 2) { p->coff[l][i] = co; co += (long long)qb * g.n_real[l] * g.c[l]; }
-            if (l <= g.L - 2) { p->goff[l][i] = go; go += (long long)qb * g.n_real[l] * g.G[l]; }
+    // arena layout for qb q-planes
+    auto layout = [&](int q) {
+        p->qb = q;
+        p->coff.assign(Lm, std::vector<long long>(n_inst, 0));
+        p->goff.assign(Lm, std::vector<long long>(n_inst, 0));
+        p->coef_elems.assign(Lm, 0);
+        p->g_elems[0] = p->g_elems[1] = 0;
+        for (int l = 0; l < Lm; ++l) {
+            long long co = 0, go = 0;
+            for (int i = 0; i < n_inst; ++i) {
+                const Geometry &g = p->geo[i];
+                if (l >= 1 && l <= g.L - 2) { p->coff[l][i] = co; co += (long long)q * g.n_real[l] * g.c[l]; }
+                if (l <= g.L - 2) { p->goff[l][i] = go; go += (long long)q * g.n_real[l] * g.G[l]; }
+            }
+            p->coef_elems[l] = co;
+            p->g_elems[l & 1] = std::max(p->g_elems[l & 1], go);
         }
-        p->coef_elems[l] = co;
-        p->g_elems[l & 1] = std::max(p->g_elems[l & 1], go);
+        const int bpw = 256 / B;
+        const int wg_blocks = (NB + bpw - 1) / bpw;
+        p->nchunk = std::min(std::max(1, 4096 / std::max(1, wg_blocks)), q);
+    };
+    auto alloc_all = [&]() {
+        bool ok = balloc(p, &p->d_tw, 4097) && balloc(p, &p->d_S, Ntot) && balloc(p, &p->d_perm, Ntot) &&
+                  balloc(p, &p->d_payout, p->Ktot) && balloc(p, &p->d_inst, n_inst) &&
+                  balloc(p, &p->d_blk_inst, NB) && balloc(p, &p->d_blocks, (size_t)p->qb * NB * (B + 1)) &&
+                  balloc(p, &p->d_g[0], p->g_elems[0]) && balloc(p, &p->d_g[1], p->g_elems[1]) &&
+                  balloc(p, &p->d_partial, (size_t)p->nchunk * Ntot) && balloc(p, &p->d_acc, Ntot) &&
+                  balloc(p, &p->d_equity, Ntot);
+        p->d_coef.assign(Lm, nullptr);
+        for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_coef[l], p->coef_elems[l]);
+        return ok;
+    };
+    if (!(HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) && HIP_OK(hipEventCreate(&p->ev0)) &&
+          HIP_OK(hipEventCreate(&p->ev1)))) {
+        batch_free(p);
+        return nullptr;
     }
-    const int bpw = 256 / B;
-    const int wg_blocks = (NB + bpw - 1) / bpw;
-    p->nchunk = std::min(std::max(1, 4096 / std::max(1, wg_blocks)), qb);
-
-    bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
-              HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1));
-    ok = ok && balloc(p, &p->d_tw, 4097) && balloc(p, &p->d_S, Ntot) && balloc(p, &p->d_perm, Ntot) &&
-         balloc(p, &p->d_payout, p->Ktot) && balloc(p, &p->d_inst, n_inst) && balloc(p, &p->d_blk_inst, NB) &&
-         balloc(p, &p->d_blocks, (size_t)qb * NB * (B + 1)) && balloc(p, &p->d_g[0], p->g_elems[0]) &&
-         balloc(p, &p->d_g[1], p->g_elems[1]) && balloc(p, &p->d_partial, (size_t)p->nchunk * Ntot) &&
-         balloc(p, &p->d_acc, Ntot) && balloc(p, &p->d_equity, Ntot);
-    p->d_coef.assign(Lm, nullptr);
-    for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_coef[l], p->coef_elems[l]);
-    if (!ok) { batch_free(p); return nullptr; }
-
+    // out-of-memory policy of single plans: halve the q-planes per pass, at
+    // most 4 times (src/gpu/gpu_memory.cu:267-279)
+    for (p->retries = 0;; ++p->retries) {
+        layout(qb);
+        if (alloc_all()) break;
+        free_buffers(p);
+        if (qb > 1 && p->retries < 4) {
+            qb = std::max(1, qb / 2);
+            continue;
+        }
+        const std::string why = g_err;
+        set_err("batch plan allocation failed: " + why);
+        batch_free(p);
+        return nullptr;
+    }
     // twiddles of the in-tile transforms (every level fits one tile)
     std::vector<double2> tw(4097);
     const long double PI = 3.141592653589793238462643383279502884L;
@@ -493,14 +531,14 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     }
     tw[4096] = make_double2(1.0, 0.0);
     p->twShift = 12;
-    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
-         HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * Ntot, hipMemcpyHostToDevice)) &&
-         HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * Ntot, hipMemcpyHostToDevice)) &&
-         HIP_OK(hipMemcpy(p->d_blk_inst, p->blk_inst.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+    const bool ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
+                    HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * Ntot, hipMemcpyHostToDevice)) &&
+                    HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * Ntot, hipMemcpyHostToDevice)) &&
+                    HIP_OK(hipMemcpy(p->d_blk_inst, p->blk_inst.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
     if (!ok) { batch_free(p); return nullptr; }
     if (p->verbose)
         fprintf(stderr, "[icm_hip batch] instances=%d players=%d blocks=%d Lmax=%d B=%d qb=%d nchunk=%d bytes=%.3f GB\n",
-                n_inst, Ntot, NB, Lm, B, qb, p->nchunk, p->bytes / 1e9);
+                n_inst, Ntot, NB, Lm, B, p->qb, p->nchunk, p->mem.bytes / 1e9);
     return p;
 }
 
@@ -526,7 +564,7 @@ bool upload_inst(IcmHipBatchPlan *p, int Q) {
 
 bool make_passes(IcmHipBatchPlan *p, int Q, bool vjp) {
     std::vector<IcmHipBatchPlan::Pass> &v = vjp ? p->vpasses : p->passes;
-    free_passes(v);
+    free_passes(p, v);
     const int nfull = Q / p->qb, rem = Q % p->qb;
     v.resize((nfull > 0 ? 1 : 0) + (rem > 0 ? 1 : 0));
     int idx = 0;
@@ -540,11 +578,16 @@ bool make_passes(IcmHipBatchPlan *p, int Q, bool vjp) {
 bool prepare_Q(IcmHipBatchPlan *p, int Q) {
     if (p->nodes_Q == Q) return true;
     drop_graphs(p);
-    free_passes(p->passes);
-    free_passes(p->vpasses);
-    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
+    free_passes(p, p->passes);
+    free_passes(p, p->vpasses);
+    if (p->d_nodes) {
+        (void)hipFree(p->d_nodes);
+        p->d_nodes = nullptr;
+        p->mem.bytes -= std::min(p->mem.bytes, sizeof(double) * (size_t)p->ninst * 2 * p->nodes_alloc_Q);
+    }
     p->nodes_Q = 0;
     if (!balloc(p, &p->d_nodes, (size_t)p->ninst * 2 * Q)) return false;
+    p->nodes_alloc_Q = Q;
     std::vector<double> nodes((size_t)p->ninst * 2 * Q);
     for (int i = 0; i < p->ninst; ++i) {
         double *lv = nodes.data() + (size_t)i * 2 * Q;
@@ -571,13 +614,22 @@ bool ensure_vjp(IcmHipBatchPlan *p, int Q) {
         }
         p->bv_elems[l & 1] = std::max(p->bv_elems[l & 1], o);
     }
+    const size_t before = p->mem.bytes;
     bool ok = balloc(p, &p->d_dblocks, (size_t)qb * p->NB * (B + 1)) && balloc(p, &p->d_bv[0], p->bv_elems[0]) &&
               balloc(p, &p->d_bv[1], p->bv_elems[1]) && balloc(p, &p->d_vin, p->Ntot) &&
               balloc(p, &p->d_v, p->Ntot) && balloc(p, &p->d_partial_ds, (size_t)p->nchunk * p->Ntot) &&
               balloc(p, &p->d_acc_ds, p->Ntot) && balloc(p, &p->d_ds, p->Ntot);
     p->d_dcoef.assign(Lm, nullptr);
     for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_dcoef[l], p->coef_elems[l]);
-    if (!ok) return false;
+    if (!ok) {   // leave no partial adjoint buffers behind
+        auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
+        f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+        f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_vin); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+        p->mem.bytes = before;
+        const std::string why = g_err;
+        set_err("batch adjoint buffers: " + why);
+        return false;
+    }
     p->vjp_ready = true;
     if (p->nodes_Q == Q) {   // tables of this Q exist: add the B views and the adjoint jobs
         drop_graphs(p);
@@ -716,7 +768,7 @@ int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode = 0) {
         (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
         memset(stats, 0, sizeof(*stats));
         stats->total_ns = (double)ms * 1e6;
-        stats->peak_vram_bytes = p->bytes;
+        stats->peak_vram_bytes = p->mem.bytes;
         stats->engine = 1;
         stats->B = p->B;
         stats->uncached_fused_levels = 2 * p->Lmax - 3;
@@ -731,6 +783,11 @@ extern "C" {
 IcmHipBatchPlan *icm_hip_batch_plan_create(int n_inst, const int *n_off, const double *S, const int *k_off,
                                            const IcmHipOptions *opts) {
     return batch_create(n_inst, n_off, S, k_off, opts);
+}
+
+IcmHipBatchPlan *icm_hip_batch_plan_create_ex(int n_inst, const int *n_off, const double *S, const int *k_off,
+                                              const IcmHipOptions *opts, const IcmHipPlanParams *params) {
+    return batch_create(n_inst, n_off, S, k_off, opts, params);
 }
 
 void icm_hip_batch_plan_destroy(IcmHipBatchPlan *plan) { batch_free(plan); }
@@ -809,8 +866,8 @@ int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
     if (!plan) { set_err("null plan"); return -1; }
     plan->sb_max = cmax < 0 ? 0 : cmax;
     drop_graphs(plan);                  // job tables change with the crossover
-    free_passes(plan->passes);
-    free_passes(plan->vpasses);
+    free_passes(plan, plan->passes);
+    free_passes(plan, plan->vpasses);
     plan->nodes_Q = 0;
     return 0;
 }
@@ -847,7 +904,7 @@ int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_playe
     if (n_blocks) *n_blocks = plan->NB;
     if (max_levels) *max_levels = plan->Lmax;
     if (q_batch) *q_batch = plan->qb;
-    if (device_bytes) *device_bytes = plan->bytes;
+    if (device_bytes) *device_bytes = plan->mem.bytes;
     return 0;
 }
 

@@ -7,10 +7,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
 
+#include "../../include/icm_hip.h"
 #include "icm_kernels.hpp"
 
 namespace icmh {
@@ -153,6 +155,102 @@ struct KernelProf : icm::Prof {
     }
 };
 
+
+// Plan configuration from the reference options struct, the engine's own
+// parameters and the environment (ICM_GPU_Q_BATCH as src/gpu/gpu_api.cu:139-144;
+// ICM_HIP_MEM_BUDGET, bytes).  Values this engine cannot honour are errors.
+struct PlanCfg {
+    int B = 0;               // forced block size (0 = tuning table)
+    int qb = 0;              // forced q-planes per batch (0 = from the budget)
+    double frac = 0.6;       // share of free device memory to plan for
+    size_t cap = 0;          // device-byte cap (0 = none)
+    bool spec_ok = true;     // spectrum cache allowed (memory_strategy != 3)
+    bool verbose = false;
+    bool graphs = true;      // ragged batches: record / replay a HIP graph
+};
+
+inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, PlanCfg &c) {
+    char buf[192];
+    if (o) {
+        if (o->force_uncached_fused_levels > 0 || o->force_uncached_cufft_levels > 0) {
+            snprintf(buf, sizeof buf,
+                     "force_uncached_fused_levels=%d / force_uncached_cufft_levels=%d: this engine has no "
+                     "per-level cache count (use -1); block size and q-batch are IcmHipPlanParams fields",
+                     o->force_uncached_fused_levels, o->force_uncached_cufft_levels);
+            set_err(buf);
+            return false;
+        }
+        c.spec_ok = o->memory_strategy != 3;
+        c.verbose = o->verbose != 0;
+        c.graphs = o->enable_graphs != 0;
+    }
+    if (pp) {
+        if (pp->block_size != 0 && pp->block_size != 16 && pp->block_size != 32 && pp->block_size != 64) {
+            snprintf(buf, sizeof buf, "block_size=%d: expected 0 (tuned), 16, 32 or 64", pp->block_size);
+            set_err(buf);
+            return false;
+        }
+        if (pp->q_batch < 0 || pp->q_batch > 256) {
+            snprintf(buf, sizeof buf, "q_batch=%d: expected 0 (auto) .. 256", pp->q_batch);
+            set_err(buf);
+            return false;
+        }
+        if (!(pp->mem_fraction >= 0.0 && pp->mem_fraction <= 1.0)) {
+            set_err("mem_fraction must lie in [0, 1] (0 = default 0.6)");
+            return false;
+        }
+        c.B = pp->block_size;
+        c.qb = pp->q_batch;
+        if (pp->mem_fraction > 0.0) c.frac = pp->mem_fraction;
+        c.cap = pp->mem_budget_bytes;
+    }
+    if (c.qb == 0) {
+        const char *e = getenv("ICM_GPU_Q_BATCH");
+        if (e && e[0]) {
+            const int v = atoi(e);
+            if (v >= 1 && v <= 256) c.qb = v;
+        }
+    }
+    if (c.cap == 0) {
+        const char *e = getenv("ICM_HIP_MEM_BUDGET");
+        if (e && e[0]) c.cap = (size_t)strtoull(e, nullptr, 10);
+    }
+    return true;
+}
+
+// Device allocation accounting shared by the plan types: a byte counter, an
+// optional cap (an allocation beyond it fails as an out-of-memory would) and
+// a dry mode that only counts (host-only planning).  A failed hipMalloc's
+// error is cleared so that later launches do not report it.
+struct DevMem {
+    size_t bytes = 0;
+    size_t cap = 0;
+    bool dry = false;
+    template <class T>
+    bool alloc(T **ptr, size_t count) {
+        if (count == 0) count = 1;
+        const size_t b = count * sizeof(T);
+        *ptr = nullptr;
+        if (cap && bytes + b > cap) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "device memory budget exceeded: %.3f GB held, %.3f GB more, cap %.3f GB",
+                     bytes / 1e9, b / 1e9, cap / 1e9);
+            set_err(buf);
+            return false;
+        }
+        if (!dry) {
+            const hipError_t e = hipMalloc((void **)ptr, b);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                *ptr = nullptr;
+                hip_ok(e, "hipMalloc");
+                return false;
+            }
+        }
+        bytes += b;
+        return true;
+    }
+};
 
 inline bool ensure_device() {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -89,7 +89,11 @@ struct IcmHipPlan {
     double *d_h = nullptr;               // optional H output (per-node entry)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    size_t bytes = 0;
+    DevMem mem;                          // device bytes held (cap / dry-run accounting)
+    int retries = 0;                     // q-batch halvings at creation (out-of-memory policy)
+    int maxF = 16;
+    bool spec_ok = true;                 // spectrum cache allowed (memory_strategy != 3)
+    double frac = 0.6;                   // share of free device memory to plan for
     KernelProf prof;
 };
 
@@ -97,22 +101,28 @@ namespace {
 
 template <class T>
 bool dalloc(IcmHipPlan *p, T **ptr, size_t count) {
-    if (count == 0) count = 1;
-    size_t b = count * sizeof(T);
-    if (!hip_ok(hipMalloc((void **)ptr, b), "hipMalloc")) return false;
-    p->bytes += b;
-    return true;
+    return p->mem.alloc(ptr, count);
+}
+
+// Frees every device buffer of the plan (forward, adjoint, subset lists) and
+// clears the byte count; the stream and events stay.
+void free_buffers(IcmHipPlan *p) {
+    auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
+    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_blocks);
+    for (double *&c : p->d_coef) f(c);
+    f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_spec); f(p->d_partial); f(p->d_acc);
+    f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
+    f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    p->q_cap = 0;
+    p->qv = 0;
+    p->spec_bytes = 0;
+    p->mem.bytes = 0;
 }
 
 void plan_free(IcmHipPlan *p) {
     if (!p) return;
-    auto f = [](void *x) { if (x) (void)hipFree(x); };
-    f(p->d_tw); f(p->d_S); f(p->d_perm); f(p->d_payout); f(p->d_nodes); f(p->d_blocks);
-    for (double *c : p->d_coef) f(c);
-    f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_spec); f(p->d_partial); f(p->d_acc);
-    f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
-    f(p->d_dblocks); for (double *c : p->d_dcoef) f(c);
-    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    free_buffers(p);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -162,10 +172,7 @@ bool spec_alloc(IcmHipPlan *p) {
         }
     }
     if (off == 0) return true;
-    if (!dalloc(p, &p->d_spec, (size_t)off)) {
-        (void)hipGetLastError();   // leave no allocation error for later launches to report
-        return false;
-    }
+    if (!dalloc(p, &p->d_spec, (size_t)off)) return false;
     p->spec_bytes = sizeof(double2) * (size_t)off;
     return true;
 }
@@ -174,43 +181,54 @@ void spec_release(IcmHipPlan *p) {
     if (!p->d_spec) return;
     (void)hipFree(p->d_spec);
     p->d_spec = nullptr;
-    p->bytes -= p->spec_bytes;
+    p->mem.bytes -= p->spec_bytes;
     p->spec_bytes = 0;
 }
 
+// Allocates the forward buffers for qb q-planes (spectrum cache if spec).
+bool alloc_buffers(IcmHipPlan *p, int qb, bool spec) {
+    const Geometry &g = p->geo;
+    const int n = g.n, k = g.k, B = g.B;
+    p->qb = qb;
+    // leaf-extract q-chunks: enough workgroups to fill the chip several times
+    // over (about 16k; 4k left a part-filled last round at n = 2^20)
+    const int bpw = 256 / B;
+    const int wg_blocks = (g.nb + bpw - 1) / bpw;
+    const int nchunk = std::max(1, 16384 / std::max(1, wg_blocks));
+    p->nchunk = std::min(nchunk, qb);
+    const long long ge = p->g_elems, be = p->big_elems;
+    bool ok = dalloc(p, &p->d_tw, (size_t)(4096 + p->maxF / 4096)) && dalloc(p, &p->d_S, n) &&
+              dalloc(p, &p->d_perm, n) && dalloc(p, &p->d_payout, k) &&
+              dalloc(p, &p->d_blocks, (size_t)qb * g.nb * (B + 1)) && dalloc(p, &p->d_g[0], (size_t)qb * ge) &&
+              dalloc(p, &p->d_g[1], (size_t)qb * ge) && dalloc(p, &p->d_partial, (size_t)p->nchunk * n) &&
+              dalloc(p, &p->d_acc, n) && dalloc(p, &p->d_equity, n);
+    if (ok && be > 0) ok = dalloc(p, &p->d_bufA, (size_t)qb * be) && dalloc(p, &p->d_bufB, (size_t)qb * be);
+    p->d_coef.assign(g.L, nullptr);
+    for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
+    if (ok && spec) ok = spec_alloc(p);
+    return ok;
+}
+
+// Plan creation.  dry: host-only planning against a simulated device with
+// dry_free bytes free (no device call is made and no buffer is uploaded).
 IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts, bool sort_players,
-                        int max_q = 256) {
-    if (!S || n < 1 || k < 1 || k > n) { set_err("invalid arguments: need 1 <= k <= n"); return nullptr; }
-    for (int i = 0; i < n; ++i) {
+                        int max_q = 256, const IcmHipPlanParams *params = nullptr, bool dry = false,
+                        size_t dry_free = 0) {
+    if ((!S && !dry) || n < 1 || k < 1 || k > n) { set_err("invalid arguments: need 1 <= k <= n"); return nullptr; }
+    for (int i = 0; S && i < n; ++i) {
         if (!(S[i] > 0.0) || !std::isfinite(S[i])) { set_err("stacks must be positive and finite"); return nullptr; }
     }
-    if (!ensure_device()) return nullptr;
+    PlanCfg cfg;
+    if (!plan_config(opts, params, cfg)) return nullptr;
+    if (!dry && !ensure_device()) return nullptr;
     IcmHipPlan *p = new IcmHipPlan();
-    const int forcedB = opts ? opts->force_uncached_fused_levels : 0;
-    const int forcedQ = opts ? opts->force_uncached_cufft_levels : 0;
-    p->verbose = opts && opts->verbose;
-    const int B = pick_B(n, k, forcedB);
+    p->verbose = cfg.verbose;
+    p->spec_ok = cfg.spec_ok;
+    p->frac = cfg.frac;
+    p->mem.dry = dry;
+    const int B = pick_B(n, k, cfg.B);
     if (!make_geometry(n, k, B, p->geo)) { set_err("invalid geometry"); delete p; return nullptr; }
     const Geometry &g = p->geo;
-
-    // Stack-descending order, index tie-break (src/gpu/gpu_api.cu:492-502).
-    p->perm.resize(n);
-    for (int i = 0; i < n; ++i) p->perm[i] = i;
-    if (sort_players)
-        std::sort(p->perm.begin(), p->perm.end(), [S](int a, int b) {
-            return S[a] > S[b] || (S[a] == S[b] && a < b);
-        });
-    p->S_sorted.resize(n);
-    for (int i = 0; i < n; ++i) p->S_sorted[i] = S[p->perm[i]];
-    p->Smax = *std::max_element(S, S + n);
-
-    long long ge = 0, be = 0;
-    const long long pb = plane_bytes(g, &ge, &be);
-    p->g_elems = ge;
-    p->big_elems = be;
-    size_t freeb = 0, totb = 0;
-    if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { plan_free(p); return nullptr; }
-    long long budget = (long long)(0.6 * (double)freeb);
     for (int l = 1; l < g.L; ++l) {
         if (g.F[l] > icm::kMaxLdsF && (long long)g.n_real[l] + (g.n_real[l] + 1) / 2 > 65535) {
             set_err("level too wide for the row-pass grid");
@@ -218,11 +236,37 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
             return nullptr;
         }
     }
-    auto planes = [&](long long per_plane) {
-        int q = (int)std::max<long long>(1, budget / std::max<long long>(per_plane, 1));
-        q = std::min(q, std::max(1, std::min(max_q, 256)));
-        if (forcedQ > 0) q = std::min(q, forcedQ);
-        // launch-shape limits: row passes put (q-plane x array) on grid.y
+
+    if (S) {
+        // Stack-descending order, index tie-break (src/gpu/gpu_api.cu:492-502).
+        p->perm.resize(n);
+        for (int i = 0; i < n; ++i) p->perm[i] = i;
+        if (sort_players)
+            std::sort(p->perm.begin(), p->perm.end(), [S](int a, int b) {
+                return S[a] > S[b] || (S[a] == S[b] && a < b);
+            });
+        p->S_sorted.resize(n);
+        for (int i = 0; i < n; ++i) p->S_sorted[i] = S[p->perm[i]];
+        p->Smax = *std::max_element(S, S + n);
+    }
+
+    long long ge = 0, be = 0;
+    const long long pb = plane_bytes(g, &ge, &be);
+    p->g_elems = ge;
+    p->big_elems = be;
+    size_t freeb = dry_free, totb = 0;
+    if (!dry && !HIP_OK(hipMemGetInfo(&freeb, &totb))) { plan_free(p); return nullptr; }
+    if (cfg.cap) freeb = std::min(freeb, cfg.cap);
+    p->mem.cap = dry ? dry_free : cfg.cap;
+    // per-plan arrays that do not scale with the q-batch: stacks, order,
+    // payout, accumulators, twiddles and at most nchunk_max q-chunk partials
+    const int wgb = (g.nb + 256 / B - 1) / (256 / B);
+    const long long nchunk_max = std::min(256, std::max(1, 16384 / std::max(1, wgb)));
+    const long long fixed = 8LL * n * (4 + nchunk_max) + 8LL * k + 16LL * (4096 + 4096);
+    const long long budget = std::max<long long>(0, (long long)(cfg.frac * (double)freeb) - fixed);
+    const int qmax = std::max(1, std::min(max_q, 256));
+    // launch-shape limit: row passes put (q-plane x array) on grid.y
+    auto shape_cap = [&](int q) {
         for (int l = 1; l < g.L; ++l) {
             if (g.F[l] > icm::kMaxLdsF) {
                 const long long arrays = (long long)g.n_real[l] + (g.n_real[l] + 1) / 2;
@@ -231,50 +275,58 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
         }
         return q;
     };
-    int qb = planes(pb);
+    auto planes = [&](long long per_plane) {
+        const int q = (int)std::max<long long>(1, budget / std::max<long long>(per_plane, 1));
+        return shape_cap(std::min(q, qmax));
+    };
+    int qb = cfg.qb > 0 ? shape_cap(std::min(cfg.qb, qmax)) : planes(pb);
     // The spectrum cache costs HBM per q-plane; take it while at least
     // min(qb, 32) q-planes still fit in one batch.
     p->spec_elems = spec_plane_elems(g);
     bool spec = false;
-    if (p->spec_elems > 0) {
+    if (p->spec_elems > 0 && cfg.spec_ok) {
         const int qs = planes(pb + 16 * p->spec_elems);
-        if (qs >= std::min(qb, 32)) {
+        if (cfg.qb > 0) {
+            spec = qs >= qb;
+        } else if (qs >= std::min(qb, 32)) {
             qb = qs;
             spec = true;
         }
     }
-    p->qb = qb;
     p->sb_max = tuned_schoolbook_max(n, k);
+    p->maxF = 16;
+    for (int l = 1; l < g.L; ++l) p->maxF = std::max(p->maxF, g.F[l]);
+    p->maxF = std::max(p->maxF, icm::kMaxLdsF);
+    p->twShift = ilog2i(p->maxF);
 
-    int maxF = 16;
-    for (int l = 1; l < g.L; ++l) maxF = std::max(maxF, g.F[l]);
-    maxF = std::max(maxF, icm::kMaxLdsF);
-    p->twShift = ilog2i(maxF);
-
-    // leaf-extract q-chunks: enough workgroups to fill the chip several times
-    // over (about 16k; 4k left a part-filled last round at n = 2^20)
-    const int bpw = 256 / B;
-    const int wg_blocks = (g.nb + bpw - 1) / bpw;
-    int nchunk = std::max(1, 16384 / std::max(1, wg_blocks));
-    p->nchunk = std::min(nchunk, qb);
-
-    bool ok = HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
-              HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1));
-    ok = ok && dalloc(p, &p->d_tw, (size_t)(4096 + maxF / 4096)) && dalloc(p, &p->d_S, n) && dalloc(p, &p->d_perm, n) &&
-         dalloc(p, &p->d_payout, k) && dalloc(p, &p->d_blocks, (size_t)qb * g.nb * (B + 1)) &&
-         dalloc(p, &p->d_g[0], (size_t)qb * ge) && dalloc(p, &p->d_g[1], (size_t)qb * ge) &&
-         dalloc(p, &p->d_partial, (size_t)p->nchunk * n) && dalloc(p, &p->d_acc, n) &&
-         dalloc(p, &p->d_equity, n);
-    if (ok && be > 0) ok = dalloc(p, &p->d_bufA, (size_t)qb * be) && dalloc(p, &p->d_bufB, (size_t)qb * be);
-    p->d_coef.assign(g.L, nullptr);
-    for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
-    if (ok && spec) ok = spec_alloc(p);
-    if (!ok) { plan_free(p); return nullptr; }
+    if (!dry && !(HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
+                  HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1)))) {
+        plan_free(p);
+        return nullptr;
+    }
+    // Out-of-memory policy of src/gpu/gpu_memory.cu:267-279: on a failed
+    // allocation free everything, halve the q-batch and try again, at most 4
+    // times.
+    for (p->retries = 0;; ++p->retries) {
+        if (alloc_buffers(p, qb, spec)) break;
+        free_buffers(p);
+        if (qb > 1 && p->retries < 4) {
+            qb = std::max(1, qb / 2);
+            continue;
+        }
+        const std::string why = g_err;
+        char buf[160];
+        snprintf(buf, sizeof buf, "plan allocation failed at q_batch=%d after %d halvings: ", qb, p->retries);
+        set_err(buf + why);
+        plan_free(p);
+        return nullptr;
+    }
+    if (dry) return p;
 
     // twiddles, rounded once from long double: 4096-entry coarse table
     // exp(-2 pi i m / 4096) followed by the maxF/4096-entry fine table
     // exp(-2 pi i f / maxF) (layout documented in fft_lds.hpp)
-    const int nfine = maxF / 4096;
+    const int nfine = p->maxF / 4096;
     std::vector<double2> tw(4096 + nfine);
     const long double PI = 3.141592653589793238462643383279502884L;
     for (int m = 0; m < 4096; ++m) {
@@ -282,17 +334,17 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
         tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
     }
     for (int f = 0; f < nfine; ++f) {
-        long double ang = -2.0L * PI * (long double)f / (long double)maxF;
+        long double ang = -2.0L * PI * (long double)f / (long double)p->maxF;
         tw[4096 + f] = make_double2((double)cosl(ang), (double)sinl(ang));
     }
-    ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
-         HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice)) &&
-         HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    const bool ok = HIP_OK(hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice)) &&
+                    HIP_OK(hipMemcpy(p->d_S, p->S_sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice)) &&
+                    HIP_OK(hipMemcpy(p->d_perm, p->perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     if (!ok) { plan_free(p); return nullptr; }
 
     if (p->verbose) {
-        fprintf(stderr, "[icm_hip] n=%d k=%d B=%d nb=%d L=%d qb=%d nchunk=%d bytes=%.3f GB\n", n, k, B, g.nb,
-                g.L, qb, p->nchunk, p->bytes / 1e9);
+        fprintf(stderr, "[icm_hip] n=%d k=%d B=%d nb=%d L=%d qb=%d nchunk=%d spec=%d retries=%d bytes=%.3f GB\n", n,
+                k, B, g.nb, g.L, p->qb, p->nchunk, p->d_spec ? 1 : 0, p->retries, p->mem.bytes / 1e9);
         for (int l = 0; l < g.L; ++l)
             fprintf(stderr, "  l=%2d n_real=%8d c=%8d G=%8d F=%8d\n", l, g.n_real[l], g.c[l], g.G[l], g.F[l]);
     }
@@ -459,7 +511,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
         memset(stats, 0, sizeof(*stats));
         stats->total_ns = (double)ms * 1e6;
-        stats->peak_vram_bytes = p->bytes;
+        stats->peak_vram_bytes = p->mem.bytes;
         stats->engine = 1;
         stats->B = B;
         stats->uncached_fused_levels = n_lds;
@@ -468,7 +520,18 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
     return 0;
 }
 
+// Frees the adjoint buffers (a failed or partial allocation leaves none).
+void free_vjp(IcmHipPlan *p, size_t bytes_before) {
+    auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
+    f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+    f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    p->qv = 0;
+    p->mem.bytes = bytes_before;
+}
+
 // VJP buffers: tangent block/level products, B adjoint ping-pong, dS arrays.
+// Sized from the plan's memory share of what is free now; on a failed
+// allocation the q-planes are halved, at most 4 times, as at plan creation.
 bool ensure_vjp(IcmHipPlan *p) {
     if (p->qv > 0) return true;
     const Geometry &g = p->geo;
@@ -479,7 +542,8 @@ bool ensure_vjp(IcmHipPlan *p) {
     per += 2 * be * 8;
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) return false;
-    const long long budget = (long long)(0.6 * (double)freeb);
+    if (p->mem.cap) freeb = std::min(freeb, p->mem.cap - std::min(p->mem.cap, p->mem.bytes));
+    const long long budget = (long long)(p->frac * (double)freeb);
     int qv = (int)std::max<long long>(1, budget / std::max<long long>(per, 1));
     qv = std::min(qv, p->qb);
     // the tangent spectrum cache, beside the forward's, while the q-batch
@@ -493,26 +557,36 @@ bool ensure_vjp(IcmHipPlan *p) {
         }
     }
     p->bv_elems = be;
-    bool ok = dalloc(p, &p->d_dblocks, (size_t)qv * g.nb * (g.B + 1)) && dalloc(p, &p->d_bv[0], (size_t)qv * be) &&
-              dalloc(p, &p->d_bv[1], (size_t)qv * be) && dalloc(p, &p->d_v, g.n) &&
-              dalloc(p, &p->d_partial_ds, (size_t)p->nchunk * g.n) && dalloc(p, &p->d_acc_ds, g.n) &&
-              dalloc(p, &p->d_ds, g.n);
-    p->d_dcoef.assign(g.L, nullptr);
-    for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * g.n_real[l] * g.c[l]);
-    if (!ok) return false;
-    if (dspec) {
-        p->dspec_off.assign(g.L, -1);
-        long long off = 0;
-        for (int l = 1; l <= g.L - 2; ++l) {
-            if (spec_level(g, l)) {
-                p->dspec_off[l] = off;
-                off += (long long)qv * g.n_real[l] * g.F[l];
+    const size_t before = p->mem.bytes;
+    for (int attempt = 0;; ++attempt) {
+        bool ok = dalloc(p, &p->d_dblocks, (size_t)qv * g.nb * (g.B + 1)) &&
+                  dalloc(p, &p->d_bv[0], (size_t)qv * be) && dalloc(p, &p->d_bv[1], (size_t)qv * be) &&
+                  dalloc(p, &p->d_v, g.n) && dalloc(p, &p->d_partial_ds, (size_t)p->nchunk * g.n) &&
+                  dalloc(p, &p->d_acc_ds, g.n) && dalloc(p, &p->d_ds, g.n);
+        p->d_dcoef.assign(g.L, nullptr);
+        for (int l = 1; ok && l <= g.L - 2; ++l)
+            ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * g.n_real[l] * g.c[l]);
+        if (ok && dspec) {
+            p->dspec_off.assign(g.L, -1);
+            long long off = 0;
+            for (int l = 1; l <= g.L - 2; ++l) {
+                if (spec_level(g, l)) {
+                    p->dspec_off[l] = off;
+                    off += (long long)qv * g.n_real[l] * g.F[l];
+                }
             }
+            // without the tangent cache the tangent builds take three passes
+            if (off > 0 && !dalloc(p, &p->d_dspec, (size_t)off)) p->d_dspec = nullptr;
         }
-        if (off > 0 && !dalloc(p, &p->d_dspec, (size_t)off)) {
-            (void)hipGetLastError();   // no tangent cache: the three-pass tangent builds
-            p->d_dspec = nullptr;
+        if (ok) break;
+        free_vjp(p, before);
+        if (qv > 1 && attempt < 4) {
+            qv = std::max(1, qv / 2);
+            continue;
         }
+        const std::string why = g_err;
+        set_err("adjoint buffers: " + why);
+        return false;
     }
     p->qv = qv;
     return true;
@@ -719,7 +793,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
         memset(stats, 0, sizeof(*stats));
         stats->total_ns = (double)ms * 1e6;
-        stats->peak_vram_bytes = p->bytes;
+        stats->peak_vram_bytes = p->mem.bytes;
         stats->engine = 1;
         stats->B = B;
     }
@@ -808,10 +882,35 @@ void icm_hip_shutdown(void) {
 
 const char *icm_hip_last_error(void) { return g_err.c_str(); }
 
-int icm_hip_release_pooled_memory(void) { return 0; }
+int icm_hip_release_pooled_memory(void) {
+    if (g_device < 0) { set_err("icm_hip_init must be called first"); return -1; }
+    if (!HIP_OK(hipSetDevice(g_device))) return -1;
+    hipMemPool_t pool = nullptr;
+    if (!HIP_OK(hipDeviceGetDefaultMemPool(&pool, g_device))) return -1;
+    return HIP_OK(hipMemPoolTrimTo(pool, 0)) ? 0 : -1;
+}
 
 IcmHipPlan *icm_hip_plan_create(int n, const double *S, int k, const IcmHipOptions *opts) {
     return plan_create(n, S, k, opts, true);
+}
+
+IcmHipPlan *icm_hip_plan_create_ex(int n, const double *S, int k, const IcmHipOptions *opts,
+                                   const IcmHipPlanParams *params) {
+    return plan_create(n, S, k, opts, true, 256, params);
+}
+
+int icm_hip_plan_dry_run(int n, int k, const IcmHipOptions *opts, const IcmHipPlanParams *params,
+                         IcmHipPlanSummary *summary, int *retries) {
+    if (!params || params->mem_budget_bytes == 0) {
+        set_err("dry run needs params->mem_budget_bytes (the simulated free device memory)");
+        return -1;
+    }
+    IcmHipPlan *p = plan_create(n, nullptr, k, opts, false, 256, params, true, params->mem_budget_bytes);
+    if (!p) return -1;
+    if (summary) icm_hip_plan_summary(p, summary);
+    if (retries) *retries = p->retries;
+    plan_free(p);
+    return 0;
 }
 
 void icm_hip_plan_destroy(IcmHipPlan *plan) { plan_free(plan); }
@@ -827,7 +926,7 @@ int icm_hip_plan_summary(const IcmHipPlan *p, IcmHipPlanSummary *s) {
         else s->n_tier3++;
     }
     s->q_batch = p->qb;
-    s->planned_peak_vram_bytes = p->bytes;
+    s->planned_peak_vram_bytes = p->mem.bytes;
     return 0;
 }
 

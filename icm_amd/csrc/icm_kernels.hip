@@ -667,6 +667,37 @@ __global__ void k_scatter(const double *__restrict__ acc, int n, const int *__re
     equity[perm[j]] = acc[j];
 }
 
+// Streaming probes (measured HBM rates beside the nominal peak): grid-stride
+// loops over n4 double4 units; the read probe leaves one sum per workgroup so
+// its loads are live.
+__global__ void __launch_bounds__(256) k_stream_read(const double4 *__restrict__ src, long long n4,
+                                                     double *__restrict__ part) {
+    __shared__ double red[256];
+    double a = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) {
+        const double4 v = src[i];
+        a += (v.x + v.y) + (v.z + v.w);
+    }
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_stream_write(double4 *__restrict__ dst, long long n4, double v) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
+        dst[i] = make_double4(v, v, v, v);
+}
+
+__global__ void __launch_bounds__(256) k_stream_copy(const double4 *__restrict__ src, double4 *__restrict__ dst,
+                                                     long long n4) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Spectrum helpers.  A complex transform of z = x + i y (x, y real) gives
 // X[K] = (Z[K] + conj(Z[-K]))/2 and Y[K] = (Z[K] - conj(Z[-K]))/(2i).
@@ -2180,6 +2211,18 @@ hipError_t launch_leaf_vjp(const VjpLeafJob &V, int nchunk, hipStream_t s, Prof 
     const double nq = J.nq, B = J.B;
     PF_END(kname("k_leaf_vjp<%d>", J.B), 8.0 * nq * J.nb * (3 * (B + 1) + V.c0) + 16.0 * J.n + 16.0 * nchunk * J.n,
            nq * J.n * (10.0 * B + 40.0));
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_probe(int mode, const double *src, double *dst, long long n4, double *part, int nwg,
+                               hipStream_t s) {
+    if (mode == 0)
+        k_stream_read<<<nwg, 256, 0, s>>>(reinterpret_cast<const double4 *>(src), n4, part);
+    else if (mode == 1)
+        k_stream_write<<<nwg, 256, 0, s>>>(reinterpret_cast<double4 *>(dst), n4, 1.0);
+    else
+        k_stream_copy<<<nwg, 256, 0, s>>>(reinterpret_cast<const double4 *>(src), reinterpret_cast<double4 *>(dst),
+                                          n4);
     return hipGetLastError();
 }
 

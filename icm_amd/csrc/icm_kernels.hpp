@@ -122,6 +122,11 @@ hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equ
 hipError_t launch_leaf_build_vjp(const VjpLeafJob &J, hipStream_t s, Prof *pf);
 hipError_t launch_leaf_vjp(const VjpLeafJob &J, int nchunk, hipStream_t s, Prof *pf);
 hipError_t launch_gather(const double *src, const int *perm, int n, double *dst, hipStream_t s);
+// Streaming probes over n4 double4 units with nwg workgroups of 256 threads:
+// mode 0 reads src (one partial sum per workgroup into part[nwg]), 1 fills
+// dst, 2 copies src to dst.
+hipError_t launch_stream_probe(int mode, const double *src, double *dst, long long n4, double *part, int nwg,
+                               hipStream_t s);
 
 // Fused adjoint levels (F <= 4096): P and dP together; A and B together.
 hipError_t launch_build_lds_t(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s, Prof *pf);

@@ -26,19 +26,37 @@ typedef struct IcmHipPlan IcmHipPlan;
 
 /* Field order and types mirror IcmGpuOptions (src/gpu/icm_gpu.h:12-34) so a
  * caller built against the reference header passes a compatible struct.
- * Fields the reference uses for cuFFT/cuFFTDx tiers have MI355X meanings or
- * are ignored, as noted. */
+ * Fields keep the reference meaning where this engine has the feature and
+ * are otherwise accepted at their defaults and ignored; a value this engine
+ * cannot honour is rejected at plan creation (NULL, icm_hip_last_error()). */
 typedef struct {
     int device_id;                    /* informational; set by the library */
     int use_cufftdx;                  /* ignored (no vendor FFT library is used) */
-    int enable_graphs;                /* ignored by the single-instance path */
+    int enable_graphs;                /* ragged batches: record/replay a HIP graph; ignored by single plans */
     int enable_q_pipeline;            /* ignored: all q-planes of a batch run in one launch per level */
-    int memory_strategy;              /* 0 = auto; others accepted and treated as auto */
-    int force_uncached_fused_levels;  /* reinterpreted: block size B (16, 32 or 64); <= 0 = auto */
-    int force_uncached_cufft_levels;  /* reinterpreted: q-planes per batch; <= 0 = auto */
+    int memory_strategy;              /* 3 = selective recompute: no spectrum cache (the propagate
+                                       * transforms the children again); any other value: device memory
+                                       * per plan, cache on when it fits (src/gpu/gpu_plan.cu:335) */
+    int force_uncached_fused_levels;  /* -1 or 0 = auto; this engine has no per-level cache count, so
+                                       * positive values are rejected */
+    int force_uncached_cufft_levels;  /* -1 or 0 = auto; positive values are rejected (as above) */
     int fast_mode;                    /* ignored */
     int verbose;                      /* print the plan to stderr */
 } IcmHipOptions;
+
+/* Plan parameters that have no IcmGpuOptions field (no reference
+ * counterpart; the reference reads ICM_GPU_Q_BATCH, src/gpu/gpu_api.cu:139-144,
+ * and sizes from a fixed VRAM share, gpu_api.cu:187).  Zero = default.
+ * Environment: ICM_GPU_Q_BATCH sets q_batch when the field is 0;
+ * ICM_HIP_MEM_BUDGET (bytes) sets mem_budget_bytes when the field is 0. */
+typedef struct {
+    int block_size;                   /* 16, 32 or 64; 0 = MI355X tuning table */
+    int q_batch;                      /* q-planes per launch (<= 256); 0 = from the memory budget */
+    double mem_fraction;              /* share of free device memory a plan plans for; 0 = 0.6 */
+    size_t mem_budget_bytes;          /* cap on the plan's device bytes: planning uses
+                                       * min(free, cap) and an allocation beyond the cap fails as if
+                                       * the device had run out; 0 = no cap */
+} IcmHipPlanParams;
 
 /* Mirrors IcmGpuRunStats (src/gpu/icm_gpu.h:36-50). */
 typedef struct {
@@ -75,10 +93,27 @@ typedef struct {
 int icm_hip_init(int device_id);               /* 1 = ok, 0 = failure */
 void icm_hip_shutdown(void);
 const char *icm_hip_last_error(void);
-int icm_hip_release_pooled_memory(void);       /* 0 = ok (no pool is kept) */
+/* Trims the device's default stream-ordered memory pool to 0 (plans
+ * allocate with hipMalloc and free on destroy, so this only returns what
+ * other hipMallocAsync users left pooled).  0 = ok, -1 before icm_hip_init or
+ * when the trim fails (src/gpu/icm_gpu.h:70-74, gpu_api.cu:94-102). */
+int icm_hip_release_pooled_memory(void);
 
-/* ---- plan (src/gpu/icm_gpu.h:79-81) ---- */
+/* ---- plan (src/gpu/icm_gpu.h:79-81) ----
+ * Device memory is allocated once per plan, sized from mem_fraction of free
+ * memory; when an allocation fails the q-batch is halved and the plan
+ * allocated again, up to 4 times (src/gpu/gpu_memory.cu:267-279). */
 IcmHipPlan *icm_hip_plan_create(int n, const double *S, int k, const IcmHipOptions *opts);
+/* As icm_hip_plan_create with IcmHipPlanParams (params may be NULL). */
+IcmHipPlan *icm_hip_plan_create_ex(int n, const double *S, int k, const IcmHipOptions *opts,
+                                   const IcmHipPlanParams *params);
+/* Host-only planning (no device call): runs the planner and the allocation
+ * sequence with its OOM retries against a simulated device whose free memory
+ * is params->mem_budget_bytes (required).  Fills the summary the plan would
+ * have and the number of q-batch halvings; returns 0, or -1 when even one
+ * q-plane does not fit (message in icm_hip_last_error()). */
+int icm_hip_plan_dry_run(int n, int k, const IcmHipOptions *opts, const IcmHipPlanParams *params,
+                         IcmHipPlanSummary *summary, int *retries);
 void icm_hip_plan_destroy(IcmHipPlan *plan);
 int icm_hip_plan_summary(const IcmHipPlan *plan, IcmHipPlanSummary *summary);
 
@@ -142,6 +177,24 @@ int icm_gpu_equity_subset(int n, const double *S, int Q, const double *payout, i
                           double *equity, const int *targets, int n_targets,
                           const IcmHipOptions *opts, IcmHipRunStats *stats);
 
+/* ---- calibration and diagnostics (src/gpu/icm_gpu.h:112-116) ----
+ * Return 1 on success, 0 on failure, as the reference.
+ * hbm: bytes copied per second by 16 device-to-device copies of 1 GiB
+ *   (each byte counted once, as gpu_api.cu:837-860; HBM traffic is twice it).
+ * fused pair: median device time per parent of one LDS-level build launch and
+ *   one LDS-level propagate launch over `batch` parents at transform size
+ *   fft_n (16 .. 4096, power of two), children of fft_n/2 + 1 coefficients;
+ *   the r2c variant reports the same kernels (they already pack two real
+ *   sequences per complex transform). */
+int icm_gpu_measure_hbm_bandwidth_gbps(double *gbps_out);
+int icm_gpu_measure_fused_pair_ns(int fft_n, int batch, int quick, double *build_ns_out, double *corr_ns_out);
+int icm_gpu_measure_fused_r2c_pair_ns(int fft_n, int batch, int quick, double *build_ns_out,
+                                      double *corr_ns_out);
+/* Streaming rates of this library's own kernels over `bytes` per array
+ * (no reference counterpart): read-only (sum), write-only (fill) and copy,
+ * each in HBM bytes moved per second (a copy counts its reads and writes). */
+int icm_hip_measure_stream_gbps(size_t bytes, double *read_gbps, double *write_gbps, double *copy_gbps);
+
 /* ---- CPU-named aliases bound by python/icm/__init__.py:92-119 ---- */
 void icm_init(const char *wisdom_path);        /* wisdom_path ignored; selects device 0 */
 void icm_equity(int n, const double *S, int Q, const double *payout, int k, double *equity);
@@ -195,11 +248,14 @@ int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int 
  * k_i <= 2048, or n_i <= 4096); other instances are rejected at plan
  * creation with a message naming them.  opts: NULL or enable_graphs != 0
  * records the first run for each Q as one HIP graph and replays it;
- * force_uncached_fused_levels = B, force_uncached_cufft_levels = q-planes
- * per pass, verbose.  Calls on one batch plan must not overlap. */
+ * verbose; the other fields as for single plans.  Calls on one batch plan
+ * must not overlap. */
 typedef struct IcmHipBatchPlan IcmHipBatchPlan;
 IcmHipBatchPlan *icm_hip_batch_plan_create(int n_inst, const int *n_off, const double *S, const int *k_off,
                                            const IcmHipOptions *opts);
+/* With IcmHipPlanParams (block size, q-planes per pass, memory budget). */
+IcmHipBatchPlan *icm_hip_batch_plan_create_ex(int n_inst, const int *n_off, const double *S, const int *k_off,
+                                              const IcmHipOptions *opts, const IcmHipPlanParams *params);
 void icm_hip_batch_plan_destroy(IcmHipBatchPlan *plan);
 int icm_hip_batch_equity(IcmHipBatchPlan *plan, int Q, const double *payout, double *equity,
                          IcmHipRunStats *stats);
