@@ -158,7 +158,9 @@ struct KernelProf : icm::Prof {
 
 // Plan configuration from the reference options struct, the engine's own
 // parameters and the environment (ICM_GPU_Q_BATCH as src/gpu/gpu_api.cu:139-144;
-// ICM_HIP_MEM_BUDGET, bytes).  Values this engine cannot honour are errors.
+// ICM_HIP_MEM_BUDGET, bytes; ICM_HIP_MEMORY_STRATEGY overriding
+// memory_strategy, as bench/bench_gpu.cu:60 does for its options).  Values
+// this engine cannot honour are errors.
 struct PlanCfg {
     int B = 0;               // forced block size (0 = tuning table)
     int qb = 0;              // forced q-planes per batch (0 = from the budget)
@@ -210,6 +212,10 @@ inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, Plan
             const int v = atoi(e);
             if (v >= 1 && v <= 256) c.qb = v;
         }
+    }
+    {   // memory strategy override, as the reference bench's ICM_GPU_BENCH_MEMORY_STRATEGY
+        const char *e = getenv("ICM_HIP_MEMORY_STRATEGY");
+        if (e && e[0]) c.spec_ok = atoi(e) != 3;
     }
     if (c.cap == 0) {
         const char *e = getenv("ICM_HIP_MEM_BUDGET");

@@ -48,7 +48,9 @@ typedef struct {
  * counterpart; the reference reads ICM_GPU_Q_BATCH, src/gpu/gpu_api.cu:139-144,
  * and sizes from a fixed VRAM share, gpu_api.cu:187).  Zero = default.
  * Environment: ICM_GPU_Q_BATCH sets q_batch when the field is 0;
- * ICM_HIP_MEM_BUDGET (bytes) sets mem_budget_bytes when the field is 0. */
+ * ICM_HIP_MEM_BUDGET (bytes) sets mem_budget_bytes when the field is 0;
+ * ICM_HIP_MEMORY_STRATEGY overrides IcmHipOptions.memory_strategy (also for
+ * the one-shot entries, which take no options). */
 typedef struct {
     int block_size;                   /* 16, 32 or 64; 0 = MI355X tuning table */
     int q_batch;                      /* q-planes per launch (<= 256); 0 = from the memory budget */

@@ -67,6 +67,42 @@ def test_node_adjoint_vs_exact_differences(n, k):
     assert err < 1e-8
 
 
+@pytest.mark.parametrize("cache", ["1", "3"])
+def test_node_adjoint_saturated_top_levels(cache, monkeypatch):
+    """a_j ~ 1e-5: the top child coefficients (products of 1 - a_j over
+    thousands of players) are O(1), so the wrapped-term fixes and the
+    missing-right-child parents of the four-step levels carry weight.
+    Exact central differences of the oracle; spectrum cache on (memory
+    strategy 1) and off (3)."""
+    monkeypatch.setenv("ICM_HIP_MEMORY_STRATEGY", cache)
+    n = k = 20000
+    rng = np.random.default_rng(20000)
+    a = 1e-5 * rng.uniform(0.2, 2.0, n)
+    a[rng.choice(n, 40, replace=False)] = rng.uniform(0.05, 0.95, 40)
+    c = rng.normal(size=n)
+    p = np.sort(rng.uniform(0, 1, k))[::-1]
+    say(f"node vjp saturated n={n} memory_strategy={cache}")
+    psi, h = icm_amd.node_vjp(a, c, p)
+    ref_psi = O.inner(a, p, engine=O.ENGINE_HYBRID, B=32, fft_min=64)
+    e_psi = float(np.max(np.abs(psi - ref_psi))) / p.max()
+
+    def phi(aa):
+        return float(np.dot(c, O.inner(aa, p, engine=O.ENGINE_HYBRID, B=32, fft_min=64)))
+
+    # Phi is linear in each a_j, so a secant over a wide interval is exact
+    # (a step of order a_j ~ 1e-5 would leave only rounding noise)
+    errs, scale = [], 0.0
+    for j in rng.choice(n, 12, replace=False):
+        ap = a.copy(); ap[j] = 0.75
+        am = a.copy(); am[j] = 0.25
+        fd = (phi(ap) - phi(am)) / 0.5
+        scale = max(scale, abs(fd))
+        errs.append(abs(h[j] - fd))
+    err = max(errs) / max(scale, 1e-300)
+    say(f"  Psi vs oracle {e_psi:.2e}; max |H - FD| / max|FD| = {err:.2e}")
+    assert e_psi < 1e-12 and err < 1e-8
+
+
 @pytest.mark.parametrize("n", [200, 2000])
 def test_vjp_v1_analytic_gradient(n):
     rng = np.random.default_rng(n)

@@ -9,6 +9,7 @@ Both add k = 1024 and 2048 at n >= 65536.  ICM_GPU_VERIFY_EXT_MAX_N caps n
 and ICM_GPU_VERIFY_EXT_LITE=1 trims the extended k grid, as in the
 reference.  Exit status 0 when every shape passes.
 """
+import ctypes
 import os
 import sys
 import time
@@ -20,8 +21,36 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 import icm_amd  # noqa: E402
-from icm_amd.cli import make_payout, make_stacks  # noqa: E402
+from icm_amd.cli import make_payout  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+
+
+def reference_stacks(n, dist):
+    """bench/bench_gpu.cu:25-47 with the same generator: glibc srand(42) /
+    rand() / RAND_MAX, so the stacks are those the reference verifies."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.rand.restype = ctypes.c_int
+    libc.srand(42)
+    rand_max = 2147483647.0
+    if dist == 0:
+        return np.array([1.0 + 99.0 * (libc.rand() / rand_max) for _ in range(n)])
+    if dist == 1:
+        S = np.ones(n)
+        S[0] = 10000.0
+        return S
+    if dist == 2:
+        return 2.0 ** (np.arange(n) * 10.0 / n)
+    if dist == 3:
+        return np.full(n, 100.0)
+    return np.array([1.0 + libc.rand() / rand_max for _ in range(n)])
+
+
+def ext_ks(n, lite=False):
+    """k list of the extended grid (bench/bench_gpu.cu:101-119)."""
+    ks = [min(100, n), max(1, n // 2), n] if lite else [min(16, n), min(100, n), min(512, n), max(1, n // 2), n]
+    if n >= 65536:
+        ks += [1024, 2048]
+    return ks
 
 
 def rel_err_max(gpu, cpu):
@@ -45,14 +74,11 @@ def run_verify(extended, Q=256, log=print):
     icm_amd.init()
     for n in ns:
         for d in dists:
-            S = make_stacks(n, d)
+            S = reference_stacks(n, d)
             if extended:
-                ks = [min(100, n), max(1, n // 2), n] if lite else [min(16, n), min(100, n), min(512, n),
-                                                                     max(1, n // 2), n]
+                ks = ext_ks(n, lite)
             else:
-                ks = [min(100, n), n]
-            if n >= 65536:
-                ks += [1024, 2048]
+                ks = [min(100, n), n] + ([1024, 2048] if n >= 65536 else [])
             for k in ks:
                 p = make_payout(n, k)
                 t0 = time.perf_counter()
