@@ -120,6 +120,8 @@ _SIGS = [
     ("icm_hip_plan_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_set_schoolbook_max", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_schoolbook_max", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_set_small_path", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_get_small_path", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_spectrum_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
@@ -357,6 +359,16 @@ class Plan:
     @property
     def schoolbook_max(self):
         return self._lib.icm_hip_plan_get_schoolbook_max(self._h)
+
+    def set_small_path(self, mode):
+        """-1: tuned choice; 0: never; 1: whenever the tree fits one workgroup."""
+        if self._lib.icm_hip_plan_set_small_path(self._h, int(mode)) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_small_path")
+
+    @property
+    def small_path(self):
+        """True when forward runs go through the one-launch small-instance kernel."""
+        return self._lib.icm_hip_plan_get_small_path(self._h) == 1
 
     def set_spectrum_cache(self, on):
         """Keep (True) or recompute (False) the build's column transforms of

@@ -103,6 +103,53 @@ def tune_batch(Q, reps, log, count=4096):
     return best, results
 
 
+def small_work(n, k):
+    """Schoolbook multiply-adds per quadrature node of the one-launch
+    small-instance kernel (the planner's small_work, same geometry)."""
+    nr, L = [n], 1
+    while (1 << (L - 1)) < n:
+        L += 1
+    for _ in range(1, L):
+        nr.append((nr[-1] + 1) // 2)
+    c = [min((1 << l) + 1, k) for l in range(L)]
+    g = [min(1 << l, k) for l in range(L)]
+    w = sum(nr[l] * c[l] * min(c[l - 1], c[l]) for l in range(1, L - 1))
+    return w + sum(nr[l - 1] * g[l - 1] * c[l - 1] for l in range(1, L))
+
+
+def tune_small(Q, reps, log):
+    """Crossover of the one-launch small-instance path against the
+    multi-launch path, as a bound on small_work(n, k)."""
+    rng = np.random.default_rng(5)
+    wins, losses, rows = [], [], []
+    for n in (64, 256, 1024, 2048, 4096):
+        for k in (1, 3, 10, 30, 100, 300, 1000):
+            if k > n:
+                continue
+            S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+            p = np.ones(k) / k
+            plan = icm_amd.Plan(S, k)
+            plan.set_small_path(1)
+            if not plan.small_path:
+                plan.close()
+                continue
+            t_small = time_plan(plan, p, Q, reps)
+            plan.set_small_path(0)
+            t_multi = time_plan(plan, p, Q, reps)
+            plan.close()
+            w = small_work(n, k)
+            (wins if t_small < t_multi else losses).append(w)
+            rows.append({"n": n, "k": k, "work": w, "small_ms": t_small, "multi_ms": t_multi})
+            log(f"small path n={n} k={k} work={w}: {t_small:.3f} ms vs {t_multi:.3f} ms multi-launch")
+    first_loss = min(losses) if losses else None
+    below = [w for w in wins if first_loss is None or w < first_loss]
+    if not below:
+        return {"max_work": 0.0, "note": "small path never faster"}, rows
+    hi = max(below)
+    bound = math.sqrt(hi * first_loss) if first_loss else 2.0 * hi
+    return {"max_work": bound, "note": f"fastest up to work {hi}, first loss at {first_loss}"}, rows
+
+
 def check_monotone(table):
     """Pass time must not decrease with n (fixed k fraction) or with k (fixed n)."""
     bad = []
@@ -125,7 +172,7 @@ def check_monotone(table):
     return bad
 
 
-def write_header(table, path, batch=None):
+def write_header(table, path, batch=None, small=None):
     ns = sorted({n for n, _ in table})
     lines = ["// tuning_mi355x.h -- planner table for MI355X (gfx950), written by",
              "// icm_amd/autotune.py from timings on the device.  First row whose limits",
@@ -149,6 +196,11 @@ def write_header(table, path, batch=None):
     lines += ["// Ragged batches (one setting per batch), tuned on the 4096-instance ragged mix.",
               "struct BatchTune {", "    int B, sb_max;", "};",
               f"static const BatchTune kBatchTune = {{{b['B']}, {b['sb_max']}}};   // {b['ms']:.3f} ms", ""]
+    sm = small or {"max_work": 262144.0, "note": "default"}
+    lines += ["// Small instances in one launch (k_small): used when the tree fits one",
+              "// workgroup's LDS and its schoolbook multiply-adds per quadrature node are at",
+              "// most this many (crossover against the multi-launch path).",
+              f"static const double kSmallMaxWork = {sm['max_work']:.1f};   // {sm['note']}", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))
 
@@ -172,7 +224,7 @@ def main(argv=None):
         table = {tuple(int(x) for x in key.split(",")): v for key, v in data["best"].items()}
         bad = check_monotone(table)
         log("acceptance: " + ("PASS" if not bad else f"{len(bad)} violations: {bad}"))
-        write_header(table, HEADER, data.get("batch_best"))
+        write_header(table, HEADER, data.get("batch_best"), data.get("small_best"))
         log(f"wrote {HEADER}")
         return 0 if not bad else 1
 
@@ -187,12 +239,14 @@ def main(argv=None):
         log(f"monotonicity: {b}")
     log("acceptance: " + ("PASS (time monotone in n and k)" if not bad else f"{len(bad)} violations"))
     batch_best, batch_all = tune_batch(args.Q, args.reps, log)
+    small_best, small_all = tune_small(args.Q, max(args.reps, 5), log)
     if args.json:
         with open(args.json, "w") as f:
             json.dump({"best": {f"{n},{k}": v for (n, k), v in table.items()}, "all": every,
-                       "monotone_violations": bad, "batch_best": batch_best, "batch_all": batch_all}, f, indent=1)
+                       "monotone_violations": bad, "batch_best": batch_best, "batch_all": batch_all,
+                       "small_best": small_best, "small_all": small_all}, f, indent=1)
     if not args.dry_run:
-        write_header(table, HEADER, batch_best)
+        write_header(table, HEADER, batch_best, small_best)
         log(f"wrote {HEADER}; rebuild with python -m icm_amd.build")
     return 0 if not bad else 1
 

@@ -47,8 +47,18 @@ struct IcmHipPlan {
     double *d_S = nullptr;
     int *d_perm = nullptr;
     double *d_payout = nullptr;          // k (copied per call)
-    double *d_nodes = nullptr;           // [2][Qcap]: logv, w
+    double *d_nodes = nullptr;           // [2][Qcap]: logv, w of the active nodes
     int q_cap = 0;
+    int nodes_Q = 0;                     // Q the uploaded nodes belong to (-1: the raw per-node entry)
+    int nodes_Qa = 0;                    // active nodes (w != 0) among them
+    // small instances: the whole quadrature in one launch (k_small), chosen
+    // when the tree fits one workgroup's LDS and the tuned rule allows it
+    int small_mode = -1;                 // -1 auto, 0 off, 1 whenever it fits
+    bool small_fits = false;
+    bool small_auto = false;             // the tuned rule's choice
+    icm::SmallJob sj{};
+    int small_gmax = 0;                  // workgroups (rows of d_small)
+    double *d_small = nullptr;           // [small_gmax][n] per-workgroup partial equities
     double *d_blocks = nullptr;          // [qb][nb][B+1]
     std::vector<double *> d_coef;        // level l (1..L-2): [qb][n_real][c]
     double *d_g[2] = {nullptr, nullptr};
@@ -114,7 +124,9 @@ void free_buffers(IcmHipPlan *p) {
     f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
     f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
     f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    f(p->d_small);
     p->q_cap = 0;
+    p->nodes_Q = 0;
     p->qv = 0;
     p->spec_bytes = 0;
     p->mem.bytes = 0;
@@ -185,6 +197,43 @@ void spec_release(IcmHipPlan *p) {
     p->spec_bytes = 0;
 }
 
+// Geometry of the small-instance kernel (blocks of one player); returns the
+// LDS doubles it needs (or a value above every tier when it cannot run).
+int small_geometry(int n, int k, icm::SmallJob &J) {
+    J.n = n;
+    J.k = k;
+    int L = 1;
+    while ((1LL << (L - 1)) < n) ++L;
+    J.L = L;
+    if (L > icm::kSmallMaxL) return 1 << 30;
+    long long off = n, gmax = 0;
+    J.nr[0] = n;
+    for (int l = 0; l < L; ++l) {
+        if (l > 0) J.nr[l] = (J.nr[l - 1] + 1) / 2;
+        J.c[l] = (int)std::min<long long>((1LL << l) + 1, k);
+        J.g[l] = (int)std::min<long long>(1LL << l, k);
+        J.off[l] = 0;
+        if (l >= 1 && l <= L - 2) {
+            J.off[l] = (int)off;
+            off += (long long)J.nr[l] * J.c[l];
+        }
+        if (l <= L - 2) gmax = std::max(gmax, (long long)J.nr[l] * J.g[l]);
+    }
+    J.goff[0] = (int)off;
+    J.goff[1] = (int)(off + gmax);
+    const long long need = off + 2 * gmax;
+    J.lds_doubles = (int)std::min<long long>(need, 1 << 30);
+    return J.lds_doubles;
+}
+
+// Schoolbook multiply-adds per quadrature node of the small-instance kernel.
+double small_work(const icm::SmallJob &J) {
+    double w = 0.0;
+    for (int l = 1; l <= J.L - 2; ++l) w += (double)J.nr[l] * J.c[l] * std::min(J.c[l - 1], J.c[l]);
+    for (int l = J.L - 1; l >= 1; --l) w += (double)J.nr[l - 1] * J.g[l - 1] * J.c[l - 1];
+    return w;
+}
+
 // Allocates the forward buffers for qb q-planes (spectrum cache if spec).
 bool alloc_buffers(IcmHipPlan *p, int qb, bool spec) {
     const Geometry &g = p->geo;
@@ -206,6 +255,7 @@ bool alloc_buffers(IcmHipPlan *p, int qb, bool spec) {
     p->d_coef.assign(g.L, nullptr);
     for (int l = 1; ok && l <= g.L - 2; ++l) ok = dalloc(p, &p->d_coef[l], (size_t)qb * g.n_real[l] * g.c[l]);
     if (ok && spec) ok = spec_alloc(p);
+    if (ok && p->small_fits) ok = dalloc(p, &p->d_small, (size_t)p->small_gmax * n);
     return ok;
 }
 
@@ -298,6 +348,16 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     for (int l = 1; l < g.L; ++l) p->maxF = std::max(p->maxF, g.F[l]);
     p->maxF = std::max(p->maxF, icm::kMaxLdsF);
     p->twShift = ilog2i(p->maxF);
+    // small-instance path: one workgroup per quadrature node (two per CU
+    // when the tree fits 78 KB of LDS)
+    p->small_fits = small_geometry(n, k, p->sj) <= icm::kSmallLdsB;
+    if (p->small_fits) {
+        int cus = 256;
+        if (!dry && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device) != hipSuccess)
+            cus = 256;
+        p->small_gmax = std::min(512, std::max(1, cus) * (p->sj.lds_doubles <= icm::kSmallLdsA ? 2 : 1));
+        p->small_auto = small_work(p->sj) <= kSmallMaxWork;
+    }
 
     if (!dry && !(HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) &&
                   HIP_OK(hipEventCreate(&p->ev0)) && HIP_OK(hipEventCreate(&p->ev1)))) {
@@ -353,10 +413,84 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
 
 bool ensure_nodes(IcmHipPlan *p, int Qa) {
     if (Qa <= p->q_cap) return true;
-    if (p->d_nodes) { (void)hipFree(p->d_nodes); p->d_nodes = nullptr; }
+    if (p->d_nodes) {
+        (void)hipFree(p->d_nodes);
+        p->d_nodes = nullptr;
+        p->mem.bytes -= sizeof(double) * 2 * (size_t)p->q_cap;
+    }
+    p->q_cap = 0;
+    p->nodes_Q = 0;
     if (!dalloc(p, &p->d_nodes, (size_t)2 * Qa)) return false;
     p->q_cap = Qa;
     return true;
+}
+
+// Quadrature nodes for Q, compacted to the active ones (w != 0,
+// src/cpu/icm.c:2558): logv at d_nodes[0 .. Qa), w at d_nodes[q_cap ..).
+// Uploaded once per Q (they depend on Q and max S only); the raw per-node
+// entry uses one node with logv = 0, w = 1.  Returns Qa, or -1.
+int upload_nodes(IcmHipPlan *p, int Q) {
+    const int key = p->raw_node ? -1 : Q;
+    if (p->nodes_Q == key && p->d_nodes) return p->nodes_Qa;
+    std::vector<double> logv(Q), w(Q);
+    make_nodes(Q, p->Smax, logv.data(), w.data());
+    if (p->raw_node) { logv.assign(1, 0.0); w.assign(1, 1.0); Q = 1; }
+    std::vector<double> lv_a, w_a;
+    for (int q = 0; q < Q; ++q)
+        if (w[q] != 0.0) { lv_a.push_back(logv[q]); w_a.push_back(w[q]); }
+    const int Qa = (int)lv_a.size();
+    const int Qs = std::max(Qa, 1);
+    if (!ensure_nodes(p, Qs)) return -1;
+    std::vector<double> nodes(2 * (size_t)p->q_cap, 0.0);
+    std::copy(lv_a.begin(), lv_a.end(), nodes.begin());
+    std::copy(w_a.begin(), w_a.end(), nodes.begin() + p->q_cap);
+    if (!HIP_OK(hipMemcpyAsync(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice,
+                               p->stream)) ||
+        !HIP_OK(hipStreamSynchronize(p->stream)))
+        return -1;
+    p->nodes_Q = key;
+    p->nodes_Qa = Qa;
+    return Qa;
+}
+
+bool use_small(const IcmHipPlan *p) {
+    if (!p->small_fits || !p->d_small || p->raw_node || p->d_a_in || p->d_inner) return false;
+    return p->small_mode == 1 || (p->small_mode < 0 && p->small_auto);
+}
+
+// The whole quadrature of a small instance: k_small over the active nodes,
+// then the fixed-order sum over workgroups and the scatter to original order.
+int run_small(IcmHipPlan *p, int Qa, const double *d_payout, IcmHipRunStats *stats) {
+    hipStream_t s = p->stream;
+    icm::SmallJob J = p->sj;
+    J.S = p->d_S;
+    J.logv = p->d_nodes;
+    J.w = p->d_nodes + p->q_cap;
+    J.payout = d_payout;
+    J.partial = p->d_small;
+    J.Qa = Qa;
+    const int G = std::max(1, std::min(Qa, p->small_gmax));
+    if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
+    if (Qa > 0 && !hip_ok(icm::launch_small(J, G, s, &p->prof, 2.0 * small_work(J) * Qa), "small-instance kernel"))
+        return -1;
+    if (!hip_ok(icm::launch_small_finalize(p->d_small, Qa > 0 ? G : 0, J.n, p->d_active, p->d_perm, p->d_acc,
+                                           p->d_equity, s, &p->prof),
+                "small-instance finalize"))
+        return -1;
+    if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (p->prof.enabled) p->prof.collect();
+    if (stats) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+        memset(stats, 0, sizeof(*stats));
+        stats->total_ns = (double)ms * 1e6;
+        stats->peak_vram_bytes = p->mem.bytes;
+        stats->engine = 1;
+        stats->B = 1;
+        stats->uncached_fused_levels = J.L - 1;
+    }
+    return 0;
 }
 
 // Level l (1 .. L-1) runs as schoolbook sums when its children keep few
@@ -374,25 +508,12 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
     const Geometry &g = p->geo;
     if (Q < 1) { set_err("Q must be positive"); return -1; }
     if (!HIP_OK(hipSetDevice(g_device))) return -1;
-    // nodes (host), compacted to the active ones (w != 0, src/cpu/icm.c:2558)
-    std::vector<double> logv(Q), w(Q);
-    make_nodes(Q, p->Smax, logv.data(), w.data());
-    if (p->raw_node) { logv.assign(1, 0.0); w.assign(1, 1.0); Q = 1; }
-    std::vector<double> nodes;
-    std::vector<double> lv_a, w_a;
-    for (int q = 0; q < Q; ++q)
-        if (w[q] != 0.0) { lv_a.push_back(logv[q]); w_a.push_back(w[q]); }
-    const int Qa = (int)lv_a.size();
-    if (!ensure_nodes(p, std::max(Qa, 1))) return -1;
-    nodes.resize(2 * (size_t)std::max(Qa, 1));
-    std::copy(lv_a.begin(), lv_a.end(), nodes.begin());
-    std::copy(w_a.begin(), w_a.end(), nodes.begin() + std::max(Qa, 1));
+    const int Qa = upload_nodes(p, Q);
+    if (Qa < 0) return -1;
+    if (use_small(p)) return run_small(p, Qa, d_payout, stats);
     hipStream_t s = p->stream;
-    if (!HIP_OK(hipMemcpyAsync(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice, s)))
-        return -1;
-    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
     const double *d_lv = p->d_nodes;
-    const double *d_w = p->d_nodes + std::max(Qa, 1);
+    const double *d_w = p->d_nodes + p->q_cap;
 
     if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
     if (!HIP_OK(hipMemsetAsync(p->d_acc, 0, sizeof(double) * g.n, s))) return -1;
@@ -599,23 +720,10 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
     if (Q < 1) { set_err("Q must be positive"); return -1; }
     if (!HIP_OK(hipSetDevice(g_device))) return -1;
     if (!ensure_vjp(p)) return -1;
-    std::vector<double> logv(Q), w(Q);
-    make_nodes(Q, p->Smax, logv.data(), w.data());
-    if (p->raw_node) { logv.assign(1, 0.0); w.assign(1, 1.0); Q = 1; }
-    std::vector<double> lv_a, w_a;
-    for (int q = 0; q < Q; ++q)
-        if (w[q] != 0.0) { lv_a.push_back(logv[q]); w_a.push_back(w[q]); }
-    const int Qa = (int)lv_a.size();
-    const int Qs = std::max(Qa, 1);
-    if (!ensure_nodes(p, Qs)) return -1;
-    std::vector<double> nodes(2 * (size_t)Qs, 0.0);
-    std::copy(lv_a.begin(), lv_a.end(), nodes.begin());
-    std::copy(w_a.begin(), w_a.end(), nodes.begin() + Qs);
+    const int Qa = upload_nodes(p, Q);
+    if (Qa < 0) return -1;
     hipStream_t s = p->stream;
-    if (!HIP_OK(hipMemcpyAsync(p->d_nodes, nodes.data(), sizeof(double) * nodes.size(), hipMemcpyHostToDevice, s)))
-        return -1;
-    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
-    const double *d_lv = p->d_nodes, *d_w = p->d_nodes + Qs;
+    const double *d_lv = p->d_nodes, *d_w = p->d_nodes + p->q_cap;
     if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
     if (d_v_orig) {
         if (!hip_ok(icm::launch_gather(d_v_orig, p->d_perm, g.n, p->d_v, s), "gather")) return -1;
@@ -1136,6 +1244,18 @@ int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax) {
 }
 
 int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan) { return plan ? plan->sb_max : -1; }
+
+int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (mode < -1 || mode > 1) { set_err("small-path mode must be -1 (tuned), 0 (off) or 1 (on)"); return -1; }
+    plan->small_mode = mode;
+    return 0;
+}
+
+int icm_hip_plan_get_small_path(const IcmHipPlan *plan) {
+    if (!plan) return -1;
+    return use_small(plan) ? 1 : 0;
+}
 
 int icm_hip_plan_set_spectrum_cache(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }

@@ -649,15 +649,54 @@ __global__ void k_gather(const double *__restrict__ src, const int *__restrict__
     if (j < n) dst[j] = src[perm[j]];
 }
 
-__global__ void k_finalize(const double *__restrict__ partial, int nchunk, int n,
-                           const unsigned char *__restrict__ active, double *__restrict__ acc,
-                           int accumulate) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// Fixed-order sums over the rows of partial[rows][n]: 64 columns per
+// workgroup, RG row groups (row b goes to group b mod RG, four independent
+// running sums per group), groups combined in order through LDS.  The same
+// association on every run: deterministic.  Optionally adds acc (the running
+// total of earlier q-batches), masks inactive players, scatters to the
+// original order.
+template <int RG>
+__global__ __launch_bounds__(64 * RG) void k_colsum(const double *__restrict__ partial, int rows, int n,
+                                                    const unsigned char *__restrict__ active,
+                                                    double *__restrict__ acc, int accumulate,
+                                                    const int *__restrict__ perm, double *__restrict__ equity) {
+    __shared__ double red[RG][64];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (j < n) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int b = grp;
+        for (; b + 3 * RG < rows; b += 4 * RG) {
+            s0 += partial[(long long)b * n + j];
+            s1 += partial[(long long)(b + RG) * n + j];
+            s2 += partial[(long long)(b + 2 * RG) * n + j];
+            s3 += partial[(long long)(b + 3 * RG) * n + j];
+        }
+        for (; b < rows; b += RG) s0 += partial[(long long)b * n + j];
+        s = (s0 + s1) + (s2 + s3);
+    }
+    if (RG > 1) {
+        red[grp][lane] = s;
+        __syncthreads();
+        if (grp != 0) return;
+        for (int r = 1; r < RG; ++r) s += red[r][lane];
+    }
     if (j >= n) return;
-    double s = accumulate ? acc[j] : 0.0;
-    for (int c = 0; c < nchunk; ++c) s += partial[(long long)c * n + j];
+    if (accumulate) s += acc[j];
     if (active && !active[j]) s = 0.0;
     acc[j] = s;
+    if (equity) equity[perm[j]] = s;
+}
+
+static hipError_t colsum(const double *partial, int rows, int n, const unsigned char *active, double *acc,
+                         int accumulate, const int *perm, double *equity, hipStream_t s) {
+    const int blocks = (n + 63) / 64;
+    if (rows <= 2)
+        k_colsum<1><<<blocks, 64, 0, s>>>(partial, rows, n, active, acc, accumulate, perm, equity);
+    else
+        k_colsum<8><<<blocks, 512, 0, s>>>(partial, rows, n, active, acc, accumulate, perm, equity);
+    return hipGetLastError();
 }
 
 __global__ void k_scatter(const double *__restrict__ acc, int n, const int *__restrict__ perm,
@@ -666,6 +705,84 @@ __global__ void k_scatter(const double *__restrict__ acc, int n, const int *__re
     if (j >= n) return;
     equity[perm[j]] = acc[j];
 }
+
+// ---------------------------------------------------------------------------
+// Small instances: one workgroup per quadrature node, the whole tree in LDS
+// (blocks of one player, schoolbook sums per level).  Level 0 is the factor
+// a_j + (1 - a_j) x of each player (a_j kept at LDS offset 0); levels
+// 1 .. L-2 are products truncated to c[l] coefficients; the propagate runs
+// g from the payout at the root down to one entry per player, which is
+// Psi_j (a level-0 node's g pairs with the empty leave-one-out product).
+// The same sums as the reference's single-kernel path
+// (src/gpu/gpu_kernels.cu:1547-1668) on this engine's ragged geometry, with
+// the weighted contributions kept per workgroup instead of atomics.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double small_coef(const double *sm, const SmallJob &J, int l, int node, int t) {
+    if (l == 0) return t == 0 ? sm[node] : 1.0 - sm[node];
+    return sm[J.off[l] + node * J.c[l] + t];
+}
+
+template <int LDS>
+__global__ __launch_bounds__(kSmallThreads) void k_small(SmallJob J) {
+    __shared__ double sm[LDS];
+    const int T = kSmallThreads, tid = threadIdx.x, n = J.n, L = J.L;
+    double *row = J.partial + (long long)blockIdx.x * n;
+    for (int j = tid; j < n; j += T) row[j] = 0.0;   // each j stays with one thread below
+    for (int q = blockIdx.x; q < J.Qa; q += gridDim.x) {
+        const double lv = J.logv[q];
+        for (int j = tid; j < n; j += T) sm[j] = a_of(J.S[j], lv);
+        __syncthreads();
+        for (int l = 1; l <= L - 2; ++l) {           // build: parent = left * right mod x^c[l]
+            const int cc = J.c[l - 1], cp = J.c[l], nrc = J.nr[l - 1], nrp = J.nr[l];
+            double *P = sm + J.off[l];
+            for (int e = tid; e < nrp * cp; e += T) {
+                const int j = e / cp, m = e - j * cp, li = 2 * j, ri = 2 * j + 1;
+                double acc = 0.0;
+                if (ri >= nrc) {
+                    acc = (m < cc) ? small_coef(sm, J, l - 1, li, m) : 0.0;
+                } else {
+                    const int t0 = max(0, m - cc + 1), t1 = min(m, cc - 1);
+                    for (int t = t0; t <= t1; ++t)
+                        acc += small_coef(sm, J, l - 1, li, t) * small_coef(sm, J, l - 1, ri, m - t);
+                }
+                P[e] = acc;
+            }
+            __syncthreads();
+        }
+        int src = -1, dst = 0;                        // g levels: -1 = the payout (root)
+        for (int l = L - 1; l >= 1; --l) {            // propagate: child g = sibling (x) parent g
+            const int Gp = (src < 0) ? J.k : J.g[l];
+            const double *gp = (src < 0) ? J.payout : sm + J.goff[src];
+            double *gc = sm + J.goff[dst];
+            const int nrc = J.nr[l - 1], cc = J.c[l - 1], gcn = J.g[l - 1];
+            for (int e = tid; e < nrc * gcn; e += T) {
+                const int i = e / gcn, m = e - i * gcn, sib = i ^ 1;
+                const double *gpar = gp + (i >> 1) * Gp;
+                double acc;
+                if (sib >= nrc) {
+                    acc = gpar[m];                    // lone child: the missing sibling is 1
+                } else {
+                    acc = 0.0;
+                    const int tmax = min(cc, Gp - m);
+                    for (int t = 0; t < tmax; ++t) acc += small_coef(sm, J, l - 1, sib, t) * gpar[m + t];
+                }
+                gc[e] = acc;
+            }
+            __syncthreads();
+            src = dst;
+            dst ^= 1;
+        }
+        const double ivv = exp(-lv), wq = J.w[q];
+        for (int j = tid; j < n; j += T) {
+            const double psi = (src < 0) ? J.payout[0] : sm[J.goff[src] + j];
+            double pw = wq * J.S[j] * sm[j] * ivv;    // w_q S_j a_j exp(-logv_q), non-finite -> 0
+            if (!isfinite(pw)) pw = 0.0;
+            row[j] += pw * psi;
+        }
+        __syncthreads();                              // before the next node overwrites the tree
+    }
+}
+
 
 // Streaming probes (measured HBM rates beside the nominal peak): grid-stride
 // loops over n4 double4 units; the read probe leaves one sum per workgroup so
@@ -2214,6 +2331,26 @@ hipError_t launch_leaf_vjp(const VjpLeafJob &V, int nchunk, hipStream_t s, Prof 
     return hipGetLastError();
 }
 
+hipError_t launch_small(const SmallJob &J, int G, hipStream_t s, Prof *pf, double flops) {
+    if (J.lds_doubles > kSmallLdsB || J.L > kSmallMaxL || G < 1) return hipErrorInvalidValue;
+    PF_BEGIN();
+    if (J.lds_doubles <= kSmallLdsA)
+        k_small<kSmallLdsA><<<G, kSmallThreads, 0, s>>>(J);
+    else
+        k_small<kSmallLdsB><<<G, kSmallThreads, 0, s>>>(J);
+    PF_END(J.lds_doubles <= kSmallLdsA ? "k_small<78KB>" : "k_small<152KB>",
+           8.0 * ((double)J.n * 2 + 2.0 * J.Qa + J.k + (double)G * J.n), flops);
+    return hipGetLastError();
+}
+
+hipError_t launch_small_finalize(const double *partial, int G, int n, const unsigned char *active, const int *perm,
+                                 double *acc, double *equity, hipStream_t s, Prof *pf) {
+    PF_BEGIN();
+    const hipError_t e = colsum(partial, G, n, active, acc, 0, perm, equity, s);
+    PF_END("k_small_finalize", 8.0 * n * ((double)G + 2.5), (double)n * G);
+    return e;
+}
+
 hipError_t launch_stream_probe(int mode, const double *src, double *dst, long long n4, double *part, int nwg,
                                hipStream_t s) {
     if (mode == 0)
@@ -2234,9 +2371,9 @@ hipError_t launch_gather(const double *src, const int *perm, int n, double *dst,
 hipError_t launch_finalize(const double *partial, int nchunk, int n, const unsigned char *active, double *acc,
                            int accumulate, hipStream_t s, Prof *pf) {
     PF_BEGIN();
-    k_finalize<<<(n + 255) / 256, 256, 0, s>>>(partial, nchunk, n, active, acc, accumulate);
+    const hipError_t e = colsum(partial, nchunk, n, active, acc, accumulate, nullptr, nullptr, s);
     PF_END("k_finalize", 8.0 * n * (nchunk + 2), (double)n * nchunk);
-    return hipGetLastError();
+    return e;
 }
 
 hipError_t launch_scatter(const double *acc, int n, const int *perm, double *equity, hipStream_t s, Prof *pf) {

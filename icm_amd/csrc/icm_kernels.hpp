@@ -215,4 +215,36 @@ hipError_t launch_prop_sb(const LevelJob &J, hipStream_t s, Prof *pf);
 hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, hipStream_t s,
                                  Prof *pf, double bytes, double flops);
 
+// Small instances in one launch: one workgroup per quadrature node (a
+// workgroup loops over nodes b, b + G, ... when there are more nodes than
+// workgroups), the whole tree with blocks of one player in LDS (product
+// levels 1 .. L-2, two g levels), schoolbook sums per level, Psi_j = g_j[0]
+// at the leaves, and the weighted contribution added to the workgroup's own
+// row of partial[G][n] in node order (deterministic).  Geometry: B = 1,
+// n_real[l] = ceil(n_real[l-1] / 2), c[l] = min(2^l + 1, k),
+// g[l] = min(2^l, k); off[l]: LDS offset (doubles) of level l's products
+// (l = 1 .. L-2); a_j at offset 0; the g levels at goff[0 / 1].
+constexpr int kSmallMaxL = 16;
+constexpr int kSmallThreads = 1024;
+struct SmallJob {
+    const double *S;        // sorted stacks [n]
+    const double *logv;     // [Qa]
+    const double *w;        // [Qa]
+    const double *payout;   // [k]
+    double *partial;        // [G][n] (sorted order)
+    int n, k, L, Qa;
+    int nr[kSmallMaxL], c[kSmallMaxL], g[kSmallMaxL], off[kSmallMaxL];
+    int goff[2];
+    int lds_doubles;        // LDS the geometry needs
+};
+// LDS tiers of the small-instance kernel (doubles): two workgroups per CU,
+// or one workgroup with most of the CU's 160 KB.
+constexpr int kSmallLdsA = 10000;
+constexpr int kSmallLdsB = 19456;
+// G workgroups; partial rows are overwritten (not accumulated).
+hipError_t launch_small(const SmallJob &J, int G, hipStream_t s, Prof *pf, double flops);
+// equity[perm[j]] = sum_{b < G} partial[b][j] in order (0 for inactive j).
+hipError_t launch_small_finalize(const double *partial, int G, int n, const unsigned char *active, const int *perm,
+                                 double *acc, double *equity, hipStream_t s, Prof *pf);
+
 }  // namespace icm

@@ -53,3 +53,8 @@ struct BatchTune {
     int B, sb_max;
 };
 static const BatchTune kBatchTune = {64, 0};   // 35.522 ms
+
+// Small instances in one launch (k_small): used when the tree fits one
+// workgroup's LDS and its schoolbook multiply-adds per quadrature node are at
+// most this many (crossover against the multi-launch path).
+static const double kSmallMaxWork = 400000.0;   // n=1024: k=100 (464k) ties, k=30 (158k) 1.6x faster

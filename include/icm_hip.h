@@ -226,6 +226,14 @@ int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on);
  * comes from the MI355X tuning table. */
 int icm_hip_plan_set_schoolbook_max(IcmHipPlan *plan, int cmax);
 int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan);
+/* Small instances in one launch (the counterpart of the reference's
+ * single-kernel path, src/gpu/gpu_kernels.cu:1547-1668, dispatched at
+ * src/gpu/gpu_api.cu:488-610): one workgroup per quadrature node holds the
+ * whole tree in LDS; used when it fits (about 150 KB) and the MI355X tuning
+ * table's crossover allows.  mode -1 = tuned (default), 0 = never, 1 =
+ * whenever it fits.  get returns 1 when forward runs take this path. */
+int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode);
+int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
 /* Spectrum cache (no counterpart in the reference, whose large levels are
  * cuFFT plans per level, src/gpu/gpu_api.cu): the column transforms of each
  * four-step build level's child pairs stay in HBM until the propagate of the
