@@ -103,6 +103,7 @@ _SIGS = [
     ("icm_gpu_measure_fused_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("icm_gpu_measure_fused_r2c_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("icm_hip_measure_stream_gbps", ctypes.c_int, [ctypes.c_size_t, _D, _D, _D]),
+    ("icm_hip_equity_noalloc", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
     ("icm_init", None, [ctypes.c_char_p]),
     ("icm_equity", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
     ("icm_equity_subset", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D, _I, ctypes.c_int]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libicm_hip.so")
-SOURCES = ["icm_kernels.hip", "icm_host.cpp", "icm_batch.cpp", "icm_probe.cpp"]
+SOURCES = ["icm_kernels.hip", "icm_host.cpp", "icm_batch.cpp", "icm_probe.cpp", "icm_noalloc.hip"]
 ARCH = os.environ.get("ICM_HIP_ARCH", "gfx950")
 
 

@@ -197,35 +197,6 @@ void spec_release(IcmHipPlan *p) {
     p->spec_bytes = 0;
 }
 
-// Geometry of the small-instance kernel (blocks of one player); returns the
-// LDS doubles it needs (or a value above every tier when it cannot run).
-int small_geometry(int n, int k, icm::SmallJob &J) {
-    J.n = n;
-    J.k = k;
-    int L = 1;
-    while ((1LL << (L - 1)) < n) ++L;
-    J.L = L;
-    if (L > icm::kSmallMaxL) return 1 << 30;
-    long long off = n, gmax = 0;
-    J.nr[0] = n;
-    for (int l = 0; l < L; ++l) {
-        if (l > 0) J.nr[l] = (J.nr[l - 1] + 1) / 2;
-        J.c[l] = (int)std::min<long long>((1LL << l) + 1, k);
-        J.g[l] = (int)std::min<long long>(1LL << l, k);
-        J.off[l] = 0;
-        if (l >= 1 && l <= L - 2) {
-            J.off[l] = (int)off;
-            off += (long long)J.nr[l] * J.c[l];
-        }
-        if (l <= L - 2) gmax = std::max(gmax, (long long)J.nr[l] * J.g[l]);
-    }
-    J.goff[0] = (int)off;
-    J.goff[1] = (int)(off + gmax);
-    const long long need = off + 2 * gmax;
-    J.lds_doubles = (int)std::min<long long>(need, 1 << 30);
-    return J.lds_doubles;
-}
-
 // Schoolbook multiply-adds per quadrature node of the small-instance kernel.
 double small_work(const icm::SmallJob &J) {
     double w = 0.0;

@@ -197,6 +197,19 @@ int icm_gpu_measure_fused_r2c_pair_ns(int fft_n, int batch, int quick, double *b
  * each in HBM bytes moved per second (a copy counts its reads and writes). */
 int icm_hip_measure_stream_gbps(size_t bytes, double *read_gbps, double *write_gbps, double *copy_gbps);
 
+/* ---- no device allocation ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): computes what icm_hip_equity computes, for instances
+ * that fit the one-launch small-instance kernel, with all device memory taken
+ * from a static device array of the library (8.5 MB per GPU, held until the
+ * process ends).  Limits: n <= 4096, Q <= 2048, and the tree must fit one
+ * workgroup's LDS (about 150 KB: e.g. k <= 100 at n = 1024); larger inputs
+ * return -1 with a message.  Host pointers, default stream, returns after
+ * the result is copied back.  Calls that overlap on one GPU race (they share
+ * that GPU's array). */
+int icm_hip_equity_noalloc(int n, const double *S, int Q, const double *payout, int k, double *equity);
+
 /* ---- CPU-named aliases bound by python/icm/__init__.py:92-119 ---- */
 void icm_init(const char *wisdom_path);        /* wisdom_path ignored; selects device 0 */
 void icm_equity(int n, const double *S, int Q, const double *payout, int k, double *equity);
