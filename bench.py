@@ -8,18 +8,24 @@ region is exactly --steps passes bracketed by device synchronisation.
 
 Prints ONE JSON line on stdout.  Extra fields:
   roofline       dominant kernel (by summed device time in the timed steps):
-                 compulsory bytes (or FLOPs) per launch / mean launch time,
-                 both measured live with HIP events on the plan's stream.
-  path_roofline  the whole pass against the implementation-independent byte
-                 model of the reference path (block size 32 geometry).
+                 algorithmic bytes of the reference-path byte model (block
+                 size 32 geometry) for the tree levels its launches serve /
+                 its mean launch time (HIP events on the plan's stream), per
+                 level the same over all of that level's launches, its own
+                 compulsory bytes as a second fraction, and the device's
+                 measured streaming rates beside the nominal 8 TB/s.
+  path_roofline  the whole pass against the same byte model.
+  configs_1_2    config 1 (n=1024, k=10) and config 2 (n=65536, k=n), each
+                 with its CPU leg (the oracle) timed in the same run.
   cpu_baseline   the CPU oracle (oracle/, a port of the reference CPU engine)
                  on a bounded sample of quadrature nodes of the same workload,
-                 extrapolated to Q = 256.
+                 extrapolated to Q = 256, on all host threads and on one.
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -53,10 +59,11 @@ def next_pow2(v):
     return p
 
 
-def path_model_bytes(n, k, B=32):
-    """HBM bytes per call-node of the reference path's level-materialising
+def path_model(n, k, B=32):
+    """Per-call-node HBM bytes of the reference path's level-materialising
     model: reference geometry (src/cpu/icm.c:1205-1263) with block size 32,
-    every level read and written once."""
+    every level read and written once.  Returns {"total", "fixed",
+    "build": {l: bytes}, "prop": {l: bytes}} (levels of that geometry)."""
     nb = (n + B - 1) // B
     N = next_pow2(nb)
     L = int(math.log2(N)) + 1
@@ -75,15 +82,46 @@ def path_model_bytes(n, k, B=32):
         peff[l] = cps // 2 + 1 if below[l] else cps
         mx = (psz[l - 1] + cps // 2) if below[l] else psz[l]
         geff[l] = min(g[l - 1] + peff[l] - 1, mx)
-    by = 3 * n + 2 * nb * 33 + nb * g[0]
-    for l in range(1, L - 1):
-        by += r[l - 1] * psz[l - 1] + r[l] * psz[l]
-    for l in range(1, L):
-        by += r[l] * geff[l] + r[l - 1] * (peff[l] + g[l - 1])
-    return 8.0 * by
+    fixed = 8.0 * (3 * n + 2 * nb * 33 + nb * g[0])
+    build = {l: 8.0 * (r[l - 1] * psz[l - 1] + r[l] * psz[l]) for l in range(1, L - 1)}
+    prop = {l: 8.0 * (r[l] * geff[l] + r[l - 1] * (peff[l] + g[l - 1])) for l in range(1, L)}
+    return {"total": fixed + sum(build.values()) + sum(prop.values()), "fixed": fixed, "build": build, "prop": prop}
+
+
+def path_model_bytes(n, k, B=32):
+    return path_model(n, k, B)["total"]
+
+
+def host_info():
+    """lscpu model, sockets, cores and threads of the host running the CPU leg."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None, "threads_per_core": None,
+            "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core"}
+        for line in out.splitlines():
+            k_, _, v = line.partition(":")
+            if k_.strip() in keys:
+                v = v.strip()
+                info[keys[k_.strip()]] = int(v) if v.isdigit() else v
+    except Exception as e:  # reported, not fatal
+        info["error"] = str(e)
+    return info
+
+
+def fft_backend():
+    """The oracle's products use its own radix-2 FFT; FFTW is named when the
+    host has it (it is not linked: the oracle never loads it)."""
+    import ctypes.util
+    lib = ctypes.util.find_library("fftw3")
+    return "own radix-2 FFT (oracle/icm_oracle.c)" + (f"; {lib} present, unused" if lib else "; no FFTW on host")
 
 
 def cpu_baseline(n, k, Q, S, p, nthreads):
+    """The oracle on a bounded sample of quadrature nodes of the workload, on
+    nthreads host threads (one node per thread) and on one thread (one node),
+    each extrapolated to Q nodes."""
     from oracle import oracle as O
     q_lo = Q // 2 - nthreads // 2
     q_hi = q_lo + nthreads
@@ -91,9 +129,133 @@ def cpu_baseline(n, k, Q, S, p, nthreads):
     O.equity(S, p, Q=Q, engine=O.ENGINE_HYBRID, B=32, fft_min=64, q_lo=q_lo, q_hi=q_hi, nthreads=nthreads)
     dt = time.perf_counter() - t0
     t_full = dt * Q / (q_hi - q_lo)
+    t1 = time.perf_counter()
+    O.equity(S, p, Q=Q, engine=O.ENGINE_HYBRID, B=32, fft_min=64, q_lo=Q // 2, q_hi=Q // 2 + 1, nthreads=1)
+    dt1 = time.perf_counter() - t1
     return {"value": n / t_full, "unit": "player-equities/s", "cores": nthreads, "kind": "port",
             "sample": f"quadrature nodes {q_lo}..{q_hi - 1} of {Q} at n={n} k={k} "
-                      f"({q_hi - q_lo} nodes, one per thread, {dt:.1f} s), extrapolated to Q={Q}"}
+                      f"({q_hi - q_lo} nodes, one per thread, {dt:.1f} s), extrapolated to Q={Q}",
+            "single_thread": {"value": n / (dt1 * Q), "unit": "player-equities/s",
+                              "sample": f"quadrature node {Q // 2} on 1 thread ({dt1:.1f} s), extrapolated to Q={Q}"},
+            "engine": "hybrid block + subproduct tree (B=32), products with >= 64 terms by FFT",
+            "fft_backend": fft_backend(), "host": host_info()}
+
+
+def roofline_fields(launches, steps, n, k, B, Q, peak_gbs):
+    """Roofline of the dominant kernel (by summed device time over the timed
+    steps) from the per-launch log [(name, level, ns, alg_bytes)]:
+      frac            algorithmic bytes of the reference-path model for the
+                      levels the kernel's launches serve / their time / peak;
+      frac_compulsory the kernel's own compulsory bytes / time / peak;
+      levels          per propagate level served: model bytes over the time
+                      of all of that level's launches (column + row + column
+                      passes of the four-step levels)."""
+    tot = {}
+    for name, lv, ns, by in launches:
+        t = tot.setdefault(name, [0, 0.0, 0.0])
+        t[0] += 1
+        t[1] += ns
+        t[2] += by
+    name, (cnt, ns, by) = max(tot.items(), key=lambda kv: kv[1][1])
+    model = path_model(n, k, 32)
+    shift = int(round(math.log2(B / 32)))     # this engine's level l = the model's level l + shift
+    def level_model(lv):              # model bytes of one level for all Q nodes (one pass)
+        lm = abs(lv) + shift
+        return Q * (model["prop"].get(lm, 0.0) if lv < 0 else model["build"].get(lm, 0.0))
+
+    served = sorted({lv for nm, lv, _, _ in launches if nm == name})
+    # every timed step runs each served level once (in one launch per q-batch)
+    per_launch_model = steps * sum(level_model(lv) for lv in served) / cnt
+    # per level: every launch with that level tag, summed over the timed steps
+    levels = {}
+    for lv in served:
+        t_lv = sum(t for nm, l2, t, b in launches if l2 == lv) / steps
+        levels[str(lv)] = {"model_GB": level_model(lv) / 1e9, "ms": t_lv / 1e6,
+                           "frac": level_model(lv) / (t_lv * 1e-9) / 1e9 / peak_gbs,
+                           "kernels": sorted({nm for nm, l2, _, _ in launches if l2 == lv})}
+    avg_s = ns / cnt * 1e-9
+    return name, {
+        "bound": "hbm", "achieved": per_launch_model / avg_s / 1e9, "peak": peak_gbs, "unit": "GB/s",
+        "frac": per_launch_model / avg_s / 1e9 / peak_gbs, "kernel": name,
+        "launches_per_step": cnt // steps, "mean_launch_us": avg_s * 1e6,
+        "alg_bytes_per_launch": per_launch_model,
+        "alg_bytes_basis": f"reference-path model, block size 32 (model level = level + {shift})",
+        "compulsory_bytes_per_launch": by / cnt, "frac_compulsory": by / cnt / avg_s / 1e9 / peak_gbs,
+        "levels": levels}
+
+
+def config1_field(icm_amd, reps=50, cpu_threads=16):
+    """Config 1 (n=1024, k=10, Q=256, stacks uniform integers in [1, 1e6],
+    payout 1/(j+1) normalised): device and wall time per call with the plan
+    reused, and the oracle's linear engine (the reference CPU's choice at
+    k=10) on all threads and on one, in the same run; parity in the same run."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(1)
+    S = rng.integers(1, 10 ** 6, 1024, endpoint=True).astype(np.float64)
+    p = 1.0 / np.arange(1, 11)
+    p /= p.sum()
+    pl = icm_amd.Plan(S, len(p))
+    small = pl.small_path
+    for _ in range(5):
+        pl.run(p, 256)
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        e = pl.run(p, 256)
+        wall.append(time.perf_counter() - t0)
+        dev.append(pl.stats.total_ns / 1e6)
+    pl.close()
+
+    def cpu_ms(nt, r=5):
+        ts = []
+        for _ in range(r):
+            t0 = time.perf_counter()
+            ref = O.equity(S, p, Q=256, engine=O.ENGINE_LINEAR, nthreads=nt)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3, ref
+
+    cpu_all, ref = cpu_ms(cpu_threads)
+    cpu_one, _ = cpu_ms(1)
+    den = np.abs(ref)
+    den[den < 1e-14] = 1.0
+    return {"workload": "n=1024 k=10 Q=256", "small_path": bool(small),
+            "gpu_device_ms": float(np.median(dev)), "gpu_wall_ms": float(np.median(wall)) * 1e3,
+            "cpu_ms": cpu_all, "cpu_threads": cpu_threads, "cpu_single_thread_ms": cpu_one,
+            "cpu_kind": "port (oracle linear engine)", "max_rel_err_vs_cpu": float(np.max(np.abs(e - ref) / den)),
+            "reference_published": {"cpu_16t_ms": 0.129, "cpu_1t_ms": 1.33, "b200_k64_ms": 0.153}}
+
+
+def config2_field(icm_amd, reps=10, cpu_threads=16):
+    """Config 2 (n=65536, k=n, lognormal(0,1) stacks scaled to 1e9 chips,
+    payout 0.8^j normalised, Q=256): GPU (plan reused) and the oracle on the
+    host threads in the same run, with their agreement."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(2)
+    n = 65536
+    S = rng.lognormal(0.0, 1.0, n)
+    S *= 1e9 / S.sum()
+    p = 0.8 ** np.arange(n)
+    p /= p.sum()
+    pl = icm_amd.Plan(S, n)
+    pl.run(p, 256)
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        e = pl.run(p, 256)
+        wall.append(time.perf_counter() - t0)
+        dev.append(pl.stats.total_ns / 1e6)
+    pl.close()
+    t0 = time.perf_counter()
+    ref = O.equity(S, p, Q=256, fft_min=64, nthreads=cpu_threads)
+    cpu_s = time.perf_counter() - t0
+    den = np.abs(ref)
+    den[den < 1e-14] = 1.0
+    gpu_ms = float(np.median(dev))
+    return {"workload": "n=65536 k=65536 Q=256", "gpu_device_ms": gpu_ms,
+            "gpu_wall_ms": float(np.median(wall)) * 1e3, "player_equities_per_s": n / (gpu_ms * 1e-3),
+            "cpu_ms": cpu_s * 1e3, "cpu_threads": cpu_threads, "cpu_kind": "port (oracle hybrid engine)",
+            "gpu_over_cpu": cpu_s * 1e3 / gpu_ms, "max_rel_err_vs_cpu": float(np.max(np.abs(e - ref) / den)),
+            "reference_published": {"b200_ms": 20.26, "cpu_16t_ms": 862.0}}
 
 
 # ragged batch: 4096 independent instances (n log-uniform [2, 4096],
@@ -129,6 +291,17 @@ def batched_field(icm_amd, torch, dev, Q, reps=5, sample=256):
         icm_amd.equity(stacks[i], pays[i], Q)
     loop_s = time.perf_counter() - t_l
     loop_inst_s = sample / loop_s
+    # the same sample with one plan per instance created beforehand (plan reuse)
+    plans = [icm_amd.Plan(stacks[i], len(pays[i])) for i in idx]
+    for pl, i in zip(plans, idx):
+        pl.run(pays[i], Q)
+    t_r = time.perf_counter()
+    for pl, i in zip(plans, idx):
+        pl.run(pays[i], Q)
+    reuse_s = time.perf_counter() - t_r
+    for pl in plans:
+        pl.close()
+    reuse_inst_s = sample / reuse_s
     inst_s = len(stacks) / (batch_ms * 1e-3)
     kern = {k_: {"launches": v[0], "ms": v[1] / 1e6,
                  "GBps": (v[2] / (v[1] * 1e-9) / 1e9) if v[1] > 0 else 0.0}
@@ -137,7 +310,9 @@ def batched_field(icm_amd, torch, dev, Q, reps=5, sample=256):
             "ms_per_batch": batch_ms, "instances_per_s": inst_s,
             "loop_dropin_instances_per_s": loop_inst_s,
             "loop_sample": f"icm_hip_equity on {sample} random instances of the batch",
-            "speedup_vs_loop": inst_s / loop_inst_s, "plan_create_ms": create_ms,
+            "speedup_vs_loop": inst_s / loop_inst_s,
+            "loop_plan_reused_instances_per_s": reuse_inst_s, "speedup_vs_plan_reused_loop": inst_s / reuse_inst_s,
+            "plan_create_ms": create_ms,
             "device_bytes": summ_b["device_bytes"], "kernels_eager": kern}
 
 
@@ -267,14 +442,10 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     value = n * args.steps / (t1 - t0)
 
-    # dominant kernel (summed device time over the timed steps)
+    # dominant kernel (summed device time over the timed steps), its roofline
+    # against the reference-path byte model and its own compulsory bytes
     tot_kernel_ns = sum(v[1] for v in kst.values())
-    name, (launches, ns, by, fl) = max(kst.items(), key=lambda kv: kv[1][1])
-    avg_s = ns / launches * 1e-9
-    bytes_per_launch = by / launches
-    flops_per_launch = fl / launches
-    hbm_frac = bytes_per_launch / avg_s / (HBM_PEAK_GBS * 1e9)
-    fp_frac = flops_per_launch / avg_s / (FP64_PEAK_TFLOPS * 1e12)
+    name, roof = roofline_fields(plan.launch_log(), args.steps, n, k, summ["B"], Q, HBM_PEAK_GBS)
     traffic = None
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (scaled by a 1 GiB copy), committed as a profile summary
@@ -283,15 +454,19 @@ def main():
         for line in open(tfile):
             if line.startswith("per_launch_bytes_json:"):
                 traffic = json.loads(line.split(":", 1)[1]).get(name)
-    if hbm_frac >= fp_frac:
-        roof = {"bound": "hbm", "achieved": bytes_per_launch / avg_s / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": hbm_frac}
-    else:
-        roof = {"bound": "fp64", "achieved": flops_per_launch / avg_s / 1e12, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": fp_frac}
-    roof.update({"traffic": traffic, "kernel": name, "launches_per_step": launches // args.steps,
-                 "mean_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch,
-                 "share_of_device_time": ns / tot_kernel_ns})
+    launches = kst[name][0]
+    roof.update({"traffic": traffic, "share_of_device_time": kst[name][1] / tot_kernel_ns,
+                 "flops_per_launch": kst[name][3] / launches,
+                 "fp64_frac": kst[name][3] / launches / (roof["mean_launch_us"] * 1e-6) / (FP64_PEAK_TFLOPS * 1e12)})
+    # measured streaming rates on this device beside the nominal 8 TB/s
+    try:
+        r_gbs, w_gbs, c_gbs = icm_amd.measure_stream_gbps(1 << 30)
+        copy_gbs = icm_amd.measure_hbm_bandwidth_gbps()
+        roof["measured_hbm"] = {"read_GBps": r_gbs, "write_GBps": w_gbs, "copy_GBps": c_gbs,
+                                "device_copy_bytes_copied_GBps": copy_gbs,
+                                "frac_of_measured_copy": roof["achieved"] / c_gbs}
+    except Exception as e:  # reported, not fatal
+        roof["measured_hbm"] = {"error": str(e)}
     model_bytes = path_model_bytes(n, k) * Q
     path_roof = {"bound": "hbm", "model_bytes_per_step": model_bytes,
                  "achieved": model_bytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -342,9 +517,16 @@ def main():
         batched = batched_field(icm_amd, torch, dev, Q)
         log(f"batched: {batched}")
 
+    nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+    small = None
+    if args.only == "all":
+        small = {"config1": config1_field(icm_amd, cpu_threads=nt)}
+        log(f"config 1: {small['config1']}")
+        small["config2"] = config2_field(icm_amd, cpu_threads=nt)
+        log(f"config 2: {small['config2']}")
+
     cpu = None
     if not args.no_cpu_baseline:
-        nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
         log(f"cpu baseline: {nt} threads ...")
         try:
             cpu = cpu_baseline(n, k, Q, S, p, nt)
@@ -366,6 +548,7 @@ def main():
         "subset": subset,
         "adjoint": adjoint,
         "batched": batched,
+        "configs_1_2": small,
         "pcie_inclusive": pcie,
         "cpu_baseline": cpu,
     }

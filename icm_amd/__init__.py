@@ -125,6 +125,9 @@ _SIGS = [
     ("icm_hip_plan_get_small_path", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_spectrum_cache", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_launch_count", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_launch_stat", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D]),
     ("icm_hip_plan_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
@@ -457,6 +460,19 @@ class Plan:
             self._lib.icm_hip_plan_kernel_stat(self._h, i, buf, 64, ctypes.byref(ln), ctypes.byref(ns),
                                                ctypes.byref(by), ctypes.byref(fl))
             out[buf.value.decode()] = (ln.value, ns.value, by.value, fl.value)
+        return out
+
+    def launch_log(self):
+        """[(name, level, ns, alg_bytes)] of every profiled launch (level > 0:
+        build of that tree level, < 0: its propagate, 0: leaves)."""
+        out = []
+        buf = ctypes.create_string_buffer(64)
+        lv = ctypes.c_int(0)
+        ns, by = ctypes.c_double(0), ctypes.c_double(0)
+        for i in range(self._lib.icm_hip_plan_launch_count(self._h)):
+            self._lib.icm_hip_plan_launch_stat(self._h, i, buf, 64, ctypes.byref(lv), ctypes.byref(ns),
+                                               ctypes.byref(by))
+            out.append((buf.value.decode(), lv.value, ns.value, by.value))
         return out
 
     def close(self):

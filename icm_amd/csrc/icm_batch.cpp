@@ -875,7 +875,7 @@ int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
 int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }
     plan->prof.enabled = on != 0;
-    plan->prof.agg.clear();
+    plan->prof.clear();
     return 0;
 }
 

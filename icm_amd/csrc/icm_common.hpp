@@ -137,14 +137,23 @@ inline int pick_B(int n, int k, int forced) {
 
 // Per-launch event timing, aggregated per kernel name (see icm::Prof).
 struct KernelProf : icm::Prof {
-    struct Rec { std::string name; hipEvent_t a, b; double bytes, flops; };
+    struct Rec { std::string name; hipEvent_t a, b; double bytes, flops; int tag; };
     struct Agg { std::string name; int launches = 0; double ns = 0, bytes = 0, flops = 0; };
+    struct Launch { std::string name; int tag; double ns, bytes, flops; };
     bool enabled = false;
+    int tag = 0;                        // set by the caller before launches: tree level (+ build, - propagate)
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     std::vector<Rec> pending;
     std::vector<Agg> agg;
+    std::vector<Launch> log;            // every launch since profiling was switched on
     hipEvent_t cur_a = nullptr;
+    void clear() {
+        agg.clear();
+        pending.clear();
+        log.clear();
+        used = 0;
+    }
     ~KernelProf() override { for (hipEvent_t e : pool) (void)hipEventDestroy(e); }
     bool on() const override { return enabled; }
     hipEvent_t take() {
@@ -163,7 +172,7 @@ struct KernelProf : icm::Prof {
         hipEvent_t b = take();
         if (!cur_a || !b) return;
         (void)hipEventRecord(b, s);
-        pending.push_back(Rec{name, cur_a, b, bytes, flops});
+        pending.push_back(Rec{name, cur_a, b, bytes, flops, tag});
         cur_a = nullptr;
     }
     // After the stream has finished: fold pending records into the aggregates.
@@ -178,6 +187,7 @@ struct KernelProf : icm::Prof {
             g->ns += (double)ms * 1e6;
             g->bytes += r.bytes;
             g->flops += r.flops;
+            if (log.size() < (1u << 20)) log.push_back(Launch{r.name, r.tag, (double)ms * 1e6, r.bytes, r.flops});
         }
         pending.clear();
         used = 0;

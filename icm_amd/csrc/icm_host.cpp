@@ -505,6 +505,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.G0 = g.G[0];
         lj.blist = nullptr;
         lj.nbl = g.nb;
+        p->prof.tag = 0;
         if (!hip_ok(icm::launch_leaf_build(lj, s, &p->prof), "leaf_build")) return -1;
 
         // views of level l's coefficients
@@ -529,6 +530,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
+            p->prof.tag = l;
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_build_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
@@ -564,6 +566,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
+            p->prof.tag = -l;
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_prop_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
@@ -585,6 +588,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         lj.partial = p->d_partial;
         lj.inner_out = p->d_inner ? p->d_inner + (long long)q0 * g.n : nullptr;
         lj.active = p->d_active;
+        p->prof.tag = 0;
         if (p->d_lists) {
             lj.blist = p->d_lists + p->list_off[L];
             lj.nbl = p->list_cnt[L];
@@ -1247,9 +1251,23 @@ int icm_hip_plan_get_spectrum_cache(const IcmHipPlan *plan) { return plan ? (pla
 int icm_hip_plan_set_profiling(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }
     plan->prof.enabled = on != 0;
-    plan->prof.agg.clear();
-    plan->prof.pending.clear();
-    plan->prof.used = 0;
+    plan->prof.clear();
+    return 0;
+}
+
+int icm_hip_plan_launch_count(const IcmHipPlan *plan) { return plan ? (int)plan->prof.log.size() : -1; }
+
+int icm_hip_plan_launch_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *level, double *ns,
+                             double *alg_bytes) {
+    if (!plan || index < 0 || index >= (int)plan->prof.log.size()) { set_err("bad launch index"); return -1; }
+    const auto &a = plan->prof.log[index];
+    if (name && name_len > 0) {
+        strncpy(name, a.name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (level) *level = a.tag;
+    if (ns) *ns = a.ns;
+    if (alg_bytes) *alg_bytes = a.bytes;
     return 0;
 }
 

@@ -256,6 +256,12 @@ int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
  * subset runs keep their own child transforms. */
 int icm_hip_plan_set_spectrum_cache(IcmHipPlan *plan, int on);
 int icm_hip_plan_get_spectrum_cache(const IcmHipPlan *plan);
+/* Every profiled launch in order: kernel name, tree level (l > 0 build of
+ * level l, -l propagate of level l, 0 leaves and reductions), device ns and
+ * compulsory bytes. */
+int icm_hip_plan_launch_count(const IcmHipPlan *plan);
+int icm_hip_plan_launch_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *level, double *ns,
+                             double *alg_bytes);
 int icm_hip_plan_kernel_count(const IcmHipPlan *plan);
 int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int name_len, int *launches,
                              double *total_ns, double *alg_bytes, double *alg_flops);
