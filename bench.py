@@ -436,6 +436,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     kst = plan.kernel_stats()
+    llog = plan.launch_log()
     plan.set_profiling(False)
     eq = d_e.cpu().numpy()
     log(f"sum(E) - sum(payout) = {eq.sum() - p.sum():.3e}")
@@ -445,7 +446,7 @@ def main():
     # dominant kernel (summed device time over the timed steps), its roofline
     # against the reference-path byte model and its own compulsory bytes
     tot_kernel_ns = sum(v[1] for v in kst.values())
-    name, roof = roofline_fields(plan.launch_log(), args.steps, n, k, summ["B"], Q, HBM_PEAK_GBS)
+    name, roof = roofline_fields(llog, args.steps, n, k, summ["B"], Q, HBM_PEAK_GBS)
     traffic = None
     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (scaled by a 1 GiB copy), committed as a profile summary

@@ -14,7 +14,7 @@ import numpy as np
 
 __all__ = ["ICMError", "init", "equity", "equity_subset", "equity_vjp", "node_inner", "node_vjp", "Plan",
            "BatchPlan", "plan_geometry", "plan_dry_run", "make_nodes", "library_path", "load_library",
-           "measure_hbm_bandwidth_gbps", "measure_stream_gbps", "measure_fused_pair_ns"]
+           "measure_hbm_bandwidth_gbps", "measure_stream_gbps", "measure_lds_gbps", "measure_fused_pair_ns"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_NAME = "libicm_hip.so"
@@ -103,6 +103,7 @@ _SIGS = [
     ("icm_gpu_measure_fused_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("icm_gpu_measure_fused_r2c_pair_ns", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("icm_hip_measure_stream_gbps", ctypes.c_int, [ctypes.c_size_t, _D, _D, _D]),
+    ("icm_hip_measure_lds_gbps", ctypes.c_int, [_D]),
     ("icm_hip_equity_noalloc", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
     ("icm_init", None, [ctypes.c_char_p]),
     ("icm_equity", None, [ctypes.c_int, _D, ctypes.c_int, _D, ctypes.c_int, _D]),
@@ -123,6 +124,7 @@ _SIGS = [
     ("icm_hip_plan_get_schoolbook_max", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_set_small_path", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_small_path", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_plan_set_column_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_spectrum_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_launch_count", ctypes.c_int, [ctypes.c_void_p]),
@@ -373,6 +375,11 @@ class Plan:
     def small_path(self):
         """True when forward runs go through the one-launch small-instance kernel."""
         return self._lib.icm_hip_plan_get_small_path(self._h) == 1
+
+    def set_column_fusion(self, on):
+        """Fused column inverse + next level's column forward in the propagate (bit-identical)."""
+        if self._lib.icm_hip_plan_set_column_fusion(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_column_fusion")
 
     def set_spectrum_cache(self, on):
         """Keep (True) or recompute (False) the build's column transforms of
@@ -731,6 +738,16 @@ def measure_stream_gbps(nbytes=1 << 30):
     if lib.icm_hip_measure_stream_gbps(int(nbytes), ctypes.byref(r), ctypes.byref(w), ctypes.byref(c)) != 1:
         raise _err(lib, "icm_hip_measure_stream_gbps")
     return r.value, w.value, c.value
+
+
+def measure_lds_gbps():
+    """LDS read rate (bytes delivered to lanes per second) of conflict-free 16-byte reads."""
+    _auto_init()
+    lib = load_library()
+    v = ctypes.c_double(0)
+    if lib.icm_hip_measure_lds_gbps(ctypes.byref(v)) != 1:
+        raise _err(lib, "icm_hip_measure_lds_gbps")
+    return v.value
 
 
 def measure_fused_pair_ns(fft_n, batch, quick=False):

@@ -104,6 +104,7 @@ struct IcmHipPlan {
     int maxF = 16;
     bool spec_ok = true;                 // spectrum cache allowed (memory_strategy != 3)
     double frac = 0.6;                   // share of free device memory to plan for
+    bool fuse_columns = true;            // fused column inverse + next level's column forward (propagate)
     KernelProf prof;
 };
 
@@ -541,10 +542,14 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                                                              : nullptr);
             if (!hip_ok(e, "build level")) return -1;
         }
-        // propagate: root g = payout (same for every q-plane)
+        // propagate: root g = payout (same for every q-plane); jobs of every
+        // level first, so that a four-step level can hand the level below its
+        // g-pair column transforms directly (column fusion)
+        std::vector<icm::LevelJob> PJ(L);
         int cur = 0;
         for (int l = L - 1; l >= 1; --l) {
-            icm::LevelJob J{};
+            icm::LevelJob &J = PJ[l];
+            J = icm::LevelJob{};
             J.child = coef_view(l - 1);
             if (l == L - 1) {
                 J.gpar = icm::CoefView{d_payout, 0, g.k, g.k};
@@ -562,11 +567,18 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                 J.plist = nullptr;
                 J.np = g.n_real[l];
             }
-            if (J.np == 0) continue;
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
+        }
+        bool g_ready = false;
+        for (int l = L - 1; l >= 1; --l) {
+            const icm::LevelJob &J = PJ[l];
+            if (J.np == 0) { g_ready = false; continue; }
             p->prof.tag = -l;
+            const bool large = !use_schoolbook(p, l) && J.F > icm::kMaxLdsF;
+            const bool fuse = large && p->fuse_columns && l >= 2 && !use_schoolbook(p, l - 1) &&
+                              PJ[l - 1].F > icm::kMaxLdsF && icm::prop_fusable(J, PJ[l - 1]);
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_prop_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
@@ -574,8 +586,10 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                                : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
                                                         (p->d_spec && !J.plist && l <= L - 2 && p->spec_off[l] >= 0)
                                                             ? p->d_spec + p->spec_off[l]
-                                                            : nullptr);
+                                                            : nullptr,
+                                                        g_ready, fuse ? &PJ[l - 1] : nullptr);
             if (!hip_ok(e, "propagate level")) return -1;
+            g_ready = fuse;
         }
         if (L == 1) {
             lj.g = d_payout; lj.g_plane = 0; lj.g_stride = g.k;
@@ -1224,6 +1238,12 @@ int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode) {
     if (!plan) { set_err("null plan"); return -1; }
     if (mode < -1 || mode > 1) { set_err("small-path mode must be -1 (tuned), 0 (off) or 1 (on)"); return -1; }
     plan->small_mode = mode;
+    return 0;
+}
+
+int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->fuse_columns = on != 0;
     return 0;
 }
 

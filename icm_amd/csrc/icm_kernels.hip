@@ -804,6 +804,23 @@ __global__ void __launch_bounds__(256) k_stream_read(const double4 *__restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// LDS read probe: each workgroup fills 32 KB of LDS from src, then every lane
+// reads 16-byte entries at consecutive addresses (conflict-free ds_read_b128)
+// iters times and sums every component, so each read is live.
+__global__ void __launch_bounds__(256) k_lds_probe(const double2 *__restrict__ src, double *__restrict__ out,
+                                                   int iters) {
+    __shared__ double2 sm[2048];
+    const int t = threadIdx.x;
+    for (int i = t; i < 2048; i += 256) sm[i] = src[i];
+    __syncthreads();
+    double a = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        const double2 v = sm[(t + it * 257) & 2047];
+        a += v.x + v.y;
+    }
+    out[(long long)blockIdx.x * 256 + t] = a;
+}
+
 __global__ void __launch_bounds__(256) k_stream_write(double4 *__restrict__ dst, long long n4, double v) {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
         dst[i] = make_double4(v, v, v, v);
@@ -1874,6 +1891,112 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
     }
 }
 
+// Propagate column inverse of a four-step level l fused with the g-pair
+// column forward of level l - 1 (also four-step, F' = F / 2).  The inverse
+// of slot a holds (g_left, g_right) of parent a's children as (Re, Im), which
+// is exactly level l-1's g pair a; its coefficients below F / 2 = F' are
+// rows r < F1 / 2 of every column, so level l-1's column n2 is the first
+// half of level l's column n2.  A workgroup inverts two column tiles of level
+// l (2 TC columns, one tile at a time, keeping the first-half rows in
+// registers), then runs level l-1's forward columns on one tile of
+// (2 TC columns x F1/2 rows) and writes them as k_col_fwd<F1/2, GRows> does
+// (same values: the scaled coefficients, zero past the count and for a
+// missing right child, then the same passes and twiddles).  Level l-1's g
+// coefficients are never stored.
+template <int F1>
+__global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__ in, LevelJob J,
+                                                     double2 *__restrict__ out, int nArr, int logF,
+                                                     const double2 *tw, int twShift) {
+    static_assert(F1 >= 8, "the next level must be a four-step level with column kernels");
+    using TL = ColLayout<F1>;
+    constexpr int H = F1 / 2;
+    using TLH = ColLayout<H>;
+    __shared__ double2 lds[kTile];
+    constexpr int TC = kTile / F1;
+    constexpr int LOGTC = ilog2c(TC);
+    constexpr int KEEP = kTile / 2 / kCT;     // first-half points per thread and tile
+    const int a = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
+    const int F = 1 << logF;
+    const double2 *src = in + ((long long)qi * nArr + a) * F;
+    const int p = slot_parent(J, a);
+    const bool has_l = p >= 0, has_r = p >= 0 && (2 * p + 1) < J.nr_child;
+    const int gcount = J.gchild.count;
+    const double invF = 1.0 / (double)F;
+    double2 held[2][KEEP];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int tile = 2 * blockIdx.x + h;
+        if constexpr (TC <= kCT) {   // inverse four-step twiddle on the load, as k_col_inv
+            constexpr int RS = kCT / TC;
+            const int c = t & (TC - 1), r0 = t >> LOGTC;
+            const int n2 = tile * TC + c;
+            double2 w = tw_big2(tw, twShift, (n2 * r0) & (F - 1), logF);
+            const double2 ws = tw_big2(tw, twShift, (n2 * RS) & (F - 1), logF);
+#pragma unroll
+            for (int u = 0; u < kCP; ++u) {
+                const int r = r0 + u * RS;
+                lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], w);
+                w = c_mul(w, ws);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kCP; ++u) {
+                const int e = t + u * kCT;
+                const int r = e >> LOGTC, c = e & (TC - 1);
+                const int n2 = tile * TC + c;
+                lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big2(tw, twShift, (n2 * r) & (F - 1), logF));
+            }
+        }
+        fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
+#pragma unroll
+        for (int u = 0; u < KEEP; ++u) {
+            const int e = t + u * kCT;            // e < kTile / 2: rows r < F1 / 2
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            const int n = r * kF2 + tile * TC + c;
+            const double2 v = lds[TL::at(c, r)];
+            held[h][u] = make_double2((has_l && n < gcount) ? v.x * invF : 0.0,
+                                      (has_r && n < gcount) ? v.y * invF : 0.0);
+        }
+        __syncthreads();                          // reads done before the tile is reused
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int u = 0; u < KEEP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            lds[TLH::at(h * TC + c, r)] = held[h][u];
+        }
+    fft_tile<H, false, kCT, TLH>(lds, tw, twShift);
+    // level l-1's four-step twiddle w_F'^(n2 k1) and store, as k_col_fwd<H>
+    constexpr int TC2 = 2 * TC;
+    constexpr int LOGTC2 = ilog2c(TC2);
+    const int Fh = F >> 1, logFh = logF - 1;
+    double2 *o = out + ((long long)qi * nArr + a) * Fh;
+    if constexpr (TC2 <= kCT) {
+        constexpr int RS = kCT / TC2;
+        const int c = t & (TC2 - 1), r0 = t >> LOGTC2;
+        const int n2 = blockIdx.x * TC2 + c;
+        double2 w = tw_big(tw, twShift, (n2 * r0) & (Fh - 1), logFh);
+        const double2 ws = tw_big(tw, twShift, (n2 * RS) & (Fh - 1), logFh);
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int r = r0 + u * RS;
+            o[r * kF2 + n2] = c_mul(lds[TLH::at(c, r)], w);
+            w = c_mul(w, ws);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC2, c = e & (TC2 - 1);
+            const int n2 = blockIdx.x * TC2 + c;
+            const double2 w = tw_big(tw, twShift, (n2 * r) & (Fh - 1), logFh);
+            o[r * kF2 + n2] = c_mul(lds[TLH::at(c, r)], w);
+        }
+    }
+}
+
 // Fused spectrum stage of a large level.  Every bin -K of row k1 lies in row
 // F1 - k1 (position F2-1-k2; rows 0 and F1/2 are their own mirrors, at
 // (F2-k2) mod F2 and F2-1-k2), so the forward row FFTs, the real-pair split,
@@ -2353,7 +2476,9 @@ hipError_t launch_small_finalize(const double *partial, int G, int n, const unsi
 
 hipError_t launch_stream_probe(int mode, const double *src, double *dst, long long n4, double *part, int nwg,
                                hipStream_t s) {
-    if (mode == 0)
+    if (mode == 3)
+        k_lds_probe<<<nwg, 256, 0, s>>>(reinterpret_cast<const double2 *>(src), dst, (int)n4);
+    else if (mode == 0)
         k_stream_read<<<nwg, 256, 0, s>>>(reinterpret_cast<const double4 *>(src), n4, part);
     else if (mode == 1)
         k_stream_write<<<nwg, 256, 0, s>>>(reinterpret_cast<double4 *>(dst), n4, 1.0);
@@ -2583,8 +2708,28 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
            nq * npairs * kF2 * fft_flops(F1));
     return hipSuccess;
 }
+static hipError_t col_inv_fwd(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
+                              int logF, const double2 *tw, int twShift, hipStream_t s) {
+    dim3 grid(F1 / 2, nArr, nq);
+#define ICM_CIF(V) case V: k_col_inv_fwd<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift); break;
+    switch (F1) {
+        ICM_CIF(8) ICM_CIF(16) ICM_CIF(32) ICM_CIF(64) ICM_CIF(128) ICM_CIF(256) ICM_CIF(512) ICM_CIF(1024)
+        ICM_CIF(2048) ICM_CIF(4096)
+    default: return hipErrorInvalidValue;
+    }
+#undef ICM_CIF
+    return hipGetLastError();
+}
+
+bool prop_fusable(const LevelJob &J, const LevelJob &Jn) {
+    return !col_fused(J.F) && J.F >= 8 * kF2 && Jn.F * 2 == J.F && !col_fused(Jn.F) && !J.plist && !Jn.plist &&
+           !J.acc_out && !J.gfix && !J.child_r.base && !J.r_missing_zero && Jn.np == J.nr_child &&
+           Jn.gpar.count == J.gchild.count && Jn.nq == J.nq && J.np == J.nr_parent;
+}
+
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                             hipStream_t s, Prof *pf, const double2 *spec, bool g_ready) {
+                             hipStream_t s, Prof *pf, const double2 *spec, bool g_ready, const LevelJob *next) {
+    if (next && !prop_fusable(J, *next)) return hipErrorInvalidValue;
     const int F1 = J.F >> 12;
     const int nrp = J.np, npairs = (nrp + 1) / 2;   // slots (all parents without pruning)
     const long long r1 = (long long)J.nq * nrp * J.F;   // parent region size
@@ -2624,6 +2769,14 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
+    if (next) {   // the next level's g pairs straight from this level's inverse columns
+        double2 *gout = bufA + (long long)next->nq * next->np * next->F;
+        PF_BEGIN();
+        ICM_TRY(col_inv_fwd(F1, bufB, J, gout, nrp, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_inv_fwd<%d>", F1), 16.0 * nq * nrp * F + 8.0 * nq * nrp * F,
+               nq * nrp * kF2 * (fft_flops(F1) + fft_flops(F1 / 2)));
+        return hipSuccess;
+    }
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StGChildren{J}, nrp, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StGChildren>", F1), 16.0 * nq * nrp * F + 8.0 * nq * J.nr_child * J.gchild.count, nq * nrp * kF2 * fft_flops(F1));

@@ -124,7 +124,8 @@ hipError_t launch_leaf_vjp(const VjpLeafJob &J, int nchunk, hipStream_t s, Prof 
 hipError_t launch_gather(const double *src, const int *perm, int n, double *dst, hipStream_t s);
 // Streaming probes over n4 double4 units with nwg workgroups of 256 threads:
 // mode 0 reads src (one partial sum per workgroup into part[nwg]), 1 fills
-// dst, 2 copies src to dst.
+// dst, 2 copies src to dst; mode 3 is the LDS read probe (src: 2048 double2,
+// dst: nwg * 256 doubles, n4 = reads per lane).
 hipError_t launch_stream_probe(int mode, const double *src, double *dst, long long n4, double *part, int nwg,
                                hipStream_t s);
 
@@ -155,9 +156,15 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
                                 double2 *specD, double2 *bufB, hipStream_t s, Prof *pf);
 // g_ready: bufA already holds this J's g-pair column transforms (the previous
 // launch had the same gpar, nq and parents), so they are not computed again.
+// next (optional): the propagate of the level below (F / 2, also a four-step
+// level with column kernels) is fused in: this level's column inverse writes
+// next's g-pair column transforms (at bufA + next.nq * next.np * next.F,
+// where next's launch reads them with g_ready) instead of the children's g
+// coefficients, which are then never stored.  Only where prop_fusable.
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
                              double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr,
-                             bool g_ready = false);
+                             bool g_ready = false, const LevelJob *next = nullptr);
+bool prop_fusable(const LevelJob &J, const LevelJob &next);
 
 // Adjoint B propagate of a four-step level in one pass: B children =
 // A * dR + B * R (JdR: A's g pair over the dP children, JR: B's g pair over

@@ -193,4 +193,29 @@ int icm_hip_measure_stream_gbps(size_t bytes, double *read_gbps, double *write_g
     return 1;
 }
 
+int icm_hip_measure_lds_gbps(double *gbps) {
+    if (!gbps) return 0;
+    *gbps = 0.0;
+    if (!ensure_device()) return 0;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device);
+    const int nwg = std::max(1, cus) * 8, iters = 4096;
+    Bufs m;
+    double *src = nullptr, *out = nullptr;
+    if (!m.get(&src, sizeof(double) * 4096) || !m.get(&out, sizeof(double) * 256 * (size_t)nwg)) return 0;
+    Timer T;
+    if (!T.ok || hipMemsetAsync(src, 0, sizeof(double) * 4096, T.s) != hipSuccess) {
+        set_err("lds probe: setup failed");
+        return 0;
+    }
+    std::vector<double> t;
+    for (int r = 0; r < 4; ++r) {
+        const double ms = T.ms([&] { return icm::launch_stream_probe(3, src, out, iters, nullptr, nwg, T.s) == hipSuccess; });
+        if (ms <= 0) { set_err("lds probe: launch failed"); return 0; }
+        if (r > 0) t.push_back(ms);
+    }
+    *gbps = (double)nwg * 256 * iters * 16 / (median(t) * 1e-3) / 1e9;
+    return 1;
+}
+
 }  // extern "C"

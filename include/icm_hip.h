@@ -196,6 +196,9 @@ int icm_gpu_measure_fused_r2c_pair_ns(int fft_n, int batch, int quick, double *b
  * (no reference counterpart): read-only (sum), write-only (fill) and copy,
  * each in HBM bytes moved per second (a copy counts its reads and writes). */
 int icm_hip_measure_stream_gbps(size_t bytes, double *read_gbps, double *write_gbps, double *copy_gbps);
+/* LDS read rate of conflict-free 16-byte reads (ds_read_b128) over all CUs,
+ * bytes delivered to lanes per second (no reference counterpart). */
+int icm_hip_measure_lds_gbps(double *gbps);
 
 /* ---- no device allocation ----
  * No counterpart in the reference.  For callers that cannot allocate device
@@ -246,6 +249,11 @@ int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan);
  * table's crossover allows.  mode -1 = tuned (default), 0 = never, 1 =
  * whenever it fits.  get returns 1 when forward runs take this path. */
 int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode);
+/* Column fusion of the forward propagate (default on): a four-step level's
+ * column inverse writes the level below's g-pair column transforms directly
+ * (no g coefficients stored and read back in between).  Results are
+ * bit-identical either way; off is for testing and timing. */
+int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on);
 int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
 /* Spectrum cache (no counterpart in the reference, whose large levels are
  * cuFFT plans per level, src/gpu/gpu_api.cu): the column transforms of each

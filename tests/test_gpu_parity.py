@@ -387,3 +387,29 @@ def test_f8192_level_in_row_kernels():
     plan.close()
     assert np.max(np.abs(e1 - e0)) / np.max(np.abs(e0)) < 1e-12
     assert np.max(np.abs(d1 - d0)) / np.max(np.abs(d0)) < 1e-11
+
+
+@pytest.mark.parametrize("n,k,Q", [(1 << 20, 1 << 20, 4), (300001, 300001, 6), (262144, 65536, 6),
+                                   (200003, 9000, 5)])
+def test_column_fusion_bit_identical(n, k, Q):
+    """The propagate's fused column inverse + next level's column forward
+    gives the same doubles as storing the g coefficients and transforming
+    them again (same values, passes and twiddles)."""
+    rng = np.random.default_rng(n + k)
+    S = np.exp(rng.uniform(0.0, math.log(1e9), n))
+    p = np.sort(rng.uniform(0.1, 1.0, k))[::-1]
+    p /= p.sum()
+    pl = icm_amd.Plan(S, k)
+    try:
+        pl.set_profiling(True)
+        fused = pl.run(p, Q)
+        names = set(pl.kernel_stats())
+        pl.set_profiling(False)
+        pl.set_column_fusion(False)
+        plain = pl.run(p, Q)
+    finally:
+        pl.close()
+    nf = sum(1 for x in names if x.startswith("k_col_inv_fwd"))
+    say(f"n={n} k={k}: fused kernels {sorted(x for x in names if x.startswith('k_col_inv_fwd'))}")
+    assert nf > 0
+    assert np.array_equal(fused, plain)
