@@ -520,8 +520,14 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             }
             return v;
         };
+        // build; a four-step level whose level above keeps its child-pair
+        // spectra in the cache hands them over directly (column fusion: its
+        // parent coefficients are then never read, except by pruned subset
+        // propagates, which switch the fusion off)
+        std::vector<icm::LevelJob> BJ(L);
         for (int l = 1; l <= L - 2; ++l) {
-            icm::LevelJob J{};
+            icm::LevelJob &J = BJ[l];
+            J = icm::LevelJob{};
             J.child = coef_view(l - 1);
             J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
             J.nr_child = g.n_real[l - 1];
@@ -531,16 +537,27 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             J.F = g.F[l];
             J.logF = ilog2i(g.F[l]);
             J.nq = nq;
+        }
+        auto spec_at = [&](int l) -> double2 * {
+            return (p->d_spec && l >= 1 && l <= L - 2 && p->spec_off[l] >= 0) ? p->d_spec + p->spec_off[l] : nullptr;
+        };
+        bool child_ready = false;
+        for (int l = 1; l <= L - 2; ++l) {
+            const icm::LevelJob &J = BJ[l];
             p->prof.tag = l;
+            const bool large = !use_schoolbook(p, l) && J.F > icm::kMaxLdsF;
+            const bool fuse = large && p->fuse_columns && !p->d_lists && l + 1 <= L - 2 && spec_at(l) &&
+                              spec_at(l + 1) && !use_schoolbook(p, l + 1) && BJ[l + 1].F > icm::kMaxLdsF &&
+                              icm::build_fusable(J, BJ[l + 1]);
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_build_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
-                                                         (p->d_spec && p->spec_off[l] >= 0)
-                                                             ? p->d_spec + p->spec_off[l]
-                                                             : nullptr);
+                                                         spec_at(l), child_ready, fuse ? &BJ[l + 1] : nullptr,
+                                                         fuse ? spec_at(l + 1) : nullptr);
             if (!hip_ok(e, "build level")) return -1;
+            child_ready = fuse;
         }
         // propagate: root g = payout (same for every q-plane); jobs of every
         // level first, so that a four-step level can hand the level below its

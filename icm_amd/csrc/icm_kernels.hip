@@ -1997,6 +1997,140 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__
     }
 }
 
+// Build column inverse of a four-step level l fused with the child-pair
+// column forward of level l + 1 (F' = 2 F).  Array a of the inverse holds
+// parents 2a and 2a + 1 of level l as (Re, Im): the child pair of level
+// l+1's parent a.  Level l+1's column n2 holds coefficients n2 + 4096 r',
+// r' < 2 F1: rows r' < F1 are level l's inverse column n2 (scaled, with the
+// exact coefficient 0 and zero past the count), row F1 of column 0 is the
+// exact top coefficient (index F) when the parents keep F + 1 terms, and the
+// rest is zero.  A workgroup inverts one column tile (TC columns), keeps it
+// in registers and runs level l+1's forward columns on its two halves (TC/2
+// columns x 2 F1 rows each), written as k_col_fwd<2 F1, ChildRows> writes
+// them (same values, passes and twiddles).  Of level l's coefficients only
+// the two end terms of every parent are stored (the level above computes its
+// own end terms exactly from them); the rest are never stored: fused only
+// where every later reader takes the spectra.
+template <int F1>
+__global__ __launch_bounds__(kCT) void k_col_inv_fwd_build(const double2 *__restrict__ in, LevelJob J,
+                                                           double2 *__restrict__ out, int nArr, int logF,
+                                                           const double2 *tw, int twShift) {
+    static_assert(F1 >= 4 && F1 <= 1024, "both levels are four-step levels with column kernels");
+    using TL = ColLayout<F1>;
+    constexpr int F1N = 2 * F1;
+    using TLN = ColLayout<F1N>;
+    __shared__ double2 lds[kTile];
+    constexpr int TC = kTile / F1;
+    constexpr int LOGTC = ilog2c(TC);
+    constexpr int TCN = TC / 2;
+    constexpr int LOGTCN = ilog2c(TCN);
+    const int tile = blockIdx.x, a = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
+    const int F = 1 << logF;
+    const double2 *src = in + ((long long)qi * nArr + a) * F;
+    if constexpr (TC <= kCT) {   // inverse four-step twiddle on the load, as k_col_inv
+        constexpr int RS = kCT / TC;
+        const int c = t & (TC - 1), r0 = t >> LOGTC;
+        const int n2 = tile * TC + c;
+        double2 w = tw_big2(tw, twShift, (n2 * r0) & (F - 1), logF);
+        const double2 ws = tw_big2(tw, twShift, (n2 * RS) & (F - 1), logF);
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int r = r0 + u * RS;
+            lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], w);
+            w = c_mul(w, ws);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            const int n2 = tile * TC + c;
+            lds[TL::at(c, r)] = c_mulc(src[r * kF2 + n2], tw_big2(tw, twShift, (n2 * r) & (F - 1), logF));
+        }
+    }
+    fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
+    // level l's parent coefficients as StParentPair would store them
+    const int p = 2 * a;
+    const bool hx = p < J.nr_parent, hy = p + 1 < J.nr_parent;
+    const int pc = J.parent.count;
+    const double invF = 1.0 / (double)F;
+    double *pb = J.parent.base + (long long)qi * J.parent.plane;
+    double2 v[kCP];
+#pragma unroll
+    for (int u = 0; u < kCP; ++u) {
+        const int e = t + u * kCT;
+        const int r = e >> LOGTC, c = e & (TC - 1);
+        const int n = r * kF2 + tile * TC + c;
+        const double2 z = lds[TL::at(c, r)];
+        double x = 0.0, y = 0.0;
+        if (n == 0) {
+            if (hx) x = child_prod_at(J, qi, p, 0);
+            y = hy ? child_prod_at(J, qi, p + 1, 0) : 1.0;   // a missing right child is the identity
+        } else if (n < pc) {
+            if (hx) x = z.x * invF;
+            if (hy) y = z.y * invF;
+        }
+        v[u] = make_double2(x, y);
+        // the level above's exact end terms read coefficients 0 and pc - 1
+        if (n == 0 || n == pc - 1) {
+            if (hx) pb[(long long)p * J.parent.stride + n] = x;
+            if (hy) pb[(long long)(p + 1) * J.parent.stride + n] = y;
+        }
+    }
+    double2 top = make_double2(0.0, 0.0);       // coefficient F, at row F1 of column 0
+    if (tile == 0 && pc > F) {
+        const int tc = J.child.count - 1;
+        if (hx) top.x = child_prod_at(J, qi, p, tc);
+        if (hy) top.y = child_prod_at(J, qi, p + 1, tc);
+        if (t == 0) {
+            if (hx) pb[(long long)p * J.parent.stride + F] = top.x;
+            if (hy) pb[(long long)(p + 1) * J.parent.stride + F] = top.y;
+        }
+    }
+    const int FN = F << 1, logFN = logF + 1;
+    double2 *o = out + ((long long)qi * nArr + a) * FN;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        __syncthreads();                          // the previous contents are read
+#pragma unroll
+        for (int u = 0; u < kCP; ++u) {
+            const int e = t + u * kCT;
+            const int r = e >> LOGTC, c = e & (TC - 1);
+            if ((c >> LOGTCN) == h) lds[TLN::at(c - h * TCN, r)] = v[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kCP / 2; ++u) {       // rows F1 .. 2 F1 - 1
+            const int e = t + u * kCT;
+            const int r = F1 + (e >> LOGTCN), c = e & (TCN - 1);
+            lds[TLN::at(c, r)] = (h == 0 && c == 0 && r == F1) ? top : make_double2(0.0, 0.0);
+        }
+        fft_tile<F1N, false, kCT, TLN>(lds, tw, twShift);
+        const int tn = 2 * tile + h;              // level l+1's column tile
+        if constexpr (TCN <= kCT) {
+            constexpr int RS = kCT / TCN;
+            const int c = t & (TCN - 1), r0 = t >> LOGTCN;
+            const int n2 = tn * TCN + c;
+            double2 w = tw_big(tw, twShift, (n2 * r0) & (FN - 1), logFN);
+            const double2 ws = tw_big(tw, twShift, (n2 * RS) & (FN - 1), logFN);
+#pragma unroll
+            for (int u = 0; u < kCP; ++u) {
+                const int r = r0 + u * RS;
+                o[r * kF2 + n2] = c_mul(lds[TLN::at(c, r)], w);
+                w = c_mul(w, ws);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kCP; ++u) {
+                const int e = t + u * kCT;
+                const int r = e >> LOGTCN, c = e & (TCN - 1);
+                const int n2 = tn * TCN + c;
+                const double2 w = tw_big(tw, twShift, (n2 * r) & (FN - 1), logFN);
+                o[r * kF2 + n2] = c_mul(lds[TLN::at(c, r)], w);
+            }
+        }
+    }
+}
+
 // Fused spectrum stage of a large level.  Every bin -K of row k1 lies in row
 // F1 - k1 (position F2-1-k2; rows 0 and F1/2 are their own mirrors, at
 // (F2-k2) mod F2 and F2-1-k2), so the forward row FFTs, the real-pair split,
@@ -2115,9 +2249,10 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
 struct RowSrc {
     const double2 *base;
     int narr, logF;
+    int qmul = 1;   // 0: one set of arrays shared by every q-plane (the payout's spectrum at the root)
     template <class RS>
     __device__ __forceinline__ void load_fft(double2 *T, int a, int qi, int rA, int rB, const double2 *tw) const {
-        rowpair_load_fft<RS>(T, base + (((long long)qi * narr + a) << logF), rA, rB, tw);
+        rowpair_load_fft<RS>(T, base + (((long long)qi * qmul * narr + a) << logF), rA, rB, tw);
     }
 };
 struct RowDst {
@@ -2649,8 +2784,30 @@ static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq
     return hipGetLastError();
 }
 
+static hipError_t col_inv_fwd_build(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
+                                    int logF, const double2 *tw, int twShift, hipStream_t s) {
+    dim3 grid(F1, nArr, nq);
+#define ICM_CIB(V) case V: k_col_inv_fwd_build<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift); break;
+    switch (F1) {
+        ICM_CIB(4) ICM_CIB(8) ICM_CIB(16) ICM_CIB(32) ICM_CIB(64) ICM_CIB(128) ICM_CIB(256) ICM_CIB(512) ICM_CIB(1024)
+    default: return hipErrorInvalidValue;
+    }
+#undef ICM_CIB
+    return hipGetLastError();
+}
+
+bool build_fusable(const LevelJob &J, const LevelJob &Jn) {
+    return !col_fused(J.F) && J.F >= 4 * kF2 && J.F <= 1024 * kF2 && Jn.F == 2 * J.F && !J.acc_out &&
+           !J.child_r.base && !J.r_missing_zero && !Jn.child_r.base && !Jn.r_missing_zero &&
+           Jn.nr_child == J.nr_parent && Jn.child.count == J.parent.count && J.parent.count <= J.F + 1 &&
+           Jn.nq == J.nq;
+}
+
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                              hipStream_t s, Prof *pf, double2 *spec) {
+                              hipStream_t s, Prof *pf, double2 *spec, bool child_ready, const LevelJob *next,
+                              double2 *next_spec) {
+    if (next && (!next_spec || !build_fusable(J, *next))) return hipErrorInvalidValue;
+    if (child_ready && !spec) return hipErrorInvalidValue;
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
@@ -2667,15 +2824,24 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
         return hipSuccess;
     }
     double2 *Y = spec ? spec : bufA;
-    PF_BEGIN();
-    ICM_TRY(col_fwd(F1, ChildRows{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,ChildRows>", F1), level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
-           nq * nrp * colFl);
+    if (!child_ready) {
+        PF_BEGIN();
+        ICM_TRY(col_fwd(F1, ChildRows{J}, Y, nrp, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,ChildRows>", F1),
+               level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
+    }
     PF_BEGIN();
     k_rowpair_build<RowBuild, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(
         RowSrc{Y, nrp, J.logF}, RowDst{bufB, npairs, J.logF}, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
+    if (next) {   // the next level's child-pair spectra straight from this level's inverse columns
+        PF_BEGIN();
+        ICM_TRY(col_inv_fwd_build(F1, bufB, J, next_spec, npairs, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_inv_fwd_build<%d>", F1), 16.0 * nq * npairs * F + 32.0 * nq * npairs * F,
+               nq * npairs * kF2 * (fft_flops(F1) + fft_flops(2 * F1)));
+        return hipSuccess;
+    }
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StParentPair{J}, npairs, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StParentPair>", F1), 16.0 * nq * npairs * F + 8.0 * nq * nrp * J.parent.count, nq * npairs * kF2 * fft_flops(F1));
@@ -2758,15 +2924,18 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,ChildRows>", F1),
                level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
     }
+    // the root's g is the payout, the same for every q-plane: its pair
+    // spectrum is computed once and read by all planes
+    const int gq = J.gpar.plane == 0 ? 1 : J.nq;
     if (!g_ready) {
         PF_BEGIN();
-        ICM_TRY(col_fwd(F1, GRows{J}, bufA + r1, npairs, J.nq, J.logF, tw, twShift, s));
-        PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+        ICM_TRY(col_fwd(F1, GRows{J}, bufA + r1, npairs, gq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * gq * npairs * F, gq * npairs * colFl);
     }
     PF_BEGIN();
     k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(
-        RowSrc{Y1, nrp, J.logF}, RowSrc{bufA + r1, npairs, J.logF}, RowDst{bufB, nrp, J.logF}, nrp, npairs, J.logF,
-        tw, twShift);
+        RowSrc{Y1, nrp, J.logF}, RowSrc{bufA + r1, npairs, J.logF, gq == 1 && J.nq > 1 ? 0 : 1},
+        RowDst{bufB, nrp, J.logF}, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     if (next) {   // the next level's g pairs straight from this level's inverse columns

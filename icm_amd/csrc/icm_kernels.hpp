@@ -145,8 +145,18 @@ hipError_t launch_prop_lds(const LevelJob &J, const double2 *tw, int twShift, hi
 // transforms of the child pairs there, and the propagate of the same level
 // and q-planes reads them instead of transforming the children again (only
 // without a parent list: slots are then the parents themselves).
+// child_ready: spec already holds this level's child-pair column transforms
+// (written by the level below's fused column inverse).  next / next_spec
+// (optional): the build of the level above (F * 2, a four-step level with
+// column kernels) is fused in: this level's column inverse writes next's
+// child-pair column transforms into next_spec instead of this level's
+// parent coefficients, which are then never stored (only where
+// build_fusable, and where every later reader of them takes the spectra).
 hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
-                              double2 *bufB, hipStream_t s, Prof *pf, double2 *spec = nullptr);
+                              double2 *bufB, hipStream_t s, Prof *pf, double2 *spec = nullptr,
+                              bool child_ready = false, const LevelJob *next = nullptr,
+                              double2 *next_spec = nullptr);
+bool build_fusable(const LevelJob &J, const LevelJob &next);
 // Adjoint tangent build of a large level: dP_X = dP_L P_R + P_L dP_R into
 // J.parent (J.child = P, J.tchild = dP of the children).  specP holds the
 // level's P-pair row-stage inputs (the spectrum cache, written by the P
