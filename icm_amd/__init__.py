@@ -377,8 +377,10 @@ class Plan:
         return self._lib.icm_hip_plan_get_small_path(self._h) == 1
 
     def set_column_fusion(self, on):
-        """Fused column inverse + next level's column forward in the propagate (bit-identical)."""
-        if self._lib.icm_hip_plan_set_column_fusion(self._h, int(bool(on))) != 0:
+        """Column fusion of the four-step levels (bit-identical): True / False, or
+        a mask (1 propagate, 2 build, 4 top build into the root, 8 the F = 8192 build)."""
+        mode = -1 if on is True else (0 if on is False else int(on))
+        if self._lib.icm_hip_plan_set_column_fusion(self._h, mode) != 0:
             raise _err(self._lib, "icm_hip_plan_set_column_fusion")
 
     def set_spectrum_cache(self, on):

@@ -104,7 +104,8 @@ struct IcmHipPlan {
     int maxF = 16;
     bool spec_ok = true;                 // spectrum cache allowed (memory_strategy != 3)
     double frac = 0.6;                   // share of free device memory to plan for
-    bool fuse_columns = true;            // fused column inverse + next level's column forward (propagate)
+    int fuse_columns = 15;               // column fusion bits: 1 propagate, 2 build, 4 build into the root,
+                                         // 8 the F = 8192 build into the level above
     KernelProf prof;
 };
 
@@ -541,23 +542,39 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         auto spec_at = [&](int l) -> double2 * {
             return (p->d_spec && l >= 1 && l <= L - 2 && p->spec_off[l] >= 0) ? p->d_spec + p->spec_off[l] : nullptr;
         };
-        bool child_ready = false;
+        // the root is not built; its propagate transforms its two children:
+        // the top build level may hand them over in bufA instead (the root's
+        // child-pair region), which nothing else writes before the root runs
+        icm::LevelJob root{};
+        root.child = coef_view(L - 2);
+        root.nr_child = g.n_real[L - 2];
+        root.nr_parent = 1;
+        root.np = 1;
+        root.F = g.F[L - 1];
+        root.logF = ilog2i(g.F[L - 1]);
+        root.nq = nq;
+        bool child_ready = false, root_ready = false;
         for (int l = 1; l <= L - 2; ++l) {
             const icm::LevelJob &J = BJ[l];
             p->prof.tag = l;
             const bool large = !use_schoolbook(p, l) && J.F > icm::kMaxLdsF;
-            const bool fuse = large && p->fuse_columns && !p->d_lists && l + 1 <= L - 2 && spec_at(l) &&
-                              spec_at(l + 1) && !use_schoolbook(p, l + 1) && BJ[l + 1].F > icm::kMaxLdsF &&
-                              icm::build_fusable(J, BJ[l + 1]);
+            const bool top = (l == L - 2);
+            const icm::LevelJob &N = top ? root : BJ[l + 1];
+            const int bit = top ? 4 : (icm::col_fused(J.F) ? 8 : 2);
+            const bool fuse = large && (p->fuse_columns & bit) && !p->d_lists &&
+                              (top ? (L >= 3 && g.F[L - 1] > icm::kMaxLdsF && !use_schoolbook(p, L - 1))
+                                   : (spec_at(l + 1) && !use_schoolbook(p, l + 1) && N.F > icm::kMaxLdsF)) &&
+                              icm::build_fusable(J, N);
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_build_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_build_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
-                                                         spec_at(l), child_ready, fuse ? &BJ[l + 1] : nullptr,
-                                                         fuse ? spec_at(l + 1) : nullptr);
+                                                         spec_at(l), child_ready, fuse ? &N : nullptr,
+                                                         fuse ? (top ? p->d_bufA : spec_at(l + 1)) : nullptr);
             if (!hip_ok(e, "build level")) return -1;
             child_ready = fuse;
+            root_ready = top && fuse;
         }
         // propagate: root g = payout (same for every q-plane); jobs of every
         // level first, so that a four-step level can hand the level below its
@@ -594,14 +611,16 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             if (J.np == 0) { g_ready = false; continue; }
             p->prof.tag = -l;
             const bool large = !use_schoolbook(p, l) && J.F > icm::kMaxLdsF;
-            const bool fuse = large && p->fuse_columns && l >= 2 && !use_schoolbook(p, l - 1) &&
+            const bool fuse = large && (p->fuse_columns & 1) && l >= 2 && !use_schoolbook(p, l - 1) &&
                               PJ[l - 1].F > icm::kMaxLdsF && icm::prop_fusable(J, PJ[l - 1]);
             hipError_t e = use_schoolbook(p, l)
                                ? icm::launch_prop_sb(J, s, &p->prof)
                            : (J.F <= icm::kMaxLdsF)
                                ? icm::launch_prop_lds(J, p->d_tw, p->twShift, s, &p->prof)
                                : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
-                                                        (p->d_spec && !J.plist && l <= L - 2 && p->spec_off[l] >= 0)
+                                                        (l == L - 1 && root_ready)
+                                                            ? p->d_bufA
+                                                        : (p->d_spec && !J.plist && l <= L - 2 && p->spec_off[l] >= 0)
                                                             ? p->d_spec + p->spec_off[l]
                                                             : nullptr,
                                                         g_ready, fuse ? &PJ[l - 1] : nullptr);
@@ -1260,7 +1279,7 @@ int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode) {
 
 int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }
-    plan->fuse_columns = on != 0;
+    plan->fuse_columns = on < 0 ? 15 : (on & 15);
     return 0;
 }
 

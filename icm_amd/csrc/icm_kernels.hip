@@ -2304,6 +2304,79 @@ template <class St> struct Col2Dst {
     }
 };
 
+// F = 8192 build level handing the level above (F' = 16384, F1' = 4) its
+// child-pair column transforms: after the tile combine, coefficients n2 and
+// n2 + 4096 of parents 2a, 2a+1 (scaled, exact coefficient 0, zero past the
+// count) and the exact top term (index 8192, column 0) are column n2 of the
+// level above; the 4-point column DFT and its twiddles w_16384^(n2 k1) are
+// the operations k_col_fwd<4, ChildRows> applies to the stored coefficients
+// (one radix-4 pass, then the same products), so the spectra are the same.
+// Only each parent's two end coefficients are stored (for the exact terms
+// of the level above).
+struct Col2DstNext {
+    LevelJob J;          // this level (F = 8192): parents, children, counts
+    double2 *out;        // the level above's child-pair column transforms
+    int nArr;            // parent pairs of this level = parents of the level above
+    int twShift;
+    __device__ __forceinline__ double2 coef(int n, double2 z, bool hx, bool hy, int qi, int p, double invF) const {
+        double x = 0.0, y = 0.0;
+        if (n == 0) {
+            if (hx) x = child_prod_at(J, qi, p, 0);
+            y = hy ? child_prod_at(J, qi, p + 1, 0) : 1.0;   // a missing right child is the identity
+        } else if (n < J.parent.count) {
+            if (hx) x = z.x * invF;
+            if (hy) y = z.y * invF;
+        }
+        return make_double2(x, y);
+    }
+    template <class RS>
+    __device__ __forceinline__ void ifft_store(double2 *T, int a, int qi, int, int, const double2 *tw) const {
+        using TL = typename RS::TL;
+        fft_tile_io<kF2, true, RS::NT, TL, RS::PTS, RS::KEEP>(T, tw, NoIO{}, NoIO{});
+        const int p = 2 * a, pc = J.parent.count, F = J.F;
+        const bool hx = p < J.nr_parent, hy = p + 1 < J.nr_parent;
+        const double invF = 1.0 / (double)F;
+        double *pb = J.parent.base + (long long)qi * J.parent.plane;
+        double2 *o = out + ((long long)qi * nArr + a) * (2 * F);
+        double2 top = make_double2(0.0, 0.0);
+        if (pc > F) {
+            const int tc = J.child.count - 1;
+            if (hx) top.x = child_prod_at(J, qi, p, tc);
+            if (hy) top.y = child_prod_at(J, qi, p + 1, tc);
+        }
+#pragma unroll
+        for (int u = 0; u < kF2 / RS::NTH; ++u) {
+            const int n2 = (int)threadIdx.x + u * RS::NTH;
+            const double2 v0 = T[TL::at(0, n2)];
+            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], tw_big(tw, twShift, n2, 13));
+            double2 a0 = coef(n2, c_add(v0, v1), hx, hy, qi, p, invF);
+            double2 a1 = coef(n2 + kF2, c_sub(v0, v1), hx, hy, qi, p, invF);
+            double2 a2 = n2 == 0 ? top : make_double2(0.0, 0.0);
+            double2 a3 = make_double2(0.0, 0.0);
+            // the end coefficients the level above reads for its exact terms
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int n = n2 + h * kF2;
+                if (n == 0 || n == pc - 1) {
+                    const double2 c = h ? a1 : a0;
+                    if (hx) pb[(long long)p * J.parent.stride + n] = c.x;
+                    if (hy) pb[(long long)(p + 1) * J.parent.stride + n] = c.y;
+                }
+            }
+            if (n2 == 0 && pc > F) {
+                if (hx) pb[(long long)p * J.parent.stride + F] = top.x;
+                if (hy) pb[(long long)(p + 1) * J.parent.stride + F] = top.y;
+            }
+            dft4<false>(a0, a1, a2, a3);
+            const double2 X[4] = {a0, a1, a2, a3};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                o[r * kF2 + n2] = c_mul(X[r], tw_big(tw, twShift, (n2 * r) & (2 * F - 1), 14));
+        }
+        __syncthreads();   // the tiles are free for writing on return
+    }
+};
+
 // Build: V[pair] rows = inverse rows of U = P_p + i P_{p+1}, P = X Y; rows
 // of child pair (slot) p from src, inverse rows of parent pair `pair` to dst.
 template <class RS, class Src, class Dst>
@@ -2797,7 +2870,7 @@ static hipError_t col_inv_fwd_build(int F1, const double2 *in, const LevelJob &J
 }
 
 bool build_fusable(const LevelJob &J, const LevelJob &Jn) {
-    return !col_fused(J.F) && J.F >= 4 * kF2 && J.F <= 1024 * kF2 && Jn.F == 2 * J.F && !J.acc_out &&
+    return (col_fused(J.F) || (J.F >= 4 * kF2 && J.F <= 1024 * kF2)) && Jn.F == 2 * J.F && !J.acc_out &&
            !J.child_r.base && !J.r_missing_zero && !Jn.child_r.base && !Jn.r_missing_zero &&
            Jn.nr_child == J.nr_parent && Jn.child.count == J.parent.count && J.parent.count <= J.F + 1 &&
            Jn.nq == J.nq;
@@ -2812,6 +2885,18 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
+    if (col_fused(J.F) && next) {   // column stage inside the row kernel, the level above's columns after it
+        PF_BEGIN();
+        k_rowpair_build<RowBuild, Col2Src<ChildRows>, Col2DstNext>
+            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
+                                                            Col2DstNext{J, next_spec, npairs, twShift}, nrp,
+                                                            npairs, J.logF, tw, twShift);
+        ICM_TRY(hipGetLastError());
+        PF_END("k_rowpair_build<F=8192,next>",
+               level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * npairs * 2 * F,
+               nq * (nrp + npairs) * (rowFl + colFl) + nq * npairs * 2 * F * 5.0);
+        return hipSuccess;
+    }
     if (col_fused(J.F)) {   // column stage inside the row kernel; nothing for the spectrum cache
         PF_BEGIN();
         k_rowpair_build<RowBuild, Col2Src<ChildRows>, Col2Dst<StParentPair>>

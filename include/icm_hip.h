@@ -249,10 +249,13 @@ int icm_hip_plan_get_schoolbook_max(const IcmHipPlan *plan);
  * table's crossover allows.  mode -1 = tuned (default), 0 = never, 1 =
  * whenever it fits.  get returns 1 when forward runs take this path. */
 int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode);
-/* Column fusion of the forward propagate (default on): a four-step level's
- * column inverse writes the level below's g-pair column transforms directly
- * (no g coefficients stored and read back in between).  Results are
- * bit-identical either way; off is for testing and timing. */
+/* Column fusion of the four-step levels (default all on): a level's column
+ * inverse writes the neighbouring level's column transforms directly instead
+ * of coefficients that the neighbour reads back and transforms.  on: bit 1 the
+ * propagate (g pairs of the level below), bit 2 the build (child pairs of the
+ * level above, into the spectrum cache), bit 4 the top build level into the
+ * root's child pairs, bit 8 the F = 8192 level into the level above; -1 = all.
+ * Results are bit-identical either way; off is for testing and timing. */
 int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on);
 int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
 /* Spectrum cache (no counterpart in the reference, whose large levels are

@@ -217,7 +217,9 @@ def test_failed_adjoint_allocation_leaves_plans_working():
         again = pl.run(p, 16)                                       # no stale error reported
     finally:
         pl.close()
-    assert np.array_equal(again, ref)
+    # the capped plan may run without the spectrum cache (and so without the
+    # column hand-overs that need it): equal to rounding, not bit for bit
+    assert np.max(np.abs(again - ref)) <= 1e-13 * np.max(np.abs(ref))
     eq, ds = icm_amd.Plan(S, len(p), q_batch=4).vjp(p, v, 16)
     assert np.max(np.abs(eq - ref)) <= 1e-13 * np.max(np.abs(ref)) and np.all(np.isfinite(ds))
 
