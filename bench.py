@@ -383,8 +383,9 @@ def main():
     ap.add_argument("--Q", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--only", choices=["all", "forward", "batched"], default="all",
-                    help="forward: the headline pass only (for profiler runs); batched: the ragged batch only")
+    ap.add_argument("--only", choices=["all", "forward", "batched", "small"], default="all",
+                    help="forward: the headline pass only (for profiler runs); batched: the ragged batch only; "
+                         "small: configs 1 and 2 only")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a plain copy kernel of known size, "
                          "the scale of profiler byte counters)")
@@ -411,6 +412,14 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {cnt * 8} bytes read, {cnt * 8} bytes written")
         del src, dst
+    if args.only == "small":
+        nt = args.cpu_threads or max(1, min(16, os.cpu_count() or 1))
+        c1 = config1_field(icm_amd, cpu_threads=nt)
+        c2 = config2_field(icm_amd, cpu_threads=nt)
+        print(json.dumps({"metric": "config 1 latency (ms) and config 2 throughput", "value": c1["gpu_device_ms"],
+                          "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+                          "configs_1_2": {"config1": c1, "config2": c2}}), flush=True)
+        return
     if args.only == "batched":
         batched = batched_field(icm_amd, torch, dev, Q)
         log(f"batched: {batched}")

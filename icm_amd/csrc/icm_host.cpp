@@ -523,8 +523,8 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
         };
         // build; a four-step level whose level above keeps its child-pair
         // spectra in the cache hands them over directly (column fusion: its
-        // parent coefficients are then never read, except by pruned subset
-        // propagates, which switch the fusion off)
+        // parent coefficients are then never read; pruned propagates read
+        // the cache through their parent lists too)
         std::vector<icm::LevelJob> BJ(L);
         for (int l = 1; l <= L - 2; ++l) {
             icm::LevelJob &J = BJ[l];
@@ -561,7 +561,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
             const bool top = (l == L - 2);
             const icm::LevelJob &N = top ? root : BJ[l + 1];
             const int bit = top ? 4 : (icm::col_fused(J.F) ? 8 : 2);
-            const bool fuse = large && (p->fuse_columns & bit) && !p->d_lists &&
+            const bool fuse = large && (p->fuse_columns & bit) &&
                               (top ? (L >= 3 && g.F[L - 1] > icm::kMaxLdsF && !use_schoolbook(p, L - 1))
                                    : (spec_at(l + 1) && !use_schoolbook(p, l + 1) && N.F > icm::kMaxLdsF)) &&
                               icm::build_fusable(J, N);
@@ -620,7 +620,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
                                : icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
                                                         (l == L - 1 && root_ready)
                                                             ? p->d_bufA
-                                                        : (p->d_spec && !J.plist && l <= L - 2 && p->spec_off[l] >= 0)
+                                                        : (p->d_spec && l <= L - 2 && p->spec_off[l] >= 0)
                                                             ? p->d_spec + p->spec_off[l]
                                                             : nullptr,
                                                         g_ready, fuse ? &PJ[l - 1] : nullptr);

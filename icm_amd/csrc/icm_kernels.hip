@@ -2249,10 +2249,12 @@ __device__ __forceinline__ void rowpair_ifft_store(double2 *T, double2 *__restri
 struct RowSrc {
     const double2 *base;
     int narr, logF;
-    int qmul = 1;   // 0: one set of arrays shared by every q-plane (the payout's spectrum at the root)
+    int qmul = 1;              // 0: one set of arrays shared by every q-plane (the payout's spectrum at the root)
+    const int *map = nullptr;  // slot -> array (pruned propagates reading the per-parent spectrum cache)
     template <class RS>
     __device__ __forceinline__ void load_fft(double2 *T, int a, int qi, int rA, int rB, const double2 *tw) const {
-        rowpair_load_fft<RS>(T, base + (((long long)qi * qmul * narr + a) << logF), rA, rB, tw);
+        const int ai = map ? map[a] : a;
+        rowpair_load_fft<RS>(T, base + (((long long)qi * qmul * narr + ai) << logF), rA, rB, tw);
     }
 };
 struct RowDst {
@@ -3001,8 +3003,10 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
                nq * (2 * nrp + npairs) * (rowFl + colFl));
         return hipSuccess;
     }
-    if (spec && J.plist) return hipErrorInvalidValue;   // cached spectra are per parent, not per slot
+    // cached spectra are per parent: pruned launches (slots over a parent
+    // list) read them through the list
     const double2 *Y1 = spec ? spec : bufA;
+    const bool mapped = spec && J.plist;
     if (!spec) {
         PF_BEGIN();
         ICM_TRY(col_fwd(F1, ChildRows{J}, bufA, nrp, J.nq, J.logF, tw, twShift, s));
@@ -3019,8 +3023,9 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     }
     PF_BEGIN();
     k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(
-        RowSrc{Y1, nrp, J.logF}, RowSrc{bufA + r1, npairs, J.logF, gq == 1 && J.nq > 1 ? 0 : 1},
-        RowDst{bufB, nrp, J.logF}, nrp, npairs, J.logF, tw, twShift);
+        mapped ? RowSrc{Y1, J.nr_parent, J.logF, 1, J.plist} : RowSrc{Y1, nrp, J.logF},
+        RowSrc{bufA + r1, npairs, J.logF, gq == 1 && J.nq > 1 ? 0 : 1}, RowDst{bufB, nrp, J.logF}, nrp, npairs,
+        J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
     if (next) {   // the next level's g pairs straight from this level's inverse columns

@@ -264,7 +264,7 @@ int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
  * same q-batch, which then does not transform the children again.  Planned on
  * by default when at least min(q-batch, 32) q-planes fit with it; on = 0
  * frees it, on = 1 allocates it again (-1 when it does not fit).  Pruned
- * subset runs keep their own child transforms. */
+ * subset runs read it through their parent lists. */
 int icm_hip_plan_set_spectrum_cache(IcmHipPlan *plan, int on);
 int icm_hip_plan_get_spectrum_cache(const IcmHipPlan *plan);
 /* Every profiled launch in order: kernel name, tree level (l > 0 build of
