@@ -174,6 +174,10 @@ def roofline_fields(launches, steps, n, k, B, Q, peak_gbs):
                            "frac": level_model(lv) / (t_lv * 1e-9) / 1e9 / peak_gbs,
                            "kernels": sorted({nm for nm, l2, _, _ in launches if l2 == lv})}
     avg_s = ns / cnt * 1e-9
+    # every kernel: launches per step, mean duration, compulsory bytes per launch
+    kernels = {nm: {"launches_per_step": c // steps, "mean_us": t / c / 1e3, "compulsory_GB": b / c / 1e9,
+                    "frac_compulsory": b / t / peak_gbs, "share": t / sum(v[1] for v in tot.values())}
+               for nm, (c, t, b) in sorted(tot.items(), key=lambda kv: -kv[1][1])}
     return name, {
         "bound": "hbm", "achieved": per_launch_model / avg_s / 1e9, "peak": peak_gbs, "unit": "GB/s",
         "frac": per_launch_model / avg_s / 1e9 / peak_gbs, "kernel": name,
@@ -181,7 +185,7 @@ def roofline_fields(launches, steps, n, k, B, Q, peak_gbs):
         "alg_bytes_per_launch": per_launch_model,
         "alg_bytes_basis": f"reference-path model, block size 32 (model level = level + {shift})",
         "compulsory_bytes_per_launch": by / cnt, "frac_compulsory": by / cnt / avg_s / 1e9 / peak_gbs,
-        "levels": levels}
+        "levels": levels, "kernels": kernels}
 
 
 def config1_field(icm_amd, reps=50, cpu_threads=16):
