@@ -125,6 +125,9 @@ _SIGS = [
     ("icm_hip_plan_set_small_path", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_small_path", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_set_column_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_set_poison", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_selfcheck_row_layouts", ctypes.c_int, []),
+    ("icm_hip_plan_poisoned_bytes", ctypes.c_longlong, [ctypes.c_void_p]),
     ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_get_spectrum_cache", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_launch_count", ctypes.c_int, [ctypes.c_void_p]),
@@ -377,11 +380,25 @@ class Plan:
         return self._lib.icm_hip_plan_get_small_path(self._h) == 1
 
     def set_column_fusion(self, on):
-        """Column fusion of the four-step levels (bit-identical): True / False, or
-        a mask (1 propagate, 2 build, 4 top build into the root, 8 the F = 8192 build)."""
+        """Column fusion of the four-step levels: True / False, or a mask (1
+        propagate, 2 build, 4 top build into the root, 8 the F = 8192 build).
+        Bits 1, 2 and 4 give results bit-identical to no fusion; bit 8 changes
+        the rounding of the F = 8192 level's hand-over (1e-13 relative)."""
         mode = -1 if on is True else (0 if on is False else int(on))
         if self._lib.icm_hip_plan_set_column_fusion(self._h, mode) != 0:
             raise _err(self._lib, "icm_hip_plan_set_column_fusion")
+
+    def set_poison(self, on):
+        """Diagnostic: NaN-fill every buffer a q-batch writes before it reads,
+        before each q-batch, so that a read of a position the batch has not
+        written shows up as NaN in the result."""
+        if self._lib.icm_hip_plan_set_poison(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_poison")
+
+    @property
+    def poisoned_bytes(self):
+        """Bytes NaN-filled by the last run (0 with poisoning off)."""
+        return self._lib.icm_hip_plan_poisoned_bytes(self._h)
 
     def set_spectrum_cache(self, on):
         """Keep (True) or recompute (False) the build's column transforms of

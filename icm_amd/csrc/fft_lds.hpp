@@ -67,7 +67,7 @@ template <int FS> struct SwzLayout {
     static constexpr int IS = FS;
     static constexpr int ITEMS = kTile / FS;
     static constexpr int SIZE = kTile;
-    __device__ static __forceinline__ int g(int item) {
+    __host__ __device__ static __forceinline__ int g(int item) {
         if constexpr (FS == 16) {
             const int b0 = item & 1, b1 = (item >> 1) & 1, b2 = (item >> 2) & 1;
             return ((b0 ^ b2) << 3) | (b1 << 2) | (b0 << 1);
@@ -80,7 +80,7 @@ template <int FS> struct SwzLayout {
             return 0;
         }
     }
-    __device__ static __forceinline__ int at(int item, int idx) {
+    __host__ __device__ static __forceinline__ int at(int item, int idx) {
         return item * FS + (idx ^ ((idx >> 3) & 7) ^ g(item));
     }
 };
@@ -102,7 +102,7 @@ template <int FS> struct Swz16Layout {
     static constexpr int ITEMS = kTile / FS;
     static constexpr int SIZE = kTile;
     static constexpr bool kMixed = (ilog2c(FS) % 4) != 0;
-    __device__ static __forceinline__ int at(int item, int idx) {
+    __host__ __device__ static __forceinline__ int at(int item, int idx) {
         if constexpr (kMixed) {
             const int g = (FS == 128) ? ((item & 1) << 3) : 0;
             return item * FS + (idx ^ (((idx >> 3) ^ (idx >> 4)) & 7) ^ g);
@@ -124,7 +124,7 @@ template <int FS> struct ColLayout {
     static constexpr int ITEMS = kTile / FS;
     static constexpr int IS = 1;
     static constexpr int SIZE = kTile;
-    __device__ static __forceinline__ int at(int item, int idx) { return idx * ITEMS + item; }
+    __host__ __device__ static __forceinline__ int at(int item, int idx) { return idx * ITEMS + item; }
 };
 
 // The thread index through an empty asm statement: every pass then derives

@@ -106,6 +106,8 @@ struct IcmHipPlan {
     double frac = 0.6;                   // share of free device memory to plan for
     int fuse_columns = 15;               // column fusion bits: 1 propagate, 2 build, 4 build into the root,
                                          // 8 the F = 8192 build into the level above
+    bool poison = false;                 // diagnostic: NaN-fill the per-batch buffers before every q-batch
+    long long poisoned_bytes = 0;        // bytes NaN-filled by the last run
     KernelProf prof;
 };
 
@@ -475,6 +477,40 @@ bool use_schoolbook(const IcmHipPlan *p, int l) {
            icm::sb_stride_prop(cc, g.G[l - 1]) <= icm::kSbLds;
 }
 
+// Diagnostic (icm_hip_plan_set_poison): fills every device buffer that a
+// q-batch writes before it reads -- block and level products, g ping-pong,
+// four-step scratch, the spectrum caches, the per-chunk partials and, for an
+// adjoint batch, the tangent and B-adjoint arrays -- with all-ones bytes (a
+// NaN in FP64).  A kernel that reads a position the current batch has not
+// written first then turns the result into NaN instead of silently reading a
+// value left by an earlier batch or run.
+bool poison_batch(IcmHipPlan *p, hipStream_t s, bool vjp) {
+    const Geometry &g = p->geo;
+    auto fill = [&](void *x, size_t bytes) {
+        if (!x || bytes == 0) return true;
+        p->poisoned_bytes += (long long)bytes;
+        return HIP_OK(hipMemsetAsync(x, 0xFF, bytes, s));
+    };
+    const size_t qb = (size_t)p->qb;
+    bool ok = fill(p->d_blocks, 8 * qb * g.nb * (g.B + 1)) && fill(p->d_g[0], 8 * qb * p->g_elems) &&
+              fill(p->d_g[1], 8 * qb * p->g_elems) && fill(p->d_partial, 8 * (size_t)p->nchunk * g.n) &&
+              fill(p->d_bufA, 16 * qb * p->big_elems) && fill(p->d_bufB, 16 * qb * p->big_elems) &&
+              fill(p->d_spec, p->spec_bytes);
+    for (int l = 1; ok && l <= g.L - 2; ++l) ok = fill(p->d_coef[l], 8 * qb * g.n_real[l] * g.c[l]);
+    if (!ok || !vjp) return ok;
+    const size_t qv = (size_t)p->qv;
+    ok = fill(p->d_dblocks, 8 * qv * g.nb * (g.B + 1)) && fill(p->d_bv[0], 8 * qv * p->bv_elems) &&
+         fill(p->d_bv[1], 8 * qv * p->bv_elems) && fill(p->d_partial_ds, 8 * (size_t)p->nchunk * g.n);
+    for (int l = 1; ok && l <= g.L - 2; ++l) ok = fill(p->d_dcoef[l], 8 * qv * g.n_real[l] * g.c[l]);
+    if (ok && p->d_dspec) {
+        size_t e = 0;
+        for (int l = 1; l <= g.L - 2; ++l)
+            if (p->dspec_off[l] >= 0) e += qv * g.n_real[l] * g.F[l];
+        ok = fill(p->d_dspec, 16 * e);
+    }
+    return ok;
+}
+
 // Runs the whole quadrature for the active nodes with device payout; leaves
 // the result in d_acc (sorted order) and d_equity (original order).
 int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats) {
@@ -484,6 +520,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
     const int Qa = upload_nodes(p, Q);
     if (Qa < 0) return -1;
     if (use_small(p)) return run_small(p, Qa, d_payout, stats);
+    p->poisoned_bytes = 0;
     hipStream_t s = p->stream;
     const double *d_lv = p->d_nodes;
     const double *d_w = p->d_nodes + p->q_cap;
@@ -497,6 +534,7 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
 
     for (int q0 = 0; q0 < Qa; q0 += p->qb) {
         const int nq = std::min(p->qb, Qa - q0);
+        if (p->poison && !poison_batch(p, s, false)) return -1;
         icm::LeafJob lj{};
         lj.S = p->d_S;
         lj.logv = d_lv + q0;
@@ -748,6 +786,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
     const int Qa = upload_nodes(p, Q);
     if (Qa < 0) return -1;
     hipStream_t s = p->stream;
+    p->poisoned_bytes = 0;
     const double *d_lv = p->d_nodes, *d_w = p->d_nodes + p->q_cap;
     if (!HIP_OK(hipEventRecord(p->ev0, s))) return -1;
     if (d_v_orig) {
@@ -792,6 +831,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
     const int qstep = std::min(p->qb, p->qv);
     for (int q0 = 0; q0 < Qa; q0 += qstep) {
         const int nq = std::min(qstep, Qa - q0);
+        if (p->poison && !poison_batch(p, s, true)) return -1;
         icm::VjpLeafJob V{};
         icm::LeafJob &lj = V.L;
         lj.S = p->d_S; lj.logv = d_lv + q0; lj.wq = d_w + q0;
@@ -1280,6 +1320,21 @@ int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode) {
 int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }
     plan->fuse_columns = on < 0 ? 15 : (on & 15);
+    return 0;
+}
+
+long long icm_hip_plan_poisoned_bytes(const IcmHipPlan *plan) { return plan ? plan->poisoned_bytes : -1; }
+
+int icm_hip_selfcheck_row_layouts(void) {
+    std::string why;
+    if (icm::selfcheck_row_layouts(why)) return 0;
+    set_err("row layout self-check: " + why);
+    return -1;
+}
+
+int icm_hip_plan_set_poison(IcmHipPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    plan->poison = on != 0;
     return 0;
 }
 

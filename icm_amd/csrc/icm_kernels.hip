@@ -15,6 +15,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <string>
+#include <vector>
 
 namespace icm {
 
@@ -2171,13 +2173,13 @@ using RowBuildT = RowR16<false>;
 using RowPropFused = RowR16<false>;   // F = 8192 propagate: its loaders hold more state (33 spills with KEEP)
 using RowPropB = RowR16<false>;       // adjoint B propagate: four g spectra live
 
-template <class TL> __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
+template <class TL> __host__ __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
 
 struct RowRep {
     int ia, ib;      // LDS positions of K and -K (r = 0 of w = 0: bins 0 and 2048)
     bool fixed;      // the two self-mirrored bins of row 0
 };
-template <class TL> __device__ __forceinline__ RowRep row_rep(int w, int r) {
+template <class TL> __host__ __device__ __forceinline__ RowRep row_rep(int w, int r) {
     RowRep q;
     q.fixed = false;
     if (w != 0) {
@@ -2217,10 +2219,13 @@ __device__ __forceinline__ void row_put(double2 *T, const RowRep &q, double2 A, 
     }
 }
 
-// Rows of workgroup w: tile 0 and tile 1.
-__device__ __forceinline__ int row_of(int w, int F1, int h) {
+// Rows of workgroup w: tile 0 and tile 1 (w = 0: the self-mirrored rows 0
+// and F1/2; w > 0: the mirror pair w, F1 - w).  The row-pair kernels run
+// rowpair_groups(F1) workgroups per (pair, q-plane).
+__host__ __device__ __forceinline__ int row_of(int w, int F1, int h) {
     return (w == 0) ? (h ? F1 / 2 : 0) : (h ? F1 - w : w);
 }
+__host__ __device__ constexpr int rowpair_groups(int F1) { return F1 / 2; }
 
 // Load both rows of one [k1][n2] array and transform them; the first radix-16
 // pass reads its points (n2 = (t mod 256) + 256 s, coalesced) straight from global
@@ -2918,7 +2923,7 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
                level_coef_bytes(J.child, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
     }
     PF_BEGIN();
-    k_rowpair_build<RowBuild, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowBuild::NTH, 0, s>>>(
+    k_rowpair_build<RowBuild, RowSrc, RowDst><<<dim3(rowpair_groups(F1), npairs, J.nq), RowBuild::NTH, 0, s>>>(
         RowSrc{Y, nrp, J.logF}, RowDst{bufB, npairs, J.logF}, nrp, npairs, J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build", 16.0 * nq * (nrp + npairs) * F, nq * (nrp + npairs) * rowFl + 30.0 * nq * nrp * F);
@@ -2951,7 +2956,7 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
     PF_END(kname("k_col_fwd<%d,ChildRows>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
            nq * nrp * colFl);
     PF_BEGIN();
-    k_rowpair_build_t<RowBuildT><<<dim3(F1 / 2, npairs, J.nq), RowBuildT::NTH, 0, s>>>(specP, specD, bufB, nrp, npairs,
+    k_rowpair_build_t<RowBuildT><<<dim3(rowpair_groups(F1), npairs, J.nq), RowBuildT::NTH, 0, s>>>(specP, specD, bufB, nrp, npairs,
                                                                                        J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build_t", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
@@ -3022,7 +3027,7 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
         PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * gq * npairs * F, gq * npairs * colFl);
     }
     PF_BEGIN();
-    k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, J.nq), RowProp::NTH, 0, s>>>(
+    k_rowpair_prop<RowProp, RowSrc, RowSrc, RowDst><<<dim3(rowpair_groups(F1), npairs, J.nq), RowProp::NTH, 0, s>>>(
         mapped ? RowSrc{Y1, J.nr_parent, J.logF, 1, J.plist} : RowSrc{Y1, nrp, J.logF},
         RowSrc{bufA + r1, npairs, J.logF, gq == 1 && J.nq > 1 ? 0 : 1}, RowDst{bufB, nrp, J.logF}, nrp, npairs,
         J.logF, tw, twShift);
@@ -3056,7 +3061,7 @@ hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const do
     ICM_TRY(col_fwd(F1, GRows{JR}, bufA, npairs, JdR.nq, JdR.logF, tw, twShift, s));
     PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
     PF_BEGIN();
-    k_rowpair_prop_b<RowPropB, RowSrc, RowSrc, RowSrc, RowSrc, RowDst><<<dim3(F1 / 2, npairs, JdR.nq), RowPropB::NTH, 0, s>>>(
+    k_rowpair_prop_b<RowPropB, RowSrc, RowSrc, RowSrc, RowSrc, RowDst><<<dim3(rowpair_groups(F1), npairs, JdR.nq), RowPropB::NTH, 0, s>>>(
         RowSrc{specP, nrp, JdR.logF}, RowSrc{specD, nrp, JdR.logF}, RowSrc{bufA + r1, npairs, JdR.logF},
         RowSrc{bufA, npairs, JdR.logF}, RowDst{bufB, nrp, JdR.logF}, nrp, npairs, JdR.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
@@ -3213,6 +3218,82 @@ hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *i
     hipError_t e = hipGetLastError();
     PF_END(build ? "k_build_sb_batch" : "k_prop_sb_batch", bytes, flops);
     return e;
+}
+
+
+// Host check of the index maps the row-pair kernels are built on, run by the
+// library's self-check entry point (no device needed):
+//  * every LDS layout maps (transform, index) one to one onto its tile;
+//  * for every F1 = F / 4096 up to 4096, the rows of the rowpair_groups(F1)
+//    workgroups cover the F1 rows exactly once;
+//  * in every workgroup, the representatives r = 0 .. 4095 of row_rep cover
+//    the 2 x 4096 LDS positions of its two tiles exactly once, and each pair
+//    (ia, ib) holds mirror bins of the F-point spectrum: with bin
+//    K = row + F1 k2, bin(ib) = F - bin(ia) mod F (both self-mirrored for the
+//    fixed pair).  A wrong mirror or a position written twice or never would
+//    mix or drop spectra in the row-pair kernels.
+template <class TL> static bool layout_bijective(std::string &why) {
+    std::vector<char> seen(TL::SIZE, 0);
+    for (int it = 0; it < TL::ITEMS; ++it) {
+        for (int idx = 0; idx < kTile / TL::ITEMS; ++idx) {
+            const int a = TL::at(it, idx);
+            if (a < 0 || a >= TL::SIZE || seen[a]++) {
+                why = "layout position out of range or repeated";
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+bool selfcheck_row_layouts(std::string &why) {
+    bool ok = layout_bijective<SwzLayout<16>>(why) && layout_bijective<SwzLayout<32>>(why) &&
+              layout_bijective<SwzLayout<64>>(why) && layout_bijective<SwzLayout<128>>(why) &&
+              layout_bijective<SwzLayout<1024>>(why) && layout_bijective<SwzLayout<4096>>(why) &&
+              layout_bijective<Swz16Layout<128>>(why) && layout_bijective<Swz16Layout<256>>(why) &&
+              layout_bijective<Swz16Layout<512>>(why) && layout_bijective<Swz16Layout<1024>>(why) &&
+              layout_bijective<Swz16Layout<2048>>(why) && layout_bijective<Swz16Layout<4096>>(why) &&
+              layout_bijective<ColLayout<16>>(why) && layout_bijective<ColLayout<512>>(why) &&
+              layout_bijective<ColLayout<4096>>(why);
+    if (!ok) return false;
+    using TL = RowR16<true>::TL;
+    std::vector<int> inv(TL::IS, -1);
+    for (int k2 = 0; k2 < kF2; ++k2) inv[row_at<TL>(k2)] = k2;
+    for (int F1 = 2; F1 <= 4096; F1 *= 2) {
+        const long long F = (long long)F1 * kF2;
+        std::vector<char> row_seen(F1, 0);
+        for (int w = 0; w < rowpair_groups(F1); ++w) {
+            const int rows[2] = {row_of(w, F1, 0), row_of(w, F1, 1)};
+            for (int h = 0; h < 2; ++h) {
+                if (rows[h] < 0 || rows[h] >= F1 || row_seen[rows[h]]++) {
+                    why = "row-pair workgroups do not cover the rows exactly once at F1 = " + std::to_string(F1);
+                    return false;
+                }
+            }
+            std::vector<char> seen(2 * TL::IS, 0);
+            auto bin = [&](int pos) {
+                const int h = pos / TL::IS;
+                return (long long)rows[h] + (long long)F1 * inv[pos % TL::IS];
+            };
+            for (int r = 0; r < kF2; ++r) {
+                const RowRep q = row_rep<TL>(w, r);
+                if (q.ia < 0 || q.ib < 0 || q.ia >= 2 * TL::IS || q.ib >= 2 * TL::IS || seen[q.ia]++ || seen[q.ib]++) {
+                    why = "row representatives do not cover the two tiles exactly once at F1 = " +
+                          std::to_string(F1) + ", w = " + std::to_string(w);
+                    return false;
+                }
+                const long long ka = bin(q.ia), kb = bin(q.ib);
+                const bool mirror = q.fixed ? ((F - ka) % F == ka && (F - kb) % F == kb && ka != kb)
+                                            : ((F - ka) % F == kb);
+                if (!mirror) {
+                    why = "row representative pairs non-mirror bins at F1 = " + std::to_string(F1) +
+                          ", w = " + std::to_string(w) + ", r = " + std::to_string(r);
+                    return false;
+                }
+            }
+        }
+    }
+    return true;
 }
 
 }  // namespace icm

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace icm {
 
@@ -263,5 +264,9 @@ hipError_t launch_small(const SmallJob &J, int G, hipStream_t s, Prof *pf, doubl
 // equity[perm[j]] = sum_{b < G} partial[b][j] in order (0 for inactive j).
 hipError_t launch_small_finalize(const double *partial, int G, int n, const unsigned char *active, const int *perm,
                                  double *acc, double *equity, hipStream_t s, Prof *pf);
+
+// Host check of the row-pair kernels' index maps (no device needed); false
+// with the reason in why on the first violation.
+bool selfcheck_row_layouts(std::string &why);
 
 }  // namespace icm

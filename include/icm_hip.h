@@ -257,6 +257,26 @@ int icm_hip_plan_set_small_path(IcmHipPlan *plan, int mode);
  * root's child pairs, bit 8 the F = 8192 level into the level above; -1 = all.
  * Results are bit-identical either way; off is for testing and timing. */
 int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on);
+
+/* Diagnostic, no counterpart in the reference: on != 0 fills every device
+ * buffer a q-batch writes before it reads (block and level products, g
+ * arrays, four-step scratch, spectrum caches, per-chunk partials; for the
+ * adjoint also its tangent and B arrays) with NaN before each q-batch of
+ * later runs, so that a read of a position the batch has not written itself
+ * shows up as NaN in the result.  Results are otherwise unchanged; the fills
+ * cost one pass over those buffers per q-batch. */
+int icm_hip_plan_set_poison(IcmHipPlan *plan, int on);
+/* Bytes NaN-filled by the plan's last run (0 with poisoning off; -1 for a
+ * null plan). */
+long long icm_hip_plan_poisoned_bytes(const IcmHipPlan *plan);
+
+/* Diagnostic, no counterpart in the reference: checks on the host (no
+ * device needed) the index maps the four-step row-pair kernels rely on --
+ * every LDS layout one to one, the row pairs of the workgroups covering every
+ * row once, and each workgroup's representatives covering both tiles once
+ * with mirror-bin pairs.  0 when they hold, -1 with the first violation in
+ * icm_hip_last_error. */
+int icm_hip_selfcheck_row_layouts(void);
 int icm_hip_plan_get_small_path(const IcmHipPlan *plan);
 /* Spectrum cache (no counterpart in the reference, whose large levels are
  * cuFFT plans per level, src/gpu/gpu_api.cu): the column transforms of each

@@ -33,6 +33,15 @@ def test_python_binding_covers_header():
     assert set(_declared_functions()) <= bound
 
 
+def test_row_pair_index_maps():
+    """The four-step row-pair kernels' index maps, checked on the host by the
+    library itself: layouts one to one, every row owned by one workgroup,
+    representatives covering both tiles once in mirror-bin pairs (F1 = 2 ..
+    4096).  A mirror off by one (ib at K2 - r instead of K2 - 1 - r) fails it."""
+    lib = icm_amd.load_library()
+    assert lib.icm_hip_selfcheck_row_layouts() == 0, lib.icm_hip_last_error()
+
+
 def test_reference_loader_contract():
     """python/icm/__init__.py binds icm_init/icm_equity/icm_equity_subset by
     name with these argtypes; they must resolve in the HIP library."""
