@@ -353,8 +353,10 @@ def jacobian_field(icm_amd, torch, dev, Q, reps=3):
 
 
 # forward + adjoint: one pass returns E and dS = d(v.E)/dS; ratio to the forward
-def fwd_vjp_ratio(icm_amd, torch, dev, Q, Sx, px, reps):
+def fwd_vjp_ratio(icm_amd, torch, dev, Q, Sx, px, reps, checkpoint=None):
     pl = icm_amd.Plan(Sx, len(px))
+    if checkpoint is not None:
+        pl.set_adjoint_checkpoint(checkpoint)
     dp = torch.tensor(px, dtype=torch.float64, device=dev)
     de = torch.empty(len(Sx), dtype=torch.float64, device=dev)
     dd = torch.empty(len(Sx), dtype=torch.float64, device=dev)
@@ -371,10 +373,12 @@ def fwd_vjp_ratio(icm_amd, torch, dev, Q, Sx, px, reps):
         pl.vjp_device(dp.data_ptr(), dv.data_ptr(), de.data_ptr(), dd.data_ptr(), Q)
     torch.cuda.synchronize()
     t_c = time.perf_counter()
+    interval, planes = pl.adjoint_layout
     pl.close()
     fwd_ms = (t_b - t_a) * 1e3 / reps
     vjp_ms = (t_c - t_b) * 1e3 / reps
-    return {"fwd_ms": fwd_ms, "fwd_plus_vjp_ms": vjp_ms, "ratio": vjp_ms / fwd_ms}
+    return {"fwd_ms": fwd_ms, "fwd_plus_vjp_ms": vjp_ms, "ratio": vjp_ms / fwd_ms,
+            "checkpoint_interval": interval, "adjoint_q_planes": planes}
 
 
 def main():
@@ -526,6 +530,9 @@ def main():
     plan.close()
     if args.only == "all":
         adjoint["n1048576_k1048576"] = fwd_vjp_ratio(icm_amd, torch, dev, Q, S, p, 2)
+        # the same with the tangent products of every 4th tree level kept and
+        # the rest recomputed in the reverse sweep (adjoint checkpointing)
+        adjoint["n1048576_k1048576_checkpoint4"] = fwd_vjp_ratio(icm_amd, torch, dev, Q, S, p, 2, checkpoint=4)
         adjoint["jacobians_512_n64"] = jacobian_field(icm_amd, torch, dev, Q)
         log(f"adjoint: {adjoint}")
         batched = batched_field(icm_amd, torch, dev, Q)

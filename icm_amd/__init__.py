@@ -126,6 +126,8 @@ _SIGS = [
     ("icm_hip_plan_get_small_path", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_plan_set_column_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_plan_set_poison", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_set_adjoint_checkpoint", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_plan_adjoint_layout", ctypes.c_int, [ctypes.c_void_p, _I, _I]),
     ("icm_hip_selfcheck_row_layouts", ctypes.c_int, []),
     ("icm_hip_plan_poisoned_bytes", ctypes.c_longlong, [ctypes.c_void_p]),
     ("icm_hip_plan_set_spectrum_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -394,6 +396,22 @@ class Plan:
         written shows up as NaN in the result."""
         if self._lib.icm_hip_plan_set_poison(self._h, int(bool(on))) != 0:
             raise _err(self._lib, "icm_hip_plan_set_poison")
+
+    def set_adjoint_checkpoint(self, interval):
+        """Adjoint memory: keep the tangent products of every ``interval``-th
+        tree level and recompute the rest in the reverse sweep (>= 2), keep
+        every level (0), or choose from free memory (-1)."""
+        if self._lib.icm_hip_plan_set_adjoint_checkpoint(self._h, int(interval)) != 0:
+            raise _err(self._lib, "icm_hip_plan_set_adjoint_checkpoint")
+
+    @property
+    def adjoint_layout(self):
+        """(checkpoint interval in use, q-planes per adjoint batch), or None
+        before the first adjoint call."""
+        m, q = ctypes.c_int(0), ctypes.c_int(0)
+        if self._lib.icm_hip_plan_adjoint_layout(self._h, ctypes.byref(m), ctypes.byref(q)) != 0:
+            return None
+        return m.value, q.value
 
     @property
     def poisoned_bytes(self):

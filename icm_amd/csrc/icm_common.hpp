@@ -208,6 +208,7 @@ struct PlanCfg {
     bool spec_ok = true;     // spectrum cache allowed (memory_strategy != 3)
     bool verbose = false;
     bool graphs = true;      // ragged batches: record / replay a HIP graph
+    int ckpt = -1;           // adjoint checkpoint interval (-1 auto, 0 keep every level, 2 .. 64)
 };
 
 inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, PlanCfg &c) {
@@ -259,6 +260,13 @@ inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, Plan
     if (c.cap == 0) {
         const char *e = getenv("ICM_HIP_MEM_BUDGET");
         if (e && e[0]) c.cap = (size_t)strtoull(e, nullptr, 10);
+    }
+    {
+        const char *e = getenv("ICM_HIP_ADJOINT_CHECKPOINT");
+        if (e && e[0]) {
+            const int v = atoi(e);
+            if (v == -1 || v == 0 || (v >= 2 && v <= 64)) c.ckpt = v;
+        }
     }
     return true;
 }

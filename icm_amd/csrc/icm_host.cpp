@@ -86,7 +86,14 @@ struct IcmHipPlan {
     // adjoint (VJP) buffers, allocated on the first VJP call
     int qv = 0;                          // q-planes per VJP batch
     double *d_dblocks = nullptr;         // [qv][nb][B+1] tangent block products
-    std::vector<double *> d_dcoef;       // level l: [qv][n_real][c] tangent products
+    std::vector<double *> d_dcoef;       // level l: [qv][n_real][c] tangent products (checkpoint levels only
+                                         //   when checkpointing)
+    int ckpt_mode = -1;                  // adjoint checkpointing: -1 auto, 0 keep every tangent level,
+                                         //   m >= 2 keep levels 0, m, 2m, ... and recompute the rest
+    int ckpt = 0;                        // interval in use (0: every tangent level kept)
+    std::vector<double *> d_dseg;        // m - 1 slots: the recomputed tangent levels of one segment
+    long long seg_elems = 0;             // per q-plane per slot
+    size_t vjp_base = 0;                 // device bytes held before the adjoint buffers
     double *d_bv[2] = {nullptr, nullptr};// B adjoint ping-pong, [qv][n_real][c]
     double2 *d_dspec = nullptr;          // tangent spectrum cache: the dP pairs' row-stage inputs,
     std::vector<long long> dspec_off;    //   per four-step build level [qv][n_real[l]][F[l]]
@@ -127,8 +134,10 @@ void free_buffers(IcmHipPlan *p) {
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_bufA); f(p->d_bufB); f(p->d_spec); f(p->d_partial); f(p->d_acc);
     f(p->d_equity); f(p->d_active); f(p->d_inner); f(p->d_lists);
     f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+    for (double *&c : p->d_dseg) f(c);
     f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
     f(p->d_small);
+    p->ckpt = 0;
     p->q_cap = 0;
     p->nodes_Q = 0;
     p->qv = 0;
@@ -250,6 +259,7 @@ IcmHipPlan *plan_create(int n, const double *S, int k, const IcmHipOptions *opts
     p->verbose = cfg.verbose;
     p->spec_ok = cfg.spec_ok;
     p->frac = cfg.frac;
+    p->ckpt_mode = cfg.ckpt;
     p->mem.dry = dry;
     const int B = pick_B(n, k, cfg.B);
     if (!make_geometry(n, k, B, p->geo)) { set_err("invalid geometry"); delete p; return nullptr; }
@@ -502,6 +512,7 @@ bool poison_batch(IcmHipPlan *p, hipStream_t s, bool vjp) {
     ok = fill(p->d_dblocks, 8 * qv * g.nb * (g.B + 1)) && fill(p->d_bv[0], 8 * qv * p->bv_elems) &&
          fill(p->d_bv[1], 8 * qv * p->bv_elems) && fill(p->d_partial_ds, 8 * (size_t)p->nchunk * g.n);
     for (int l = 1; ok && l <= g.L - 2; ++l) ok = fill(p->d_dcoef[l], 8 * qv * g.n_real[l] * g.c[l]);
+    for (size_t i = 0; ok && i < p->d_dseg.size(); ++i) ok = fill(p->d_dseg[i], 8 * qv * p->seg_elems);
     if (ok && p->d_dspec) {
         size_t e = 0;
         for (int l = 1; l <= g.L - 2; ++l)
@@ -708,48 +719,89 @@ int run_plan(IcmHipPlan *p, int Q, const double *d_payout, IcmHipRunStats *stats
 void free_vjp(IcmHipPlan *p, size_t bytes_before) {
     auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
     f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
+    for (double *&c : p->d_dseg) f(c);
     f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_dspec); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
     p->qv = 0;
+    p->ckpt = 0;
     p->mem.bytes = bytes_before;
 }
 
+// Tangent level l's array under checkpoint interval m: its own for every
+// level when m < 2 and for the checkpoint levels l = m, 2m, ...; otherwise
+// slot (l mod m) - 1, which holds one segment's levels at a time.
+static bool ckpt_keeps(int m, int l) { return m < 2 || l % m == 0; }
+static double *dcoef_of(IcmHipPlan *p, int l) {
+    return ckpt_keeps(p->ckpt, l) ? p->d_dcoef[l] : p->d_dseg[l % p->ckpt - 1];
+}
+
 // VJP buffers: tangent block/level products, B adjoint ping-pong, dS arrays.
-// Sized from the plan's memory share of what is free now; on a failed
+// Sized from the plan's memory share of what is free now.  Checkpointing
+// (interval m) keeps the tangent products of levels 0, m, 2m, ... plus m - 1
+// slots for one segment, recomputed from its checkpoint in the down-sweep;
+// auto mode (ckpt_mode -1) takes the interval that leaves the most q-planes
+// when keeping every level leaves fewer than min(q-batch, 32).  On a failed
 // allocation the q-planes are halved, at most 4 times, as at plan creation.
 bool ensure_vjp(IcmHipPlan *p) {
     if (p->qv > 0) return true;
     const Geometry &g = p->geo;
-    long long per = (long long)g.nb * (g.B + 1) * 8;
     long long be = 0;
-    for (int l = 1; l <= g.L - 2; ++l) per += (long long)g.n_real[l] * g.c[l] * 8;
     for (int l = 0; l < g.L; ++l) be = std::max(be, (long long)g.n_real[l] * g.c[l]);
-    per += 2 * be * 8;
+    auto lev = [&](int l) { return (long long)g.n_real[l] * g.c[l]; };
+    auto seg_of = [&](int m) {       // elements per slot
+        long long e = 0;
+        for (int l = 1; l <= g.L - 2; ++l)
+            if (!ckpt_keeps(m, l)) e = std::max(e, lev(l));
+        return e;
+    };
+    auto per_of = [&](int m) {       // bytes per q-plane
+        long long e = (long long)g.nb * (g.B + 1) + 2 * be;
+        for (int l = 1; l <= g.L - 2; ++l)
+            if (ckpt_keeps(m, l)) e += lev(l);
+        if (m >= 2) e += (long long)(m - 1) * seg_of(m);
+        return 8 * e;
+    };
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) return false;
     if (p->mem.cap) freeb = std::min(freeb, p->mem.cap - std::min(p->mem.cap, p->mem.bytes));
     const long long budget = (long long)(p->frac * (double)freeb);
-    int qv = (int)std::max<long long>(1, budget / std::max<long long>(per, 1));
-    qv = std::min(qv, p->qb);
+    auto planes = [&](long long per) {
+        return (int)std::min<long long>(p->qb, std::max<long long>(1, budget / std::max<long long>(per, 1)));
+    };
+    int m = p->ckpt_mode >= 2 ? p->ckpt_mode : 0;
+    int qv = planes(per_of(m));
+    if (p->ckpt_mode == -1 && qv < std::min(p->qb, 32)) {
+        int best = 0;
+        long long bper = per_of(0);
+        for (int mm = 2; mm <= std::max(2, g.L - 2); ++mm)
+            if (per_of(mm) < bper) { bper = per_of(mm); best = mm; }
+        if (best && planes(bper) > qv) { m = best; qv = planes(bper); }
+    }
     // the tangent spectrum cache, beside the forward's, while the q-batch
-    // keeps at least min(qv, 32) planes
+    // keeps at least min(qv, 32) planes (not with checkpointing: recomputed
+    // segments would have to rebuild it)
     bool dspec = false;
-    if (p->d_spec && p->spec_elems > 0) {
-        const int qd = std::min<long long>(p->qb, std::max<long long>(1, budget / (per + 16 * p->spec_elems)));
+    if (m == 0 && p->d_spec && p->spec_elems > 0) {
+        const int qd = planes(per_of(0) + 16 * p->spec_elems);
         if (qd >= std::min(qv, 32)) {
             qv = qd;
             dspec = true;
         }
     }
     p->bv_elems = be;
+    p->seg_elems = m >= 2 ? seg_of(m) : 0;
     const size_t before = p->mem.bytes;
     for (int attempt = 0;; ++attempt) {
+        p->ckpt = m;
         bool ok = dalloc(p, &p->d_dblocks, (size_t)qv * g.nb * (g.B + 1)) &&
                   dalloc(p, &p->d_bv[0], (size_t)qv * be) && dalloc(p, &p->d_bv[1], (size_t)qv * be) &&
                   dalloc(p, &p->d_v, g.n) && dalloc(p, &p->d_partial_ds, (size_t)p->nchunk * g.n) &&
                   dalloc(p, &p->d_acc_ds, g.n) && dalloc(p, &p->d_ds, g.n);
         p->d_dcoef.assign(g.L, nullptr);
         for (int l = 1; ok && l <= g.L - 2; ++l)
-            ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * g.n_real[l] * g.c[l]);
+            if (ckpt_keeps(m, l)) ok = dalloc(p, &p->d_dcoef[l], (size_t)qv * lev(l));
+        p->d_dseg.assign(m >= 2 ? m - 1 : 0, nullptr);
+        for (size_t i = 0; ok && i < p->d_dseg.size(); ++i)
+            ok = p->seg_elems == 0 || dalloc(p, &p->d_dseg[i], (size_t)qv * p->seg_elems);
         if (ok && dspec) {
             p->dspec_off.assign(g.L, -1);
             long long off = 0;
@@ -773,6 +825,7 @@ bool ensure_vjp(IcmHipPlan *p) {
         return false;
     }
     p->qv = qv;
+    p->vjp_base = before;
     return true;
 }
 
@@ -803,7 +856,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             v.base = tangent ? p->d_dblocks : p->d_blocks;
             v.plane = (long long)g.nb * (B + 1); v.stride = B + 1; v.count = g.c[0];
         } else {
-            v.base = tangent ? p->d_dcoef[l] : p->d_coef[l];
+            v.base = tangent ? dcoef_of(p, l) : p->d_coef[l];
             v.plane = (long long)g.n_real[l] * g.c[l]; v.stride = g.c[l]; v.count = g.c[l];
         }
         return v;
@@ -844,7 +897,9 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         V.dblocks = p->d_dblocks;
         V.c0 = g.c[0];
         if (!hip_ok(icm::launch_leaf_build_vjp(V, s, &p->prof), "leaf_build_vjp")) return -1;
-        for (int l = 1; l <= L - 2; ++l) {
+        // P_X and dP_X of level l (recompute: dP_X only; P is kept for every
+        // level, and a fused LDS level rewrites it with the same values)
+        auto build_level = [&](int l, bool recompute) -> bool {
             icm::LevelJob J{};
             J.nr_child = g.n_real[l - 1]; J.nr_parent = g.n_real[l]; J.np = g.n_real[l];
             J.F = g.F[l]; J.logF = ilog2i(g.F[l]); J.nq = nq;
@@ -852,33 +907,43 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
             if (J.F <= icm::kMaxLdsF) {                           // P_X and dP_X in one pass
                 J.tchild = view(l - 1, true);
-                J.tparent = icm::CoefOut{p->d_dcoef[l], J.parent.plane, J.parent.stride, J.parent.count};
-                if (!hip_ok(icm::launch_build_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp build level"))
-                    return -1;
-                continue;
+                J.tparent = icm::CoefOut{dcoef_of(p, l), J.parent.plane, J.parent.stride, J.parent.count};
+                return hip_ok(icm::launch_build_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp build level");
             }
-            if (!level(J, true, spec_of(l))) return -1;           // P_X = P_L P_R
-            J.parent.base = p->d_dcoef[l];
+            if (!recompute && !level(J, true, spec_of(l))) return false;   // P_X = P_L P_R
+            J.parent.base = dcoef_of(p, l);
             if (dspec_of(l)) {                                    // dP_X from both pairs' spectra
                 J.tchild = view(l - 1, true);
-                if (!hip_ok(icm::launch_build_large_t(J, p->d_tw, p->twShift, spec_of(l), dspec_of(l), p->d_bufB, s,
-                                                      &p->prof),
-                            "vjp tangent build level"))
-                    return -1;
-                continue;
+                return hip_ok(icm::launch_build_large_t(J, p->d_tw, p->twShift, spec_of(l), dspec_of(l), p->d_bufB,
+                                                        s, &p->prof),
+                              "vjp tangent build level");
             }
             J.child = view(l - 1, true);
             J.child_r = view(l - 1, false);
-            if (!level(J, true)) return -1;                       // dP_X = dP_L P_R
+            if (!level(J, true)) return false;                    // dP_X = dP_L P_R
             J.child = view(l - 1, false);
             J.child_r = view(l - 1, true);
             J.r_missing_zero = 1;
             J.acc_out = 1;
-            if (!level(J, true)) return -1;                       //      + P_L dP_R
-        }
+            return level(J, true);                                //      + P_L dP_R
+        };
+        for (int l = 1; l <= L - 2; ++l)
+            if (!build_level(l, false)) return -1;
+        // checkpointing: the slots now hold the segment of the highest
+        // non-checkpoint level; a lower segment is rebuilt from its
+        // checkpoint when the down-sweep first needs one of its levels
+        const int m = p->ckpt;
+        int resident = -1;
+        for (int l = L - 2; m >= 2 && l >= 1; --l)
+            if (!ckpt_keeps(m, l)) { resident = l / m; break; }
         int ca = 0, cb = 0;  // current A / B buffers (A in d_g, B in d_bv)
         for (int l = L - 1; l >= 1; --l) {
             const bool root = (l == L - 1);
+            if (m >= 2 && l - 1 >= 1 && !ckpt_keeps(m, l - 1) && (l - 1) / m != resident) {
+                for (int j = (l - 1) / m * m + 1; j <= l - 1; ++j)
+                    if (!build_level(j, true)) return -1;
+                resident = (l - 1) / m;
+            }
             icm::CoefView Apar = root ? icm::CoefView{d_payout, 0, g.k, g.k}
                                       : icm::CoefView{p->d_g[ca], (long long)g.n_real[l] * g.G[l], g.G[l], g.G[l]};
             icm::CoefView Bpar{p->d_bv[cb], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
@@ -1330,6 +1395,28 @@ int icm_hip_selfcheck_row_layouts(void) {
     if (icm::selfcheck_row_layouts(why)) return 0;
     set_err("row layout self-check: " + why);
     return -1;
+}
+
+int icm_hip_plan_set_adjoint_checkpoint(IcmHipPlan *plan, int interval) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (!(interval == -1 || interval == 0 || (interval >= 2 && interval <= 64))) {
+        set_err("adjoint checkpoint interval must be -1 (auto), 0 (keep every level) or 2 .. 64");
+        return -1;
+    }
+    if (interval != plan->ckpt_mode && plan->qv > 0) {
+        if (!HIP_OK(hipSetDevice(g_device)) || !HIP_OK(hipStreamSynchronize(plan->stream))) return -1;
+        free_vjp(plan, plan->vjp_base);   // reallocated on the next adjoint call
+    }
+    plan->ckpt_mode = interval;
+    return 0;
+}
+
+int icm_hip_plan_adjoint_layout(const IcmHipPlan *plan, int *interval, int *q_planes) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (plan->qv <= 0) { set_err("no adjoint buffers yet (they are sized on the first adjoint call)"); return -1; }
+    if (interval) *interval = plan->ckpt;
+    if (q_planes) *q_planes = std::min(plan->qb, plan->qv);
+    return 0;
 }
 
 int icm_hip_plan_set_poison(IcmHipPlan *plan, int on) {

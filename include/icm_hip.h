@@ -265,6 +265,21 @@ int icm_hip_plan_set_column_fusion(IcmHipPlan *plan, int on);
  * later runs, so that a read of a position the batch has not written itself
  * shows up as NaN in the result.  Results are otherwise unchanged; the fills
  * cost one pass over those buffers per q-batch. */
+/* Adjoint memory, no counterpart in the reference (it has no adjoint):
+ * interval m >= 2 keeps the tangent products of tree levels 0, m, 2m, ...
+ * and recomputes each segment between two checkpoints from the lower one
+ * when the reverse sweep reaches it (one extra tangent build per pass, no
+ * tangent spectrum cache); 0 keeps every level; -1 (default, or the
+ * ICM_HIP_ADJOINT_CHECKPOINT environment variable at plan creation) keeps
+ * every level when that leaves at least min(q-batch, 32) q-planes per
+ * adjoint batch and otherwise takes the interval that leaves the most.
+ * Results are bit-identical for every interval.  Changing it frees the
+ * adjoint buffers; the next adjoint call sizes them again. */
+int icm_hip_plan_set_adjoint_checkpoint(IcmHipPlan *plan, int interval);
+/* Interval in use (0: every level kept) and q-planes per adjoint batch,
+ * after the first adjoint call; -1 before it. */
+int icm_hip_plan_adjoint_layout(const IcmHipPlan *plan, int *interval, int *q_planes);
+
 int icm_hip_plan_set_poison(IcmHipPlan *plan, int on);
 /* Bytes NaN-filled by the plan's last run (0 with poisoning off; -1 for a
  * null plan). */

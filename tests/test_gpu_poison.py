@@ -81,13 +81,15 @@ def test_forward_reads_only_what_its_batch_wrote(n, k, q_batch, Q, fusion):
         plan.close()
 
 
+@pytest.mark.parametrize("ckpt", [0, 3])
 @pytest.mark.parametrize("n,q_batch,Q", [(20000, 2, 5), (65536, 1, 3)])
-def test_adjoint_reads_only_what_its_batch_wrote(n, q_batch, Q):
+def test_adjoint_reads_only_what_its_batch_wrote(n, q_batch, Q, ckpt):
     rng, S, p = _inputs(n, n, n + 77)
     v = rng.normal(size=n)
     plan = icm_amd.Plan(S, n, q_batch=q_batch)
     try:
-        say(f"poison adjoint n={n} q_batch={q_batch} Q={Q} cache={plan.spectrum_cache}")
+        plan.set_adjoint_checkpoint(ckpt)
+        say(f"poison adjoint n={n} q_batch={q_batch} Q={Q} checkpoint interval {ckpt} cache={plan.spectrum_cache}")
         e_ref, ds_ref = plan.vjp(p, v, Q=Q)
         plan.set_poison(True)
         e, ds = plan.vjp(p, v, Q=Q)

@@ -208,3 +208,72 @@ def test_vjp_spectrum_cache_matches_recompute(n, k, q_batch, Q):
     err = np.max(np.abs(d1 - d0)) / scale
     say(f"  dS cache vs recompute: {err:.2e} of max|dS|")
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("n,k,q_batch,Q", [(65536, 65536, 0, 16), (20000, 20000, 2, 5), (131072, 1000, 0, 16)])
+def test_vjp_checkpointing_matches_keeping_every_level(n, k, q_batch, Q):
+    """Checkpointing keeps the tangent products of every m-th tree level and
+    rebuilds each segment from its lower checkpoint in the reverse sweep.
+    The rebuilt levels come from the same kernels on the same inputs, so every
+    interval gives the same bits as keeping every level without the tangent
+    spectrum cache (the cache-off plan), and agrees with the default
+    (cached) adjoint to rounding; the oracle's finite differences pin dS at a
+    few coordinates."""
+    rng = np.random.default_rng(n + k + 11)
+    S = np.exp(rng.uniform(0.0, math.log(1e4), n))
+    r = np.arange(k, dtype=float)
+    p = (k - r) ** 2
+    p /= p.sum()
+    v = rng.normal(size=n)
+    plan = icm_amd.Plan(S, k, q_batch=q_batch)
+    L = len(icm_amd.plan_geometry(n, k, plan.summary()["B"])["F"])
+    say(f"vjp checkpoint n={n} k={k} q_batch={q_batch} Q={Q} levels={L}")
+    e_def, d_def = plan.vjp(p, v, Q=Q)
+    assert plan.adjoint_layout[0] == 0
+    runs = {}
+    for m in (2, 3, 4, L):
+        plan.set_adjoint_checkpoint(m)
+        runs[m] = plan.vjp(p, v, Q=Q)
+        say(f"  interval {m}: layout {plan.adjoint_layout}")
+        assert plan.adjoint_layout[0] == m
+    plan.set_adjoint_checkpoint(0)
+    plan.set_spectrum_cache(False)
+    e0, d0 = plan.vjp(p, v, Q=Q)
+    plan.close()
+    for m, (e, d) in runs.items():
+        assert np.array_equal(e, e0) and np.array_equal(d, d0), \
+            f"interval {m}: max |dS diff| {np.max(np.abs(d - d0)):.3e}"
+    err = np.max(np.abs(d0 - d_def)) / np.max(np.abs(d_def))
+    say(f"  checkpointed vs cached adjoint: {err:.2e} of max|dS|")
+    assert err < 1e-11 and np.max(np.abs(e0 - e_def)) <= 1e-12 * np.max(np.abs(e_def))
+
+
+def test_vjp_checkpointing_chosen_under_a_memory_budget():
+    """With a device budget that leaves fewer than 32 q-planes per adjoint
+    batch when every tangent level is kept, the auto mode checkpoints and
+    gets more planes; the results match the unconstrained plan to rounding."""
+    n = k = 65536
+    rng = np.random.default_rng(99)
+    S = rng.uniform(1.0, 100.0, n)
+    p = np.arange(k, 0, -1, dtype=float)
+    p /= p.sum()
+    v = rng.normal(size=n)
+    ref = icm_amd.Plan(S, k, q_batch=64)
+    e_ref, d_ref = ref.vjp(p, v, Q=64)
+    ref.close()
+    # the forward's 64 planes fit; the 150 MB left hold about 23 adjoint
+    # planes with every tangent level (6.3 MB each) and about 35 with
+    # checkpoints (4.2 MB each)
+    fwd, _ = icm_amd.plan_dry_run(n, k, 10 ** 12, q_batch=64)
+    budget = fwd["planned_peak_vram_bytes"] + 150_000_000
+    plan = icm_amd.Plan(S, k, q_batch=64, mem_budget_bytes=budget, mem_fraction=1.0)
+    try:
+        e, d = plan.vjp(p, v, Q=64)
+        m, qv = plan.adjoint_layout
+        say(f"budget {budget / 1e9:.2f} GB: forward q_batch {plan.summary()['q_batch']}, adjoint interval {m}, "
+            f"{qv} planes")
+        assert m >= 2
+    finally:
+        plan.close()
+    assert np.max(np.abs(d - d_ref)) <= 1e-11 * np.max(np.abs(d_ref))
+    assert np.max(np.abs(e - e_ref)) <= 1e-12 * np.max(np.abs(e_ref))
