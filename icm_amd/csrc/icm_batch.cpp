@@ -12,6 +12,12 @@
 // plane.  The first run with a given Q records the launches as one HIP graph;
 // later runs replay it (one hipGraphLaunch per batch).
 //
+// Instances with a level above one LDS transform (F > 4096: k > 2048 and
+// n > 4096) cannot share those launches; a batch that holds some becomes a
+// composite: the other instances run as an inner batch, and each large one
+// through a single-instance plan (four-step levels), writing its segment of
+// the batch's flat arrays.
+//
 // Reference semantics per instance are those of icm_equity
 // (src/cpu/icm.h:26-41, src/cpu/icm.c:2523-2695); the batch entry itself has
 // no counterpart in the reference.
@@ -98,6 +104,16 @@ struct IcmHipBatchPlan {
     DevMem mem;                          // device bytes held (cap accounting)
     int retries = 0;                     // q-pass halvings at creation (out-of-memory policy)
     KernelProf prof;
+    // composite batch: instances whose levels fit one LDS transform run as
+    // `inner` (flat index maps into this batch's arrays), the others through
+    // single-instance plans; d_payout / d_equity / d_vin / d_ds hold the
+    // whole batch in its flat original order
+    bool composite = false;
+    IcmHipBatchPlan *inner = nullptr;
+    std::vector<int> large_ids;
+    std::vector<IcmHipPlan *> large;
+    int *d_kmap = nullptr;               // inner payout index -> batch payout index
+    int *d_nmap = nullptr;               // inner player index -> batch player index
 };
 
 namespace {
@@ -148,6 +164,10 @@ void free_buffers(IcmHipBatchPlan *p) {
 
 void batch_free(IcmHipBatchPlan *p) {
     if (!p) return;
+    if (p->inner) batch_free(p->inner);
+    for (IcmHipPlan *q : p->large) icm_hip_plan_destroy(q);
+    auto f = [](auto *&x) { if (x) (void)hipFree((void *)x); x = nullptr; };
+    f(p->d_kmap); f(p->d_nmap);
     free_buffers(p);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
@@ -376,7 +396,87 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
 }
 
 IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, const int *k_off,
-                              const IcmHipOptions *opts, const IcmHipPlanParams *params = nullptr) {
+                              const IcmHipOptions *opts, const IcmHipPlanParams *params = nullptr);
+
+// A batch with instances whose levels exceed one LDS transform: those get
+// single-instance plans (each planned for an equal share of the memory
+// fraction), the rest an inner batch over compacted offsets.
+IcmHipBatchPlan *composite_create(int n_inst, const int *n_off, const double *S, const int *k_off,
+                                  const IcmHipOptions *opts, const IcmHipPlanParams *params, const PlanCfg &cfg,
+                                  const std::vector<int> &large_ids) {
+    IcmHipBatchPlan *p = new IcmHipBatchPlan();
+    p->composite = true;
+    p->ninst = n_inst;
+    p->Ntot = n_off[n_inst];
+    p->Ktot = k_off[n_inst];
+    p->n_off.assign(n_off, n_off + n_inst + 1);
+    p->k_off.assign(k_off, k_off + n_inst + 1);
+    p->large_ids = large_ids;
+    p->verbose = cfg.verbose;
+    const int nl = (int)large_ids.size();
+    IcmHipPlanParams prm{};
+    if (params) prm = *params;
+    prm.mem_fraction = cfg.frac / (nl + 1);
+    if (cfg.cap) prm.mem_budget_bytes = cfg.cap / (nl + 1);
+    std::vector<char> is_large(n_inst, 0);
+    for (int i : large_ids) {
+        is_large[i] = 1;
+        IcmHipPlan *q = icm_hip_plan_create_ex(n_off[i + 1] - n_off[i], S + n_off[i], k_off[i + 1] - k_off[i], opts,
+                                               &prm);
+        if (!q) {
+            const std::string why = g_err;
+            char b[96];
+            snprintf(b, sizeof b, "batch instance %d (n=%d, k=%d): ", i, n_off[i + 1] - n_off[i],
+                     k_off[i + 1] - k_off[i]);
+            set_err(b + why);
+            batch_free(p);
+            return nullptr;
+        }
+        p->large.push_back(q);
+    }
+    // the other instances, compacted
+    std::vector<int> no{0}, ko{0}, nmap, kmap;
+    std::vector<double> Sc;
+    for (int i = 0; i < n_inst; ++i) {
+        if (is_large[i]) continue;
+        for (int j = n_off[i]; j < n_off[i + 1]; ++j) { Sc.push_back(S[j]); nmap.push_back(j); }
+        for (int j = k_off[i]; j < k_off[i + 1]; ++j) kmap.push_back(j);
+        no.push_back((int)nmap.size());
+        ko.push_back((int)kmap.size());
+    }
+    if (!(HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) && HIP_OK(hipEventCreate(&p->ev0)) &&
+          HIP_OK(hipEventCreate(&p->ev1)))) {
+        batch_free(p);
+        return nullptr;
+    }
+    if (no.size() > 1) {
+        IcmHipPlanParams ip{};
+        if (params) ip = *params;
+        if (cfg.cap) ip.mem_budget_bytes = cfg.cap / (nl + 1);
+        p->inner = batch_create((int)no.size() - 1, no.data(), Sc.data(), ko.data(), opts, &ip);
+        if (!p->inner) { batch_free(p); return nullptr; }
+        if (!balloc(p, &p->d_nmap, nmap.size()) || !balloc(p, &p->d_kmap, kmap.size()) ||
+            !HIP_OK(hipMemcpy(p->d_nmap, nmap.data(), sizeof(int) * nmap.size(), hipMemcpyHostToDevice)) ||
+            !HIP_OK(hipMemcpy(p->d_kmap, kmap.data(), sizeof(int) * kmap.size(), hipMemcpyHostToDevice))) {
+            batch_free(p);
+            return nullptr;
+        }
+        p->B = p->inner->B;
+        p->qb = p->inner->qb;
+    }
+    if (!balloc(p, &p->d_payout, p->Ktot) || !balloc(p, &p->d_equity, p->Ntot) || !balloc(p, &p->d_vin, p->Ntot) ||
+        !balloc(p, &p->d_ds, p->Ntot)) {
+        batch_free(p);
+        return nullptr;
+    }
+    if (p->verbose)
+        fprintf(stderr, "[icm_hip batch] composite: %d instances in the inner batch, %d through single plans\n",
+                n_inst - nl, nl);
+    return p;
+}
+
+IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, const int *k_off,
+                              const IcmHipOptions *opts, const IcmHipPlanParams *params) {
     if (n_inst < 1 || !n_off || !S || !k_off) { set_err("invalid arguments: need n_inst >= 1 and offsets"); return nullptr; }
     if (n_off[0] != 0 || k_off[0] != 0) { set_err("offset arrays must start at 0"); return nullptr; }
     for (int i = 0; i < n_inst; ++i) {
@@ -394,6 +494,20 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     PlanCfg cfg;
     if (!plan_config(opts, params, cfg)) return nullptr;
     if (!ensure_device()) return nullptr;
+    {
+        const int Bb = cfg.B ? cfg.B : kBatchTune.B;
+        std::vector<int> large;
+        for (int i = 0; i < n_inst; ++i) {
+            Geometry g;
+            if (!make_geometry(n_off[i + 1] - n_off[i], k_off[i + 1] - k_off[i], Bb, g)) {
+                set_err("invalid geometry");
+                return nullptr;
+            }
+            for (int l = 1; l < g.L; ++l)
+                if (g.F[l] > icm::kMaxLdsF) { large.push_back(i); break; }
+        }
+        if (!large.empty()) return composite_create(n_inst, n_off, S, k_off, opts, params, cfg, large);
+    }
 
     IcmHipBatchPlan *p = new IcmHipBatchPlan();
     p->ninst = n_inst;
@@ -415,18 +529,6 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
         const int n = n_off[i + 1] - n_off[i], k = k_off[i + 1] - k_off[i];
         Geometry &g = p->geo[i];
         if (!make_geometry(n, k, B, g)) { set_err("invalid geometry"); delete p; return nullptr; }
-        for (int l = 1; l < g.L; ++l) {
-            if (g.F[l] > icm::kMaxLdsF) {
-                char b[192];
-                snprintf(b, sizeof b,
-                         "instance %d (n=%d, k=%d) needs a level transform of %d points; ragged batches take "
-                         "instances whose levels fit %d (k <= 2048 or n <= 4096); use a single-instance plan",
-                         i, n, k, g.F[l], icm::kMaxLdsF);
-                set_err(b);
-                delete p;
-                return nullptr;
-            }
-        }
         p->Lmax = std::max(p->Lmax, g.L);
         p->boff[i] = NB;
         NB += g.nb;
@@ -603,7 +705,7 @@ bool prepare_Q(IcmHipBatchPlan *p, int Q) {
 
 // Adjoint buffers: tangent products like the forward's, B adjoint ping-pong.
 bool ensure_vjp(IcmHipBatchPlan *p, int Q) {
-    if (p->vjp_ready) return true;
+    if (p->vjp_ready || p->composite) return true;   // a composite holds its cotangent / dS arrays from creation
     const int Lm = p->Lmax, qb = p->qb, B = p->B;
     p->bvoff.assign(Lm, std::vector<long long>(p->ninst, 0));
     for (int l = 0; l < Lm; ++l) {
@@ -776,6 +878,54 @@ int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode = 0) {
     return 0;
 }
 
+// Composite run (payout, and cotangent for mode 1, already in d_payout /
+// d_vin): the inner batch on its compacted arrays, then every large instance
+// on its segment.
+int composite_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode) {
+    hipStream_t s = p->stream;
+    double total_ns = 0.0;
+    size_t peak = p->mem.bytes;
+    IcmHipRunStats st{};
+    if (p->inner) {
+        IcmHipBatchPlan *b = p->inner;
+        if (mode == 1 && !ensure_vjp(b, Q)) return -1;
+        if (!hip_ok(icm::launch_gather(p->d_payout, p->d_kmap, b->Ktot, b->d_payout, s), "gather") ||
+            (mode == 1 && !hip_ok(icm::launch_gather(p->d_vin, p->d_nmap, b->Ntot, b->d_vin, s), "gather")) ||
+            !HIP_OK(hipStreamSynchronize(s)))
+            return -1;
+        if (batch_run(b, Q, &st, mode) != 0) return -1;
+        total_ns += st.total_ns;
+        peak += b->mem.bytes;
+        if (!hip_ok(icm::launch_scatter(b->d_equity, b->Ntot, p->d_nmap, p->d_equity, s, nullptr), "scatter") ||
+            (mode == 1 && !hip_ok(icm::launch_scatter(b->d_ds, b->Ntot, p->d_nmap, p->d_ds, s, nullptr), "scatter")))
+            return -1;
+    }
+    if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    for (size_t j = 0; j < p->large.size(); ++j) {
+        const int i = p->large_ids[j];
+        const double *pay = p->d_payout + p->k_off[i];
+        double *eq = p->d_equity + p->n_off[i];
+        const int r = mode == 1 ? icm_hip_vjp_with_plan_device(p->large[j], Q, pay, p->d_vin + p->n_off[i], eq,
+                                                               p->d_ds + p->n_off[i], &st)
+                                : icm_hip_equity_with_plan_device(p->large[j], Q, pay, eq, &st);
+        if (r != 0) return -1;
+        total_ns += st.total_ns;
+        peak += (size_t)st.peak_vram_bytes;
+    }
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->total_ns = total_ns;
+        stats->peak_vram_bytes = peak;
+        stats->engine = 1;
+        stats->B = p->B;
+    }
+    return 0;
+}
+
+int run_any(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode = 0) {
+    return p->composite ? composite_run(p, Q, stats, mode) : batch_run(p, Q, stats, mode);
+}
+
 }  // namespace
 
 extern "C" {
@@ -799,7 +949,7 @@ int icm_hip_batch_equity(IcmHipBatchPlan *plan, int Q, const double *payout, dou
     if (!HIP_OK(hipMemcpyAsync(plan->d_payout, payout, sizeof(double) * plan->Ktot, hipMemcpyHostToDevice,
                                plan->stream)))
         return -1;
-    if (batch_run(plan, Q, stats) != 0) return -1;
+    if (run_any(plan, Q, stats) != 0) return -1;
     return HIP_OK(hipMemcpy(equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost)) ? 0 : -1;
 }
 
@@ -810,7 +960,7 @@ int icm_hip_batch_equity_device(IcmHipBatchPlan *plan, int Q, const double *d_pa
     if (!HIP_OK(hipMemcpyAsync(plan->d_payout, d_payout, sizeof(double) * plan->Ktot, hipMemcpyDeviceToDevice,
                                plan->stream)))
         return -1;
-    if (batch_run(plan, Q, stats) != 0) return -1;
+    if (run_any(plan, Q, stats) != 0) return -1;
     if (!HIP_OK(hipMemcpyAsync(d_equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice,
                                plan->stream)))
         return -1;
@@ -826,7 +976,7 @@ int icm_hip_batch_vjp(IcmHipBatchPlan *plan, int Q, const double *payout, const 
                                plan->stream)) ||
         !HIP_OK(hipMemcpyAsync(plan->d_vin, v, sizeof(double) * plan->Ntot, hipMemcpyHostToDevice, plan->stream)))
         return -1;
-    if (batch_run(plan, Q, stats, 1) != 0) return -1;
+    if (run_any(plan, Q, stats, 1) != 0) return -1;
     if (equity && !HIP_OK(hipMemcpy(equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost)))
         return -1;
     if (dS && !HIP_OK(hipMemcpy(dS, plan->d_ds, sizeof(double) * plan->Ntot, hipMemcpyDeviceToHost))) return -1;
@@ -842,7 +992,7 @@ int icm_hip_batch_vjp_device(IcmHipBatchPlan *plan, int Q, const double *d_payou
     if (!HIP_OK(hipMemcpyAsync(plan->d_payout, d_payout, sizeof(double) * plan->Ktot, hipMemcpyDeviceToDevice, s)) ||
         !HIP_OK(hipMemcpyAsync(plan->d_vin, d_v, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice, s)))
         return -1;
-    if (batch_run(plan, Q, stats, 1) != 0) return -1;
+    if (run_any(plan, Q, stats, 1) != 0) return -1;
     if (d_equity &&
         !HIP_OK(hipMemcpyAsync(d_equity, plan->d_equity, sizeof(double) * plan->Ntot, hipMemcpyDeviceToDevice, s)))
         return -1;
@@ -857,6 +1007,7 @@ int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const in
     IcmHipBatchPlan *p = batch_create(n_inst, n_off, S, k_off, nullptr);
     if (!p) return -1;
     p->use_graph = false;   // one run: recording a graph would not pay
+    if (p->inner) p->inner->use_graph = false;
     const int r = icm_hip_batch_equity(p, Q, payout, equity, nullptr);
     batch_free(p);
     return r;
@@ -864,6 +1015,11 @@ int icm_hip_equity_batch(int n_inst, const int *n_off, const double *S, const in
 
 int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
     if (!plan) { set_err("null plan"); return -1; }
+    if (plan->composite) {
+        for (IcmHipPlan *q : plan->large)
+            if (icm_hip_plan_set_schoolbook_max(q, cmax) != 0) return -1;
+        return plan->inner ? icm_hip_batch_set_schoolbook_max(plan->inner, cmax) : 0;
+    }
     plan->sb_max = cmax < 0 ? 0 : cmax;
     drop_graphs(plan);                  // job tables change with the crossover
     free_passes(plan, plan->passes);
@@ -874,15 +1030,21 @@ int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
 
 int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on) {
     if (!plan) { set_err("null plan"); return -1; }
+    if (plan->composite) return plan->inner ? icm_hip_batch_set_profiling(plan->inner, on) : 0;
     plan->prof.enabled = on != 0;
     plan->prof.clear();
     return 0;
 }
 
-int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan) { return plan ? (int)plan->prof.agg.size() : 0; }
+int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan) {
+    if (plan && plan->composite) return plan->inner ? icm_hip_batch_kernel_count(plan->inner) : 0;
+    return plan ? (int)plan->prof.agg.size() : 0;
+}
 
 int icm_hip_batch_kernel_stat(const IcmHipBatchPlan *plan, int index, char *name, int name_len, int *launches,
                               double *total_ns, double *alg_bytes, double *alg_flops) {
+    if (plan && plan->composite && plan->inner)
+        return icm_hip_batch_kernel_stat(plan->inner, index, name, name_len, launches, total_ns, alg_bytes, alg_flops);
     if (!plan || index < 0 || index >= (int)plan->prof.agg.size()) { set_err("bad kernel index"); return -1; }
     const auto &a = plan->prof.agg[index];
     if (name && name_len > 0) {
@@ -899,6 +1061,31 @@ int icm_hip_batch_kernel_stat(const IcmHipBatchPlan *plan, int index, char *name
 int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_players, int *n_blocks, int *max_levels,
                           int *q_batch, size_t *device_bytes) {
     if (!plan) { set_err("null plan"); return -1; }
+    if (plan->composite) {   // the inner batch's figures plus the single plans'
+        int nb = 0, lm = 0, qb = 0;
+        size_t bytes = plan->mem.bytes;
+        if (plan->inner) {
+            nb = plan->inner->NB;
+            lm = plan->inner->Lmax;
+            qb = plan->inner->qb;
+            bytes += plan->inner->mem.bytes;
+        }
+        for (IcmHipPlan *q : plan->large) {
+            IcmHipPlanSummary sm{};
+            if (icm_hip_plan_summary(q, &sm) != 0) return -1;
+            nb += (sm.n + sm.B - 1) / sm.B;
+            lm = std::max(lm, sm.n_levels);
+            qb = qb ? std::min(qb, sm.q_batch) : sm.q_batch;
+            bytes += sm.planned_peak_vram_bytes;
+        }
+        if (n_inst) *n_inst = plan->ninst;
+        if (n_players) *n_players = plan->Ntot;
+        if (n_blocks) *n_blocks = nb;
+        if (max_levels) *max_levels = lm;
+        if (q_batch) *q_batch = qb;
+        if (device_bytes) *device_bytes = bytes;
+        return 0;
+    }
     if (n_inst) *n_inst = plan->ninst;
     if (n_players) *n_players = plan->Ntot;
     if (n_blocks) *n_blocks = plan->NB;

@@ -318,10 +318,12 @@ int icm_hip_plan_kernel_stat(const IcmHipPlan *plan, int index, char *name, int 
  * n_off[i+1]-1 of S and payout entries k_off[i] .. k_off[i+1]-1 of payout
  * (n_off[0] = k_off[0] = 0, 1 <= k_i <= n_i); equities come back in the flat
  * order of S.  Each instance gets what icm_equity(n_i, S_i, Q, payout_i,
- * k_i, equity_i) computes, with its own nodes (its own max stack).  Every
- * tree level of every instance must fit one LDS transform (<= 4096 points:
- * k_i <= 2048, or n_i <= 4096); other instances are rejected at plan
- * creation with a message naming them.  opts: NULL or enable_graphs != 0
+ * k_i, equity_i) computes, with its own nodes (its own max stack).
+ * Instances whose tree levels all fit one LDS transform (<= 4096 points:
+ * k_i <= 2048, or n_i <= 4096) share one launch per tree level; the others
+ * (four-step levels) each run through a single-instance plan inside the
+ * batch, planned for an equal share of the memory fraction.
+ * opts: NULL or enable_graphs != 0
  * records the first run for each Q as one HIP graph and replays it;
  * verbose; the other fields as for single plans.  Calls on one batch plan
  * must not overlap. */

@@ -136,7 +136,39 @@ def test_batch_edge_instances():
     assert np.max(np.abs(one - out[3])) <= 1e-15
 
 
-def test_batch_rejects_instances_beyond_lds_levels():
-    S = np.random.default_rng(1).uniform(1, 100, 8192)
-    with pytest.raises(icm_amd.ICMError, match="single-instance plan"):
-        icm_amd.BatchPlan([np.ones(10), S], [3, 8192])
+def test_batch_with_four_step_instances_vs_oracle():
+    """Instances with a level above one LDS transform (k > 2048 and n > 4096)
+    run inside the batch through single-instance plans (four-step levels):
+    a mixed batch holding an n = 65536, k = n instance and an n = 8192,
+    k = 3000 one beside batched instances, every instance against the
+    oracle (rel < 1e-8, abs <= 1e-11 max|E|); a second run equals the first;
+    the batch adjoint matches each instance's own adjoint."""
+    rng = np.random.default_rng(65536)
+    sizes = [(300, 20), (65536, 65536), (1000, 1000), (8192, 3000), (17, 5), (4096, 4096)]
+    stacks = [rng.lognormal(0.0, 1.0, n) * 100.0 for n, _ in sizes]
+    payouts = []
+    for _, k in sizes:
+        p = 0.8 ** np.arange(k)
+        payouts.append(p / p.sum())
+    bp = icm_amd.BatchPlan(stacks, [len(p) for p in payouts])
+    try:
+        info = bp.summary()
+        say(f"mixed batch: {info}")
+        out = bp.run(payouts, 256)
+        again = bp.run(payouts, 256)
+        v = [rng.normal(size=len(S)) for S in stacks]
+        eq_v, ds_v = bp.vjp(payouts, v, 256)
+    finally:
+        bp.close()
+    for i, (S, p) in enumerate(zip(stacks, payouts)):
+        ref = O.equity(S, p, Q=256, B=32)
+        r = rel_err(out[i], ref)
+        a = float(np.max(np.abs(out[i] - ref)) / np.max(np.abs(ref)))
+        say(f"  instance {i} n={len(S)} k={len(p)}: rel {r:.2e}, abs {a:.2e}")
+        assert r < 1e-8 and a <= 1e-11
+        assert np.array_equal(out[i], again[i])
+        assert np.max(np.abs(eq_v[i] - out[i])) <= 1e-13 * np.max(np.abs(out[i]))
+        e1, d1 = icm_amd.equity_vjp(S, p, v[i], Q=256)
+        d_err = float(np.max(np.abs(ds_v[i] - d1)) / np.max(np.abs(d1)))
+        say(f"    adjoint vs single instance: {d_err:.2e} of max|dS|")
+        assert d_err < 1e-11
