@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -126,6 +127,188 @@ __device__ __forceinline__ double leaf_psi(const double *P, const double *g, dou
     return eq;
 }
 
+// leaf_psi for PPL players of one block at once: each coefficient read from
+// LDS feeds PPL recurrences.  Per player the operations and their order are
+// those of leaf_psi (the branch not taken by a player is computed for the
+// others and discarded).
+template <int B, int PPL>
+__device__ __forceinline__ void leaf_psi_n(const double *P, const double *g, const double *a, int bsize, int pk,
+                                           double *eq) {
+    bool fw = false, bw = false, zr = false;
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        fw |= a[u] > 0.5;
+        bw |= (a[u] <= 0.5) && (a[u] > 1e-15);
+        zr |= a[u] <= 1e-15;
+    }
+    double ef[PPL], eb[PPL], ez = 0.0;
+    if (fw) {
+        double ia[PPL], c[PPL], qv[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            const double bm = 1.0 - a[u];
+            ia[u] = 1.0 / a[u];
+            c[u] = -bm * ia[u];
+            qv[u] = P[0] * ia[u];
+            ef[u] = g[0] * qv[u];
+        }
+        for (int m = 1; m < pk; ++m) {
+            const double pm = P[m], gm = g[m];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                qv[u] = c[u] * qv[u] + pm * ia[u];
+                ef[u] += gm * qv[u];
+            }
+        }
+    }
+    if (bw) {
+        double ib[PPL], c[PPL], qv[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            const double bm = 1.0 - a[u];
+            ib[u] = 1.0 / bm;
+            c[u] = -a[u] * ib[u];
+            qv[u] = P[bsize] * ib[u];
+            eb[u] = 0.0;
+            if (bsize - 1 < pk) eb[u] += g[bsize - 1] * qv[u];
+        }
+        int m = bsize - 2;
+        for (; m >= pk; --m) {
+            const double pm = P[m + 1];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) qv[u] = c[u] * qv[u] + pm * ib[u];
+        }
+        for (; m >= 0; --m) {
+            const double pm = P[m + 1], gm = g[m];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                qv[u] = c[u] * qv[u] + pm * ib[u];
+                eb[u] += gm * qv[u];
+            }
+        }
+    }
+    if (zr)
+        for (int m = 0; m < pk; ++m) ez += g[m] * P[m + 1];
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) eq[u] = a[u] > 0.5 ? ef[u] : (a[u] > 1e-15 ? eb[u] : ez);
+}
+
+// Leaf extract with PPL players of one block per lane (players li, li + B/PPL,
+// ...): B/PPL lanes per block, 256 PPL / B blocks per workgroup.  LDS reads
+// per player drop by PPL (the LDS-staged broadcast reads of the block
+// products and g bound the one-player kernel).  Same results, bit for bit,
+// as k_leaf_extract.
+template <int B, int PPL>
+__global__ __launch_bounds__(256) void k_leaf_extract_n(LeafJob J) {
+    constexpr int LPB = B / PPL;                 // lanes per block
+    constexpr int BPW = 256 / LPB;               // blocks per workgroup
+    constexpr int NP = BPW * (B + 1);            // block coefficients per q-plane
+    constexpr int NPT = (NP + 255) / 256;        // of them per thread
+    constexpr int NG = BPW * B;                  // g entries per q-plane (PPL per thread)
+    static_assert(LPB * PPL == B && BPW * LPB == 256, "lanes must tile the blocks");
+    __shared__ double shP[2][NP];
+    __shared__ double shG[2][NG];
+    const int t = threadIdx.x;
+    const int lb = t / LPB, li = t % LPB;
+    const int sfirst = blockIdx.x * BPW;
+    auto block_of = [&](int slot) { return slot < J.nbl ? (J.blist ? J.blist[slot] : slot) : -1; };
+    const int b = block_of(sfirst + lb);
+    const int chunk = blockIdx.y;
+    const int q0 = chunk * J.q_per_chunk;
+    const int q1 = min(q0 + J.q_per_chunk, J.nq);
+    const int bsize = (b >= 0) ? min(B, J.n - b * B) : 0;
+    const int pk = min(J.G0, bsize);
+    int jj[PPL];
+    bool valid[PPL], act[PPL];
+    double Sj[PPL];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        jj[u] = b * B + li + u * LPB;
+        valid[u] = (b >= 0) && (jj[u] < J.n);
+        Sj[u] = valid[u] ? J.S[jj[u]] : 0.0;
+        act[u] = valid[u] && (!J.active || J.active[jj[u]]);
+        any |= act[u];
+    }
+    const long long bplane = (long long)J.nb * (B + 1);
+    int bP[NPT], mP[NPT], bG[PPL], mG[PPL];
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int e = t + i * 256;
+        bP[i] = (e < NP) ? block_of(sfirst + e / (B + 1)) : -1;
+        mP[i] = e % (B + 1);
+    }
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+        const int e = t + i * 256;
+        bG[i] = block_of(sfirst + e / B);
+        mG[i] = e % B;
+        if (mG[i] >= J.G0) bG[i] = -1;
+    }
+    double pr[NPT], gr[PPL];
+    auto fetch = [&](int q) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i)
+            pr[i] = (bP[i] >= 0) ? J.blocks[q * bplane + (long long)bP[i] * (B + 1) + mP[i]] : 0.0;
+#pragma unroll
+        for (int i = 0; i < PPL; ++i)
+            gr[i] = (bG[i] >= 0) ? J.g[q * J.g_plane + (long long)bG[i] * J.g_stride + mG[i]] : 0.0;
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i)
+            if (t + i * 256 < NP) shP[buf][t + i * 256] = pr[i];
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) shG[buf][t + i * 256] = gr[i];
+    };
+    if (q0 < q1) {
+        fetch(q0);
+        stash(0);
+    }
+    __syncthreads();
+    double acc[PPL];
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) acc[u] = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        const int cur = (q - q0) & 1;
+        const bool more = q + 1 < q1;
+        if (more) fetch(q + 1);
+        if (any) {
+            const double lv = J.logv[q];
+            double a[PPL], eq[PPL];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u)
+                a[u] = !act[u] ? 0.75 : (J.a_in ? J.a_in[(long long)q * J.n + jj[u]] : a_of(Sj[u], lv));
+            leaf_psi_n<B, PPL>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk, eq);
+            const double el = exp(-lv), wq = J.wq[q];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                if (!act[u]) continue;
+                if (J.inner_out) J.inner_out[(long long)q * J.n + jj[u]] = eq[u];
+                double pw = wq * Sj[u] * a[u] * el;
+                if (!isfinite(pw)) pw = 0.0;
+                acc[u] += pw * eq[u];
+            }
+        }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+    }
+    if (J.partial) {
+#pragma unroll
+        for (int u = 0; u < PPL; ++u)
+            if (valid[u]) J.partial[(long long)chunk * J.n + jj[u]] = act[u] ? acc[u] : 0.0;
+    }
+}
+
+// Players per lane of the B = 64 leaf extracts: 2, or 1 when
+// ICM_HIP_LEAF_PPL=1 (read at each launch; a captured batch graph keeps the
+// choice of its capture) to compare with the one-player kernels.  Four per
+// lane measured slower (171 VGPRs, two waves per SIMD).
+inline int leaf_ppl() {
+    const char *e = getenv("ICM_HIP_LEAF_PPL");
+    return (e && atoi(e) == 1) ? 1 : 2;
+}
+
 template <int B>
 __global__ __launch_bounds__(256) void k_leaf_extract(LeafJob J) {
     constexpr int BPW = 256 / B;
@@ -227,6 +410,117 @@ __global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__rest
         }
     }
     store_block_products<B>(P, sh, blocks + (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1), min(64, NB - b0));
+}
+
+// Ragged-batch leaf extract with PPL players of one block per lane (as
+// k_leaf_extract_n); the same results, bit for bit, as k_leaf_extract_batch.
+template <int B, int PPL>
+__global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *__restrict__ inst,
+                                                              const int *__restrict__ blk_inst,
+                                                              const double *__restrict__ S,
+                                                              const double *__restrict__ blocks, int NB, int nq,
+                                                              int q0, int q_per_chunk, double *partial, int Ntot) {
+    constexpr int LPB = B / PPL;
+    constexpr int BPW = 256 / LPB;
+    constexpr int NP = BPW * (B + 1);
+    constexpr int NPT = (NP + 255) / 256;
+    constexpr int NG = BPW * B;
+    static_assert(LPB * PPL == B && BPW * LPB == 256, "lanes must tile the blocks");
+    __shared__ double shP[2][NP];
+    __shared__ double shG[2][NG];
+    const int t = threadIdx.x;
+    const int lb = t / LPB, li = t % LPB;
+    const int gfirst = blockIdx.x * BPW;
+    const int gb = gfirst + lb;
+    const int chunk = blockIdx.y;
+    const int qa = chunk * q_per_chunk;
+    const int qe = min(qa + q_per_chunk, nq);
+    const BatchInst *I = (gb < NB) ? &inst[blk_inst[gb]] : nullptr;
+    const int b = I ? gb - I->boff : 0;
+    const int bsize = I ? min(B, I->n - b * B) : 0;
+    const int pk = I ? min(I->G0, bsize) : 0;
+    int jj[PPL];
+    bool valid[PPL];
+    double Sj[PPL];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        jj[u] = b * B + li + u * LPB;
+        valid[u] = I && jj[u] < I->n;
+        Sj[u] = valid[u] ? S[I->poff + jj[u]] : 0.0;
+        any |= valid[u];
+    }
+    const long long bplane = (long long)NB * (B + 1);
+    int gP[NPT], mP[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int e = t + i * 256;
+        gP[i] = (e < NP) ? gfirst + e / (B + 1) : NB;
+        mP[i] = e % (B + 1);
+    }
+    const double *gbase[PPL];
+    long long gplane[PPL];
+#pragma unroll
+    for (int i = 0; i < PPL; ++i) {
+        const int e = t + i * 256;
+        const int gG = gfirst + e / B, mG = e % B;
+        gbase[i] = nullptr;
+        gplane[i] = 0;
+        if (gG < NB) {
+            const BatchInst &I2 = inst[blk_inst[gG]];
+            if (mG < I2.G0) {
+                gbase[i] = I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG;
+                gplane[i] = I2.g0.plane;
+            }
+        }
+    }
+    double pr[NPT], gr[PPL];
+    auto fetch = [&](int q) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i)
+            pr[i] = (gP[i] < NB) ? blocks[q * bplane + (long long)gP[i] * (B + 1) + mP[i]] : 0.0;
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) gr[i] = gbase[i] ? gbase[i][q * gplane[i]] : 0.0;
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NPT; ++i)
+            if (t + i * 256 < NP) shP[buf][t + i * 256] = pr[i];
+#pragma unroll
+        for (int i = 0; i < PPL; ++i) shG[buf][t + i * 256] = gr[i];
+    };
+    if (qa < qe) {
+        fetch(qa);
+        stash(0);
+    }
+    __syncthreads();
+    double acc[PPL];
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) acc[u] = 0.0;
+    for (int q = qa; q < qe; ++q) {
+        const int cur = (q - qa) & 1;
+        const bool more = q + 1 < qe;
+        if (more) fetch(q + 1);
+        if (any) {
+            const double lv = I->logv[q0 + q];
+            double a[PPL], eq[PPL];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) a[u] = valid[u] ? a_of(Sj[u], lv) : 0.75;
+            leaf_psi_n<B, PPL>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk, eq);
+            const double el = exp(-lv), wq = I->w[q0 + q];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                double pw = wq * Sj[u] * a[u] * el;
+                if (!isfinite(pw)) pw = 0.0;
+                acc[u] += pw * eq[u];
+            }
+        }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < PPL; ++u)
+        if (valid[u]) partial[(long long)chunk * Ntot + I->poff + jj[u]] = acc[u];
 }
 
 template <int B>
@@ -2628,7 +2922,12 @@ hipError_t launch_leaf_extract(const LeafJob &J, int nchunk, hipStream_t s, Prof
     switch (J.B) {
     case 16: k_leaf_extract<16><<<grid, 256, 0, s>>>(J); break;
     case 32: k_leaf_extract<32><<<grid, 256, 0, s>>>(J); break;
-    case 64: k_leaf_extract<64><<<grid, 256, 0, s>>>(J); break;
+    case 64: {
+        const int ppl = leaf_ppl();
+        if (ppl == 2) k_leaf_extract_n<64, 2><<<dim3((J.nbl + 7) / 8, nchunk), 256, 0, s>>>(J);
+        else k_leaf_extract<64><<<grid, 256, 0, s>>>(J);
+        break;
+    }
     default: return hipErrorInvalidValue;
     }
     const double nq = J.nq, B = J.B;
@@ -3108,10 +3407,16 @@ hipError_t launch_leaf_extract_batch(int B, const BatchInst *inst, const int *bl
         k_leaf_extract_batch<32><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial,
                                                      Ntot);
         break;
-    case 64:
-        k_leaf_extract_batch<64><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial,
-                                                     Ntot);
+    case 64: {
+        const int ppl = leaf_ppl();
+        if (ppl == 2)
+            k_leaf_extract_batch_n<64, 2><<<dim3((NB + 7) / 8, nchunk), 256, 0, s>>>(
+                inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk, partial, Ntot);
+        else
+            k_leaf_extract_batch<64><<<grid, 256, 0, s>>>(inst, blk_inst, S, blocks, NB, nq, q0, q_per_chunk,
+                                                         partial, Ntot);
         break;
+    }
     default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -172,3 +172,20 @@ def test_batch_with_four_step_instances_vs_oracle():
         d_err = float(np.max(np.abs(ds_v[i] - d1)) / np.max(np.abs(d1)))
         say(f"    adjoint vs single instance: {d_err:.2e} of max|dS|")
         assert d_err < 1e-11
+
+
+def test_batch_leaf_extract_players_per_lane_bit_identical(monkeypatch):
+    """The batch leaf extract with 2 players of a block per lane gives the
+    same bits as one player per lane (a fresh plan per setting: a captured
+    graph keeps the kernel it was recorded with)."""
+    stacks, payouts = ragged_instances(400, seed=21, n_max=3000)
+    out = {}
+    for ppl in ("1", "2"):
+        monkeypatch.setenv("ICM_HIP_LEAF_PPL", ppl)
+        bp = icm_amd.BatchPlan(stacks, [len(p) for p in payouts])
+        try:
+            out[ppl] = np.concatenate(bp.run(payouts, 256))
+        finally:
+            bp.close()
+    say(f"batch leaf extract per-lane players: {len(stacks)} instances")
+    assert np.array_equal(out["1"], out["2"])

@@ -428,3 +428,33 @@ def test_column_fusion_bit_identical(n, k, Q):
     # compiler may contract products differently: rounding-level agreement
     assert np.array_equal(fused, plain)
     assert np.max(np.abs(full - plain)) <= 1e-13 * max(np.max(np.abs(plain)), p.sum())
+
+
+@pytest.mark.parametrize("n,k", [(20000, 20000), (65536, 1000), (5000, 37), (4133, 64), (100000, 100000)])
+def test_leaf_extract_players_per_lane_bit_identical(n, k, monkeypatch):
+    """The B = 64 leaf extract with 2 players of a block per lane (one LDS
+    read feeding two division recurrences) gives the same bits as one
+    player per lane: full and partial blocks, k below one block, pruned
+    subsets and the per-node entry."""
+    rng = np.random.default_rng(n + k)
+    S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+    p = np.arange(k, 0, -1, dtype=float) ** 1.5
+    p /= p.sum()
+    a = rng.uniform(0.0, 1.0, n)
+    a[::7] = 1e-16                      # the a <= 1e-15 branch beside the others
+    targets = np.arange(0, n, max(1, n // 50))
+    plan = icm_amd.Plan(S, k, block_size=64)
+    say(f"leaf extract players per lane n={n} k={k}")
+    out = {}
+    for ppl in ("1", "2"):
+        monkeypatch.setenv("ICM_HIP_LEAF_PPL", ppl)
+        full = plan.run(p, Q=64)
+        plan.set_targets(targets)
+        sub = plan.run(p, Q=64)
+        plan.set_targets(None)
+        out[ppl] = (full, sub, icm_amd.node_inner(a, p))
+    plan.close()
+    for ppl in ("2",):
+        for i, what in enumerate(("full", "subset", "node")):
+            assert np.array_equal(out["1"][i], out[ppl][i]), \
+                f"{ppl} per lane, {what}: max |diff| {np.max(np.abs(out['1'][i] - out[ppl][i])):.3e}"
