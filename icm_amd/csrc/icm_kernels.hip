@@ -729,6 +729,257 @@ __device__ __forceinline__ void leaf_vjp_player(const double *P, const double *D
     }
 }
 
+// leaf_vjp_player for PPL players of one block at once, with the
+// differences dA[m] = A[m] - A[m+1] and dB[m] = Bv[m] - Bv[m+1] staged
+// beside A (the same values leaf_vjp_player forms per step).  Per player the
+// operations and their order are those of leaf_vjp_player; a branch not taken
+// by a player is computed for the others and discarded.
+template <int B, int PPL>
+__device__ __forceinline__ void leaf_vjp_n(const double *P, const double *D, const double *A, const double *dA,
+                                           const double *dB, const double *a, const double *c, int bsize, int pk,
+                                           double *psi, double *h) {
+    bool fw = false, bw = false, zr = false;
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        fw |= a[u] > 0.5;
+        bw |= (a[u] <= 0.5) && (a[u] > 1e-15);
+        zr |= a[u] <= 1e-15;
+    }
+    using Y = std::true_type;
+    using N = std::false_type;
+    double pf[PPL], hf[PPL], pb[PPL], hb[PPL], pz[PPL], hz[PPL];
+    if (fw) {
+        double ia[PPL], r[PPL], qv[PPL], wv[PPL];
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            const double bm = 1.0 - a[u];
+            ia[u] = 1.0 / a[u];
+            r[u] = -bm * ia[u];
+            qv[u] = 0.0;
+            wv[u] = 0.0;
+            pf[u] = 0.0;
+            hf[u] = 0.0;
+        }
+        auto step = [&](int m, auto psi_on, auto w_on) {
+            const double Pm = P[m], Am = A[m], dBm = dB[m];
+            double Dm = 0.0, dAm = 0.0;
+            if constexpr (decltype(w_on)::value) {
+                Dm = D[m];
+                dAm = dA[m];
+            }
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                qv[u] = Pm * ia[u] + r[u] * qv[u];
+                if constexpr (decltype(psi_on)::value) pf[u] += Am * qv[u];
+                hf[u] += qv[u] * dBm;
+                if constexpr (decltype(w_on)::value) {
+                    wv[u] = (Dm - c[u] * qv[u]) * ia[u] + r[u] * wv[u];
+                    hf[u] += wv[u] * dAm;
+                }
+            }
+        };
+        int m = 0;
+        for (; m < pk && m <= bsize - 2; ++m) step(m, Y{}, Y{});
+        for (; m < pk; ++m) step(m, Y{}, N{});
+        for (; m <= bsize - 2; ++m) step(m, N{}, Y{});
+        for (; m < bsize; ++m) step(m, N{}, N{});
+    }
+    if (bw) {
+        double ib[PPL], r[PPL], qv[PPL], wv[PPL];
+        int m = bsize - 1;
+        const double Ptop = P[bsize];
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            const double bm = 1.0 - a[u];
+            ib[u] = 1.0 / bm;
+            r[u] = -a[u] * ib[u];
+            qv[u] = 0.0;
+            wv[u] = 0.0;
+            pb[u] = 0.0;
+            hb[u] = 0.0;
+            qv[u] = Ptop * ib[u];
+            if (m < pk) pb[u] += A[m] * qv[u];
+            hb[u] += qv[u] * dB[m];
+            if (m >= 1) {
+                wv[u] = (D[m] - c[u] * qv[u]) * ib[u] + r[u] * wv[u];
+                hb[u] += wv[u] * dA[m - 1];
+            }
+        }
+        auto step = [&](int m, auto psi_on, auto w_on) {
+            const double Pm = P[m + 1], Am = A[m], dBm = dB[m];
+            double Dm = 0.0, dAm = 0.0;
+            if constexpr (decltype(w_on)::value) {
+                Dm = D[m];
+                dAm = dA[m - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                qv[u] = Pm * ib[u] + r[u] * qv[u];
+                if constexpr (decltype(psi_on)::value) pb[u] += Am * qv[u];
+                hb[u] += qv[u] * dBm;
+                if constexpr (decltype(w_on)::value) {
+                    wv[u] = (Dm - c[u] * qv[u]) * ib[u] + r[u] * wv[u];
+                    hb[u] += wv[u] * dAm;
+                }
+            }
+        };
+        --m;
+        for (; m >= pk && m >= 1; --m) step(m, N{}, Y{});
+        for (; m >= pk; --m) step(m, N{}, N{});
+        for (; m >= 1; --m) step(m, Y{}, Y{});
+        for (; m >= 0; --m) step(m, Y{}, N{});
+    }
+    if (zr) {
+#pragma unroll
+        for (int u = 0; u < PPL; ++u) {
+            pz[u] = 0.0;
+            hz[u] = 0.0;
+        }
+        for (int m = 0; m < bsize; ++m) {
+            const double qv = P[m + 1], dBm = dB[m];
+            const bool wo = m <= bsize - 2;
+            const double Dm1 = wo ? D[m + 1] : 0.0, Pm2 = (wo && m + 2 <= B) ? P[m + 2] : 0.0;
+            const double dAm = wo ? dA[m] : 0.0, Am = (m < pk) ? A[m] : 0.0;
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                if (m < pk) pz[u] += Am * qv;
+                hz[u] += qv * dBm;
+                if (wo) {
+                    const double wv = Dm1 - c[u] * (m + 2 <= B ? Pm2 : 0.0);
+                    hz[u] += wv * dAm;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        psi[u] = a[u] > 0.5 ? pf[u] : (a[u] > 1e-15 ? pb[u] : pz[u]);
+        h[u] = a[u] > 0.5 ? hf[u] : (a[u] > 1e-15 ? hb[u] : hz[u]);
+    }
+}
+
+// Adjoint leaves with PPL players of one block per lane and the next
+// q-plane's operands loaded while the current one is computed (as
+// k_leaf_extract_n); the same results, bit for bit, as k_leaf_vjp.
+template <int B, int PPL>
+__global__ __launch_bounds__(256) void k_leaf_vjp_n(VjpLeafJob V) {
+    const LeafJob &J = V.L;
+    constexpr int LPB = B / PPL;
+    constexpr int BPW = 256 / LPB;
+    constexpr int NE = BPW * (B + 1);            // staged elements per array and q-plane
+    constexpr int NT = (NE + 255) / 256;         // of them per thread
+    static_assert(LPB * PPL == B && BPW * LPB == 256, "lanes must tile the blocks");
+    __shared__ double shP[2][NE], shD[2][NE], shA[2][NE], shdA[2][NE], shdB[2][NE];
+    const int t = threadIdx.x;
+    const int lb = t / LPB, li = t % LPB;
+    const int bfirst = blockIdx.x * BPW;
+    const int b = bfirst + lb;
+    const int chunk = blockIdx.y;
+    const int q0 = chunk * J.q_per_chunk;
+    const int q1 = min(q0 + J.q_per_chunk, J.nq);
+    const int bsize = (b < J.nb) ? min(B, J.n - b * B) : 0;
+    const int pk = min(J.G0, bsize);
+    int jj[PPL];
+    bool valid[PPL];
+    double Sj[PPL], vj[PPL];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        jj[u] = b * B + li + u * LPB;
+        valid[u] = (b < J.nb) && (jj[u] < J.n);
+        Sj[u] = valid[u] ? J.S[jj[u]] : 0.0;
+        vj[u] = (valid[u] && V.v) ? V.v[jj[u]] : 0.0;
+        any |= valid[u];
+    }
+    const long long bplane = (long long)J.nb * (B + 1);
+    int eb[NT], em[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int e = t + i * 256;
+        eb[i] = (e < NE) ? bfirst + e / (B + 1) : J.nb;
+        em[i] = e % (B + 1);
+    }
+    double rP[NT], rD[NT], rA[NT], rdA[NT], rdB[NT];
+    auto fetch = [&](int q) {
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            const bool ok = eb[i] < J.nb;
+            const int m = em[i];
+            const long long o = q * bplane + (long long)eb[i] * (B + 1) + m;
+            rP[i] = ok ? J.blocks[o] : 0.0;
+            rD[i] = ok ? V.dblocks[o] : 0.0;
+            const double *ga = J.g + q * J.g_plane + (long long)eb[i] * J.g_stride;
+            const double a0 = (ok && m < J.G0) ? ga[m] : 0.0;
+            const double a1 = (ok && m + 1 < J.G0) ? ga[m + 1] : 0.0;
+            rA[i] = a0;
+            rdA[i] = a0 - a1;
+            const double *gb = V.bvec ? V.bvec + q * V.b_plane + (long long)eb[i] * V.b_stride : nullptr;
+            const double b0 = (ok && gb && m < V.c0) ? gb[m] : 0.0;
+            const double b1 = (ok && gb && m + 1 < V.c0) ? gb[m + 1] : 0.0;
+            rdB[i] = b0 - b1;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            const int e = t + i * 256;
+            if (e < NE) {
+                shP[buf][e] = rP[i];
+                shD[buf][e] = rD[i];
+                shA[buf][e] = rA[i];
+                shdA[buf][e] = rdA[i];
+                shdB[buf][e] = rdB[i];
+            }
+        }
+    };
+    if (q0 < q1) {
+        fetch(q0);
+        stash(0);
+    }
+    __syncthreads();
+    double accE[PPL], accD[PPL];
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) accE[u] = accD[u] = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        const int cur = (q - q0) & 1;
+        const bool more = q + 1 < q1;
+        if (more) fetch(q + 1);
+        if (any) {
+            const double lv = J.logv[q], wq = J.wq[q];
+            const double ivv = exp(-lv);
+            double a[PPL], c[PPL], pw[PPL], psi[PPL], h[PPL];
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                a[u] = !valid[u] ? 0.75 : (J.a_in ? J.a_in[(long long)q * J.n + jj[u]] : a_of(Sj[u], lv));
+                pw[u] = wq * Sj[u] * a[u] * ivv;
+                if (!isfinite(pw[u])) pw[u] = 0.0;
+                c[u] = V.c_in ? (valid[u] ? V.c_in[(long long)q * J.n + jj[u]] : 0.0) : vj[u] * pw[u];
+            }
+            const int o = lb * (B + 1);
+            leaf_vjp_n<B, PPL>(shP[cur] + o, shD[cur] + o, shA[cur] + o, shdA[cur] + o, shdB[cur] + o, a, c, bsize,
+                               pk, psi, h);
+#pragma unroll
+            for (int u = 0; u < PPL; ++u) {
+                if (!valid[u]) continue;
+                if (J.inner_out) J.inner_out[(long long)q * J.n + jj[u]] = psi[u];
+                if (V.h_out) V.h_out[(long long)q * J.n + jj[u]] = h[u];
+                accE[u] += pw[u] * psi[u];
+                double dterm = vj[u] * wq * ivv * a[u] * (1.0 + Sj[u] * lv) * psi[u];
+                if (!isfinite(dterm)) dterm = 0.0;
+                accD[u] += dterm + h[u] * a[u] * lv;
+            }
+        }
+        if (more) stash(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < PPL; ++u) {
+        if (!valid[u]) continue;
+        if (J.partial) J.partial[(long long)chunk * J.n + jj[u]] = accE[u];
+        if (V.partial_ds) V.partial_ds[(long long)chunk * J.n + jj[u]] = accD[u];
+    }
+}
+
 // Per player j of block b (A = forward g of the block, Bv = adjoint of P_b):
 //   Q = P_b / f_j,  W = (dP_b - c_j Q) / f_j          (same division branch)
 //   Psi = sum_{m<pk} A[m] Q[m]
@@ -2959,7 +3210,10 @@ hipError_t launch_leaf_vjp(const VjpLeafJob &V, int nchunk, hipStream_t s, Prof 
     switch (J.B) {
     case 16: k_leaf_vjp<16><<<grid, 256, 0, s>>>(V); break;
     case 32: k_leaf_vjp<32><<<grid, 256, 0, s>>>(V); break;
-    case 64: k_leaf_vjp<64><<<grid, 256, 0, s>>>(V); break;
+    case 64:
+        if (leaf_ppl() == 2) k_leaf_vjp_n<64, 2><<<dim3((J.nb + 7) / 8, nchunk), 256, 0, s>>>(V);
+        else k_leaf_vjp<64><<<grid, 256, 0, s>>>(V);
+        break;
     default: return hipErrorInvalidValue;
     }
     const double nq = J.nq, B = J.B;

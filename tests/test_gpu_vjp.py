@@ -277,3 +277,30 @@ def test_vjp_checkpointing_chosen_under_a_memory_budget():
         plan.close()
     assert np.max(np.abs(d - d_ref)) <= 1e-11 * np.max(np.abs(d_ref))
     assert np.max(np.abs(e - e_ref)) <= 1e-12 * np.max(np.abs(e_ref))
+
+
+@pytest.mark.parametrize("n,k", [(20000, 20000), (65536, 1000), (5000, 37), (4133, 64)])
+def test_adjoint_leaves_players_per_lane_bit_identical(n, k, monkeypatch):
+    """The B = 64 adjoint leaves with 2 players of a block per lane (staged
+    differences of A and of B's adjoint, double-buffered planes) give the same
+    bits as one player per lane: equities and dS, and per node Psi and H
+    with a covering all three division branches."""
+    rng = np.random.default_rng(n + 3 * k)
+    S = np.exp(rng.uniform(0.0, math.log(1e6), n))
+    p = np.arange(k, 0, -1, dtype=float) ** 2
+    p /= p.sum()
+    v = rng.normal(size=n)
+    a = rng.uniform(0.0, 1.0, n)
+    a[::5] = 1e-17
+    c = rng.normal(size=n)
+    out = {}
+    for ppl in ("1", "2"):
+        monkeypatch.setenv("ICM_HIP_LEAF_PPL", ppl)
+        plan = icm_amd.Plan(S, k, block_size=64)
+        e, d = plan.vjp(p, v, Q=32)
+        plan.close()
+        out[ppl] = (e, d) + tuple(icm_amd.node_vjp(a, c, p))
+    say(f"adjoint leaves per-lane players n={n} k={k}")
+    for i, what in enumerate(("E", "dS", "Psi", "H")):
+        assert np.array_equal(out["1"][i], out["2"][i]), \
+            f"{what}: max |diff| {np.max(np.abs(out['1'][i] - out['2'][i])):.3e}"
