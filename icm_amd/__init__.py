@@ -154,6 +154,9 @@ _SIGS = [
      [ctypes.c_void_p, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_size_t)]),
     ("icm_hip_batch_set_profiling", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_batch_set_schoolbook_max", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_batch_set_persistent", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("icm_hip_batch_is_persistent", ctypes.c_int, [ctypes.c_void_p]),
+    ("icm_hip_batch_set_spin_limit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("icm_hip_batch_kernel_count", ctypes.c_int, [ctypes.c_void_p]),
     ("icm_hip_batch_kernel_stat", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I, _D, _D, _D]),
@@ -648,6 +651,21 @@ class BatchPlan:
         """Schoolbook / transform crossover of the forward levels."""
         if self._lib.icm_hip_batch_set_schoolbook_max(self._h, int(cmax)) != 0:
             raise _err(self._lib, "icm_hip_batch_set_schoolbook_max")
+
+    def set_persistent(self, on=True):
+        """Run the forward batch as one persistent launch per q-pass (block
+        size 64) instead of one launch per phase and tree level."""
+        if self._lib.icm_hip_batch_set_persistent(self._h, int(bool(on))) != 0:
+            raise _err(self._lib, "icm_hip_batch_set_persistent")
+
+    @property
+    def persistent(self):
+        return bool(self._lib.icm_hip_batch_is_persistent(self._h))
+
+    def set_spin_limit(self, spins):
+        """Bound (polls) of a persistent work item's wait for its inputs."""
+        if self._lib.icm_hip_batch_set_spin_limit(self._h, int(spins)) != 0:
+            raise _err(self._lib, "icm_hip_batch_set_spin_limit")
 
     def set_profiling(self, on=True):
         if self._lib.icm_hip_batch_set_profiling(self._h, int(bool(on))) != 0:

@@ -12,6 +12,14 @@
 // plane.  The first run with a given Q records the launches as one HIP graph;
 // later runs replay it (one hipGraphLaunch per batch).
 //
+// With block size 64 (the tuned default) a q-pass can instead run as ONE
+// persistent launch (k_batch_persist): workgroups take the same work items,
+// plus leaf, extract and reduction items, by ticket, and each instance moves
+// to its next level as soon as its own previous level is done (per-instance
+// completion counters), with no barrier between levels.  The g arrays then
+// give every instance its own region per level parity, so instances at
+// different levels never share memory.
+//
 // Instances with a level above one LDS transform (F > 4096: k > 2048 and
 // n > 4096) cannot share those launches; a batch that holds some becomes a
 // composite: the other instances run as an inner batch, and each large one
@@ -51,6 +59,12 @@ struct IcmHipBatchPlan {
     int sb_max = 0;                      // schoolbook levels: c_{l-1} <= sb_max
     bool use_graph = true;
     bool verbose = false;
+    bool persistent = false;             // one persistent launch per q-pass (needs persist_ok)
+    bool persist_ok = false;             // block size the persistent kernel is built for
+    int persist_grid = 0;                // resident workgroups of the persistent kernel
+    int spin_limit = 1 << 20;            // bound of a persistent item's dependency wait (polls, < 1 s)
+    int *d_ctr = nullptr;                // [ninst] finished persistent items per instance
+    int *d_sched = nullptr;              // [kPSchedSize] queue tickets, workgroups done, error flag
     // arena offsets (doubles) per level and instance, sized for qb planes
     std::vector<std::vector<long long>> coff, goff;
     std::vector<long long> coef_elems;   // per level
@@ -83,6 +97,13 @@ struct IcmHipBatchPlan {
         std::vector<double> b_bytes, b_flops, p_bytes, p_flops;
         std::vector<int> bs_off, bs_cnt, ps_off, ps_cnt;
         std::vector<double> bs_bytes, bs_flops, ps_bytes, ps_flops;
+        // persistent launch: ordered items and per-instance need counts
+        icm::PItem *d_pitems = nullptr;
+        int n_pitems = 0;
+        int *d_need = nullptr;           // [2][ninst]: before extract, before finalize
+        int *d_qtab = nullptr;           // queue item offsets [kPMaxQueues + 1], block / player ends [kPMaxQueues][2]
+        std::vector<int> qcut;           // first instance of each queue (+ ninst)
+        int nch = 1, qpc = 1;            // q-chunks of the extract and q-planes per chunk
     };
     std::vector<Pass> passes;            // index 0: nq = qb, 1: the remainder (if any)
     // adjoint pass (allocated on the first VJP call)
@@ -140,6 +161,9 @@ void free_passes(IcmHipBatchPlan *p, std::vector<IcmHipBatchPlan::Pass> &v) {
     for (auto &ps : v) {
         if (ps.d_jobs) (void)hipFree(ps.d_jobs);
         if (ps.d_items) (void)hipFree(ps.d_items);
+        if (ps.d_pitems) (void)hipFree(ps.d_pitems);
+        if (ps.d_need) (void)hipFree(ps.d_need);
+        if (ps.d_qtab) (void)hipFree(ps.d_qtab);
         p->mem.bytes -= std::min(p->mem.bytes, ps.dev_bytes);
     }
     v.clear();
@@ -157,6 +181,7 @@ void free_buffers(IcmHipBatchPlan *p) {
     f(p->d_g[0]); f(p->d_g[1]); f(p->d_partial); f(p->d_acc); f(p->d_equity);
     f(p->d_dblocks); for (double *&c : p->d_dcoef) f(c);
     f(p->d_bv[0]); f(p->d_bv[1]); f(p->d_vin); f(p->d_v); f(p->d_partial_ds); f(p->d_acc_ds); f(p->d_ds);
+    f(p->d_ctr); f(p->d_sched);
     p->nodes_Q = 0;
     p->vjp_ready = false;
     p->mem.bytes = 0;
@@ -293,11 +318,130 @@ bool make_vjp_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
                                               hipMemcpyHostToDevice)));
 }
 
+// q-chunks of a pass's leaf extract: (chunks, q-planes per chunk).
+void extract_chunks(const IcmHipBatchPlan *p, int nq, int &nch, int &qpc) {
+    const int nchunk = std::min(p->nchunk, nq);
+    qpc = (nq + nchunk - 1) / nchunk;
+    nch = (nq + qpc - 1) / qpc;
+}
+
+// The persistent launch's item lists for one pass, one per XCD queue.  The
+// instances are cut into kPMaxQueues contiguous ranges of about equal work
+// (level items, schoolbook ones at half weight, and blocks); queue x holds,
+// for its instances only: leaf items over its blocks, every level's work
+// items (in launch order: build levels up, propagate levels down), extract
+// items over its blocks and finalize items over its players.  An instance's
+// items of one phase all need its finished items of the earlier phases;
+// those come first in its queue, so every wait is on smaller tickets of the
+// same queue, and no item reads what another XCD writes.
+bool make_persist_items(IcmHipBatchPlan *p, IcmHipBatchPlan::Pass &ps, const std::vector<int2> &items,
+                        const std::vector<int> &job_inst) {
+    const int ni = p->ninst, NB = p->NB, Lm = p->Lmax, NQ = icm::kPMaxQueues;
+    // level steps in launch order: (offset, count, kind)
+    struct Rng { int off, cnt, kind; };
+    std::vector<std::vector<Rng>> steps;
+    for (int l = 1; l <= Lm - 2; ++l)
+        steps.push_back({{ps.b_off[l], ps.b_cnt[l], icm::kPBuildLds}, {ps.bs_off[l], ps.bs_cnt[l], icm::kPBuildSb}});
+    for (int l = Lm - 1; l >= 1; --l)
+        steps.push_back({{ps.p_off[l], ps.p_cnt[l], icm::kPPropLds}, {ps.ps_off[l], ps.ps_cnt[l], icm::kPPropSb}});
+    // work per instance and the queue cuts
+    std::vector<double> wi(ni, 0.0);
+    double witems = 0.0;
+    for (const auto &st : steps)
+        for (const Rng &r : st)
+            for (int x = r.off; x < r.off + r.cnt; ++x) {
+                const double w = (r.kind == icm::kPBuildSb || r.kind == icm::kPPropSb) ? 0.5 : 1.0;
+                wi[job_inst[items[x].x]] += w;
+                witems += w;
+            }
+    std::vector<double> cost(ni);
+    double total = 0.0;
+    for (int i = 0; i < ni; ++i) {
+        cost[i] = (witems > 0 ? 0.7 * wi[i] / witems : 0.0) + (witems > 0 ? 0.3 : 1.0) * p->geo[i].nb / NB;
+        total += cost[i];
+    }
+    std::vector<int> cut(NQ + 1, ni);
+    cut[0] = 0;
+    {
+        double acc = 0.0;
+        int x = 1;
+        for (int i = 0; i < ni && x < NQ; ++i) {
+            acc += cost[i];
+            while (x < NQ && acc >= total * x / NQ) cut[x++] = i + 1;
+        }
+        for (int q = 1; q <= NQ; ++q) cut[q] = std::max(cut[q], cut[q - 1]);
+    }
+    std::vector<icm::PItem> pit;
+    std::vector<int> base(ni, 0), step(ni, 0), touched, need(2 * (size_t)ni, 0), qtab(3 * NQ + 1, 0);
+    extract_chunks(p, ps.nq, ps.nch, ps.qpc);
+    const int exb = icm::persist_extract_blocks(p->B);
+    auto inst_of_player = [&](int j) {
+        return (int)(std::upper_bound(p->n_off.begin(), p->n_off.end(), j) - p->n_off.begin()) - 1;
+    };
+    for (int q = 0; q < NQ; ++q) {
+        qtab[q] = (int)pit.size();
+        const int ilo = cut[q], ihi = cut[q + 1];
+        const int blo = ilo < ni ? p->boff[ilo] : NB, bhi = ihi < ni ? p->boff[ihi] : NB;
+        const int plo = p->n_off[ilo], phi = p->n_off[ihi];
+        qtab[NQ + 1 + 2 * q] = bhi;
+        qtab[NQ + 1 + 2 * q + 1] = phi;
+        if (ilo >= ihi) continue;
+        for (int gb0 = blo; gb0 < bhi; gb0 += icm::kPLeafBlocks) {
+            const int i0 = p->blk_inst[gb0], i1 = p->blk_inst[std::min(gb0 + icm::kPLeafBlocks, bhi) - 1];
+            pit.push_back(icm::PItem{icm::kPLeaf | (i0 << 3), gb0, 0, i1});
+            for (int i = i0; i <= i1; ++i) ++base[i];
+        }
+        for (const auto &st : steps) {
+            for (const Rng &r : st)
+                for (int x = r.off; x < r.off + r.cnt; ++x) {
+                    const int i = job_inst[items[x].x];
+                    if (i < ilo || i >= ihi) continue;
+                    pit.push_back(icm::PItem{r.kind | (i << 3), items[x].x, items[x].y, base[i]});
+                    if (step[i]++ == 0) touched.push_back(i);
+                }
+            for (int i : touched) { base[i] += step[i]; step[i] = 0; }
+            touched.clear();
+        }
+        for (int i = ilo; i < ihi; ++i) need[i] = base[i];
+        for (int c = 0; c < ps.nch; ++c)
+            for (int gb0 = blo; gb0 < bhi; gb0 += exb) {
+                const int i0 = p->blk_inst[gb0], i1 = p->blk_inst[std::min(gb0 + exb, bhi) - 1];
+                pit.push_back(icm::PItem{icm::kPExtract | (i0 << 3), gb0, c, i1});
+                for (int i = i0; i <= i1; ++i) ++base[i];
+            }
+        for (int i = ilo; i < ihi; ++i) need[ni + i] = base[i];
+        for (int j0 = plo; j0 < phi; j0 += icm::kPFinalPlayers) {
+            const int i0 = inst_of_player(j0), i1 = inst_of_player(std::min(j0 + icm::kPFinalPlayers, phi) - 1);
+            pit.push_back(icm::PItem{icm::kPFinal | (i0 << 3), j0, 0, i1});
+        }
+    }
+    qtab[NQ] = (int)pit.size();
+    for (const icm::PItem &x : pit) {   // a multi-instance item's waits and counts are one per thread
+        const int k = x.kind_inst & 7;
+        if ((k == icm::kPLeaf || k == icm::kPExtract || k == icm::kPFinal) && x.need - (x.kind_inst >> 3) >= 512) {
+            set_err("persistent batch: an item spans more than 512 instances");
+            return false;
+        }
+    }
+    ps.n_pitems = (int)pit.size();
+    ps.qcut = cut;
+    const size_t before = p->mem.bytes;
+    if (!balloc(p, &ps.d_pitems, std::max<size_t>(1, pit.size())) || !balloc(p, &ps.d_need, need.size()) ||
+        !balloc(p, &ps.d_qtab, qtab.size()))
+        return false;
+    ps.dev_bytes += p->mem.bytes - before;
+    return (pit.empty() ||
+            HIP_OK(hipMemcpy(ps.d_pitems, pit.data(), sizeof(icm::PItem) * pit.size(), hipMemcpyHostToDevice))) &&
+           HIP_OK(hipMemcpy(ps.d_need, need.data(), sizeof(int) * need.size(), hipMemcpyHostToDevice)) &&
+           HIP_OK(hipMemcpy(ps.d_qtab, qtab.data(), sizeof(int) * qtab.size(), hipMemcpyHostToDevice));
+}
+
 // Job tables and work items for passes of nq q-planes.
 bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
     ps.nq = nq;
     const int Lm = p->Lmax;
     std::vector<icm::LevelJob> jobs;
+    std::vector<int> job_inst;
     std::vector<int2> items;
     for (auto *v : {&ps.b_off, &ps.b_cnt, &ps.p_off, &ps.p_cnt, &ps.bs_off, &ps.bs_cnt, &ps.ps_off, &ps.ps_cnt})
         v->assign(Lm, 0);
@@ -339,6 +483,7 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
                 ps.b_flops[l] += (double)nq * npair * 3.0 * fft5(J.F);
             }
             jobs.push_back(J);
+            job_inst.push_back(i);
         }
         ps.b_off[l] = (int)items.size();
         ps.b_cnt[l] = (int)fft.size();
@@ -377,6 +522,7 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
                 ps.p_flops[l] += (double)nq * npair * 5.0 * fft5(J.F);
             }
             jobs.push_back(J);
+            job_inst.push_back(i);
         }
         ps.p_off[l] = (int)items.size();
         ps.p_cnt[l] = (int)fft.size();
@@ -392,7 +538,8 @@ bool make_pass(IcmHipBatchPlan *p, int nq, IcmHipBatchPlan::Pass &ps) {
     return (jobs.empty() || HIP_OK(hipMemcpy(ps.d_jobs, jobs.data(), sizeof(icm::LevelJob) * jobs.size(),
                                              hipMemcpyHostToDevice))) &&
            (items.empty() || HIP_OK(hipMemcpy(ps.d_items, items.data(), sizeof(int2) * items.size(),
-                                              hipMemcpyHostToDevice)));
+                                              hipMemcpyHostToDevice))) &&
+           (!p->persist_ok || make_persist_items(p, ps, items, job_inst));
 }
 
 IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, const int *k_off,
@@ -519,6 +666,9 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     p->use_graph = cfg.graphs;
     p->B = cfg.B ? cfg.B : kBatchTune.B;
     const int B = p->B;
+    p->persist_grid = icm::batch_persist_grid(B);
+    p->persist_ok = icm::persist_extract_blocks(B) > 0 && p->persist_grid > 0;
+    p->persistent = p->persist_ok && cfg.batch_persistent;
     p->geo.resize(n_inst);
     p->boff.assign(n_inst, 0);
     p->Smax.assign(n_inst, 0.0);
@@ -560,8 +710,16 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
     }
     long long per_plane = (long long)NB * (B + 1);
     for (int l = 1; l < Lm; ++l) per_plane += cplane[l];
+    // g: per parity, each instance's largest level of that parity (one
+    // region per instance, so instances at different levels never overlap)
     long long gmax[2] = {0, 0};
-    for (int l = 0; l < Lm; ++l) gmax[l & 1] = std::max(gmax[l & 1], gplane[l]);
+    for (int i = 0; i < n_inst; ++i) {
+        const Geometry &g = p->geo[i];
+        long long m[2] = {0, 0};
+        for (int l = 0; l <= g.L - 2; ++l) m[l & 1] = std::max(m[l & 1], (long long)g.n_real[l] * g.G[l]);
+        gmax[0] += m[0];
+        gmax[1] += m[1];
+    }
     per_plane += gmax[0] + gmax[1];
     size_t freeb = 0, totb = 0;
     if (!HIP_OK(hipMemGetInfo(&freeb, &totb))) { delete p; return nullptr; }
@@ -580,15 +738,27 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
         p->coef_elems.assign(Lm, 0);
         p->g_elems[0] = p->g_elems[1] = 0;
         for (int l = 0; l < Lm; ++l) {
-            long long co = 0, go = 0;
+            long long co = 0;
             for (int i = 0; i < n_inst; ++i) {
                 const Geometry &g = p->geo[i];
                 if (l >= 1 && l <= g.L - 2) { p->coff[l][i] = co; co += (long long)q * g.n_real[l] * g.c[l]; }
-                if (l <= g.L - 2) { p->goff[l][i] = go; go += (long long)q * g.n_real[l] * g.G[l]; }
             }
             p->coef_elems[l] = co;
-            p->g_elems[l & 1] = std::max(p->g_elems[l & 1], go);
         }
+        // g levels of one parity share the instance's region of that parity
+        long long go[2] = {0, 0};
+        for (int i = 0; i < n_inst; ++i) {
+            const Geometry &g = p->geo[i];
+            long long m[2] = {0, 0};
+            for (int l = 0; l <= g.L - 2; ++l) {
+                p->goff[l][i] = go[l & 1];
+                m[l & 1] = std::max(m[l & 1], (long long)q * g.n_real[l] * g.G[l]);
+            }
+            go[0] += m[0];
+            go[1] += m[1];
+        }
+        p->g_elems[0] = go[0];
+        p->g_elems[1] = go[1];
         const int bpw = 256 / B;
         const int wg_blocks = (NB + bpw - 1) / bpw;
         p->nchunk = std::min(std::max(1, 4096 / std::max(1, wg_blocks)), q);
@@ -602,6 +772,10 @@ IcmHipBatchPlan *batch_create(int n_inst, const int *n_off, const double *S, con
                   balloc(p, &p->d_equity, Ntot);
         p->d_coef.assign(Lm, nullptr);
         for (int l = 1; ok && l <= Lm - 2; ++l) ok = balloc(p, &p->d_coef[l], p->coef_elems[l]);
+        if (ok && p->persist_ok)
+            ok = balloc(p, &p->d_ctr, n_inst) && balloc(p, &p->d_sched, icm::kPSchedSize) &&
+                 HIP_OK(hipMemset(p->d_ctr, 0, sizeof(int) * n_inst)) &&
+                 HIP_OK(hipMemset(p->d_sched, 0, sizeof(int) * icm::kPSchedSize));
         return ok;
     };
     if (!(HIP_OK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) && HIP_OK(hipEventCreate(&p->ev0)) &&
@@ -740,9 +914,57 @@ bool ensure_vjp(IcmHipBatchPlan *p, int Q) {
     return true;
 }
 
+// The batch as one persistent launch per q-pass (the last pass scatters).
+bool enqueue_persistent(IcmHipBatchPlan *p, int Q) {
+    hipStream_t s = p->stream;
+    for (int q0 = 0; q0 < Q; q0 += p->qb) {
+        const int nq = std::min(p->qb, Q - q0);
+        const IcmHipBatchPlan::Pass &ps = (nq == p->qb) ? p->passes[0] : p->passes.back();
+        icm::PersistArgs A{};
+        A.items = ps.d_pitems;
+        A.nitems = ps.n_pitems;
+        A.qoff = ps.d_qtab;
+        A.qend = ps.d_qtab + icm::kPMaxQueues + 1;
+        A.nqueues = icm::kPMaxQueues;
+        A.jobs = ps.d_jobs;
+        A.inst = p->d_inst;
+        A.blk_inst = p->d_blk_inst;
+        A.S = p->d_S;
+        A.blocks = p->d_blocks;
+        A.NB = p->NB;
+        A.nq = nq;
+        A.q0 = q0;
+        A.ninst = p->ninst;
+        A.tw = p->d_tw;
+        A.twShift = p->twShift;
+        A.nchunk = ps.nch;
+        A.q_per_chunk = ps.qpc;
+        A.partial = p->d_partial;
+        A.Ntot = p->Ntot;
+        A.acc = p->d_acc;
+        A.accumulate = q0 > 0 ? 1 : 0;
+        A.perm = p->d_perm;
+        A.equity = (q0 + nq >= Q) ? p->d_equity : nullptr;
+        A.need_ex = ps.d_need;
+        A.need_fin = ps.d_need + p->ninst;
+        A.ctr = p->d_ctr;
+        A.sched = p->d_sched;
+        A.spin_limit = p->spin_limit;
+        double bytes = 0.0, flops = 0.0;
+        for (int l = 0; l < p->Lmax; ++l)
+            bytes += ps.b_bytes[l] + ps.p_bytes[l] + ps.bs_bytes[l] + ps.ps_bytes[l],
+            flops += ps.b_flops[l] + ps.p_flops[l] + ps.bs_flops[l] + ps.ps_flops[l];
+        const int grid = std::min(p->persist_grid, std::max(1, ps.n_pitems));
+        if (!hip_ok(icm::launch_batch_persist(p->B, A, grid, s, &p->prof, bytes, flops), "persistent batch"))
+            return false;
+    }
+    return true;
+}
+
 // Enqueues the whole batch for Q nodes on the plan's stream (payout already
 // in d_payout); leaves equities in d_equity.
 bool enqueue(IcmHipBatchPlan *p, int Q) {
+    if (p->persistent) return enqueue_persistent(p, Q);
     hipStream_t s = p->stream;
     const int B = p->B, NB = p->NB;
     for (int q0 = 0; q0 < Q; q0 += p->qb) {
@@ -769,9 +991,8 @@ bool enqueue(IcmHipBatchPlan *p, int Q) {
                                                    &p->prof, ps.ps_bytes[l], ps.ps_flops[l]),
                         "batch schoolbook propagate level"))
                 return false;
-        const int nchunk = std::min(p->nchunk, nq);
-        const int qpc = (nq + nchunk - 1) / nchunk;
-        const int nch = (nq + qpc - 1) / qpc;
+        int nch = 1, qpc = 1;
+        extract_chunks(p, nq, nch, qpc);
         double g0 = 0.0;
         for (int i = 0; i < p->ninst; ++i) g0 += (double)p->geo[i].nb * p->geo[i].G[0];
         const double le_bytes = 8.0 * ((double)nq * (NB * (B + 1) + g0) + (double)p->Ntot * (1 + nch));
@@ -864,6 +1085,23 @@ int batch_run(IcmHipBatchPlan *p, int Q, IcmHipRunStats *stats, int mode = 0) {
     }
     if (!HIP_OK(hipEventRecord(p->ev1, s))) return -1;
     if (!HIP_OK(hipStreamSynchronize(s))) return -1;
+    if (mode == 0 && p->persistent) {   // error flag of the persistent launch
+        int flag = 0;
+        if (!HIP_OK(hipMemcpy(&flag, p->d_sched + icm::kPSchedErr, sizeof(int), hipMemcpyDeviceToHost))) return -1;
+        if (flag) {
+            const int zero = 0;
+            (void)hipMemcpy(p->d_sched + icm::kPSchedErr, &zero, sizeof(int), hipMemcpyHostToDevice);
+            if (flag == 2) {   // an XCD queue no workgroup served: run this plan with per-level launches
+                if (p->verbose) fprintf(stderr, "[icm_hip batch] persistent launch left a queue unserved; "
+                                                "switching to per-level launches\n");
+                p->persistent = false;
+                drop_graph(p, 0);
+                return batch_run(p, Q, stats, mode);
+            }
+            set_err("persistent batch: a work item's dependency wait passed its bound (results discarded)");
+            return -1;
+        }
+    }
     if (p->prof.enabled) p->prof.collect();
     if (stats) {
         float ms = 0.f;
@@ -1025,6 +1263,29 @@ int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax) {
     free_passes(plan, plan->passes);
     free_passes(plan, plan->vpasses);
     plan->nodes_Q = 0;
+    return 0;
+}
+
+int icm_hip_batch_set_persistent(IcmHipBatchPlan *plan, int on) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (plan->composite) return plan->inner ? icm_hip_batch_set_persistent(plan->inner, on) : 0;
+    if (on && !plan->persist_ok) { set_err("the persistent batch kernel is built for block size 64 only"); return -1; }
+    if (plan->persistent != (on != 0)) drop_graph(plan, 0);
+    plan->persistent = on != 0;
+    return 0;
+}
+
+int icm_hip_batch_is_persistent(const IcmHipBatchPlan *plan) {
+    if (plan && plan->composite) return plan->inner ? icm_hip_batch_is_persistent(plan->inner) : 0;
+    return plan && plan->persistent ? 1 : 0;
+}
+
+int icm_hip_batch_set_spin_limit(IcmHipBatchPlan *plan, int spins) {
+    if (!plan) { set_err("null plan"); return -1; }
+    if (plan->composite) return plan->inner ? icm_hip_batch_set_spin_limit(plan->inner, spins) : 0;
+    if (spins < 1) { set_err("spin limit must be positive"); return -1; }
+    plan->spin_limit = spins;
+    drop_graph(plan, 0);   // the limit is a launch argument of the captured graph
     return 0;
 }
 

@@ -209,6 +209,7 @@ struct PlanCfg {
     bool verbose = false;
     bool graphs = true;      // ragged batches: record / replay a HIP graph
     int ckpt = -1;           // adjoint checkpoint interval (-1 auto, 0 keep every level, 2 .. 64)
+    bool batch_persistent = false;  // ragged batches: one persistent launch per q-pass
 };
 
 inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, PlanCfg &c) {
@@ -260,6 +261,10 @@ inline bool plan_config(const IcmHipOptions *o, const IcmHipPlanParams *pp, Plan
     if (c.cap == 0) {
         const char *e = getenv("ICM_HIP_MEM_BUDGET");
         if (e && e[0]) c.cap = (size_t)strtoull(e, nullptr, 10);
+    }
+    {
+        const char *e = getenv("ICM_HIP_BATCH_PERSISTENT");
+        if (e && e[0]) c.batch_persistent = atoi(e) != 0;
     }
     {
         const char *e = getenv("ICM_HIP_ADJOINT_CHECKPOINT");

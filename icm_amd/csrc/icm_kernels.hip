@@ -414,28 +414,36 @@ __global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__rest
 
 // Ragged-batch leaf extract with PPL players of one block per lane (as
 // k_leaf_extract_n); the same results, bit for bit, as k_leaf_extract_batch.
-template <int B, int PPL>
-__global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *__restrict__ inst,
-                                                              const int *__restrict__ blk_inst,
-                                                              const double *__restrict__ S,
-                                                              const double *__restrict__ blocks, int NB, int nq,
-                                                              int q0, int q_per_chunk, double *partial, int Ntot) {
-    constexpr int LPB = B / PPL;
-    constexpr int BPW = 256 / LPB;
-    constexpr int NP = BPW * (B + 1);
-    constexpr int NPT = (NP + 255) / 256;
-    constexpr int NG = BPW * B;
+template <int B, int PPL> struct LeafExN {
+    static constexpr int LPB = B / PPL;          // lanes per block
+    static constexpr int BPW = 256 / LPB;        // blocks per 256 threads
+    static constexpr int NP = BPW * (B + 1);     // block-product doubles per q-plane
+    static constexpr int NG = BPW * B;           // g doubles per q-plane
     static_assert(LPB * PPL == B && BPW * LPB == 256, "lanes must tile the blocks");
-    __shared__ double shP[2][NP];
-    __shared__ double shG[2][NG];
-    const int t = threadIdx.x;
+};
+
+// Body of the batch extract over blocks gfirst .. gfirst + BPW - 1 (those
+// below gend; NB is the block count of the arrays) and q-chunk `chunk`, run
+// by 256 threads (t = 0 .. 255) with LDS shP[2][NP], shG[2][NG].
+// Its barrier count depends on the chunk only, so two halves of a 512-thread
+// workgroup may run it on neighbouring block groups of one chunk.
+template <int B, int PPL>
+__device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int chunk,
+                                                          const BatchInst *__restrict__ inst,
+                                                          const int *__restrict__ blk_inst,
+                                                          const double *__restrict__ S,
+                                                          const double *__restrict__ blocks, int NB, int gend,
+                                                          int nq, int q0, int q_per_chunk, double *partial, int Ntot,
+                                                          double (*shP)[LeafExN<B, PPL>::NP],
+                                                          double (*shG)[LeafExN<B, PPL>::NG]) {
+    constexpr int LPB = LeafExN<B, PPL>::LPB;
+    constexpr int NP = LeafExN<B, PPL>::NP;
+    constexpr int NPT = (NP + 255) / 256;
     const int lb = t / LPB, li = t % LPB;
-    const int gfirst = blockIdx.x * BPW;
     const int gb = gfirst + lb;
-    const int chunk = blockIdx.y;
     const int qa = chunk * q_per_chunk;
     const int qe = min(qa + q_per_chunk, nq);
-    const BatchInst *I = (gb < NB) ? &inst[blk_inst[gb]] : nullptr;
+    const BatchInst *I = (gb < gend) ? &inst[blk_inst[gb]] : nullptr;
     const int b = I ? gb - I->boff : 0;
     const int bsize = I ? min(B, I->n - b * B) : 0;
     const int pk = I ? min(I->G0, bsize) : 0;
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *_
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int e = t + i * 256;
-        gP[i] = (e < NP) ? gfirst + e / (B + 1) : NB;
+        gP[i] = (e < NP && gfirst + e / (B + 1) < gend) ? gfirst + e / (B + 1) : NB;
         mP[i] = e % (B + 1);
     }
     const double *gbase[PPL];
@@ -466,7 +474,7 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *_
         const int gG = gfirst + e / B, mG = e % B;
         gbase[i] = nullptr;
         gplane[i] = 0;
-        if (gG < NB) {
+        if (gG < gend) {
             const BatchInst &I2 = inst[blk_inst[gG]];
             if (mG < I2.G0) {
                 gbase[i] = I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG;
@@ -521,6 +529,19 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *_
 #pragma unroll
     for (int u = 0; u < PPL; ++u)
         if (valid[u]) partial[(long long)chunk * Ntot + I->poff + jj[u]] = acc[u];
+}
+
+template <int B, int PPL>
+__global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *__restrict__ inst,
+                                                              const int *__restrict__ blk_inst,
+                                                              const double *__restrict__ S,
+                                                              const double *__restrict__ blocks, int NB, int nq,
+                                                              int q0, int q_per_chunk, double *partial, int Ntot) {
+    using X = LeafExN<B, PPL>;
+    __shared__ double shP[2][X::NP];
+    __shared__ double shG[2][X::NG];
+    leaf_extract_batch_n_body<B, PPL>(threadIdx.x, blockIdx.x * X::BPW, blockIdx.y, inst, blk_inst, S, blocks, NB, NB,
+                                      nq, q0, q_per_chunk, partial, Ntot, shP, shG);
 }
 
 template <int B>
@@ -1509,14 +1530,27 @@ struct FlatMap {    // v = q * npair + pair over (q-plane, pair), from v0
     }
 };
 
-template <int F, class Map, int LT = kLT>
+// The thread index for a body run inside the persistent batch kernel's item
+// loop (OPQ = true): opaque to the compiler at each call, so nothing derived
+// from it is hoisted out of the loop and kept live across every kind of item
+// (which pushed that kernel past 128 VGPRs).  No instruction is emitted.
+template <bool OPQ> __device__ __forceinline__ int body_tid() {
+    int t = threadIdx.x;
+    if constexpr (OPQ) {
+        asm volatile("" : "+v"(t));
+        __builtin_assume(t >= 0 && t < 1024);
+    }
+    return t;
+}
+
+template <int F, class Map, int LT = kLT, bool OPQ = false>
 __device__ __forceinline__ void build_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                int twShift) {
     using TL = typename LdsTL<F, LT>::type;
     constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
-    const int t = threadIdx.x;
+    const int t = body_tid<OPQ>();
     (void)twShift;
 
     double2 P1[LP / 2];
@@ -1596,13 +1630,13 @@ __global__ __launch_bounds__(lds_build_threads<F>(), lds_build_threads<F>() == 2
 // memory and the inverses store straight to the children's g (two LDS round
 // trips fewer per transform); with per-pass LDS addresses this fits 128
 // VGPRs at 512 threads (two workgroups per CU) without spills.
-template <int F, class Map>
+template <int F, class Map, bool OPQ = false>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift) {
     using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
-    const int t = threadIdx.x;
+    const int t = body_tid<OPQ>();
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
 
     auto ldg = [&](int item, int m) {
@@ -1694,16 +1728,18 @@ __global__ __launch_bounds__(kLT, 4) void k_prop_lds(LevelJob J, const double2 *
 // register window over R (build) or g (propagate), so each LDS load feeds
 // 4 (build) or 8 (propagate) FMAs and the broadcast coefficient is shared by
 // the lanes of a parent.  Zero padding in LDS replaces bounds tests in the
-// inner loop.  A missing right child is the identity polynomial.
+// inner loop.  A missing right child is the identity polynomial.  NT threads
+// share a workgroup's parents (256 in the schoolbook kernels, 512 in the
+// persistent batch kernel); each output's sum is the same either way.
 // ---------------------------------------------------------------------------
 constexpr int kSbT = 256;
 
-template <class Map>
+template <class Map, int NT = kSbT, bool OPQ = false>
 __device__ __forceinline__ void build_sb_body(const LevelJob &J, double *sh, const Map &M, int ppw) {
-    const int t = threadIdx.x;
+    const int t = body_tid<OPQ>();
     const int cc = J.child.count, pc = J.parent.count;
     const int stride = sb_stride_build(cc);     // L[cc] | 4 zeros, R[cc], 6 zeros
-    for (int e = t; e < ppw * stride; e += kSbT) {
+    for (int e = t; e < ppw * stride; e += NT) {
         const int sl = e / stride, x = e - sl * stride;
         int qi, p;
         M.at(sl, qi, p);
@@ -1721,7 +1757,7 @@ __device__ __forceinline__ void build_sb_body(const LevelJob &J, double *sh, con
     }
     __syncthreads();
     const int nqd = (pc + 3) >> 2;
-    for (int task = t; task < ppw * nqd; task += kSbT) {
+    for (int task = t; task < ppw * nqd; task += NT) {
         const int sl = task / nqd, m0 = (task - sl * nqd) * 4;
         int qi, p;
         M.at(sl, qi, p);
@@ -1750,13 +1786,13 @@ __device__ __forceinline__ void build_sb_body(const LevelJob &J, double *sh, con
     }
 }
 
-template <class Map>
+template <class Map, int NT = kSbT, bool OPQ = false>
 __device__ __forceinline__ void prop_sb_body(const LevelJob &J, double *sh, const Map &M, int ppw) {
-    const int t = threadIdx.x;
+    const int t = body_tid<OPQ>();
     const int cc = J.child.count, gc = J.gchild.count, gp = J.gpar.count;
     const int glen = gc + cc + 8;
     const int stride = sb_stride_prop(cc, gc);  // L[cc], R[cc], g[gc + cc + 8]
-    for (int e = t; e < ppw * stride; e += kSbT) {
+    for (int e = t; e < ppw * stride; e += NT) {
         const int sl = e / stride, x = e - sl * stride;
         int qi, pr;
         M.at(sl, qi, pr);
@@ -1778,7 +1814,7 @@ __device__ __forceinline__ void prop_sb_body(const LevelJob &J, double *sh, cons
     }
     __syncthreads();
     const int nqd = (gc + 3) >> 2;
-    for (int task = t; task < ppw * nqd; task += kSbT) {
+    for (int task = t; task < ppw * nqd; task += NT) {
         const int sl = task / nqd, m0 = (task - sl * nqd) * 4;
         int qi, pr;
         M.at(sl, qi, pr);
@@ -1891,6 +1927,360 @@ __global__ __launch_bounds__(512) void k_prop_lds_batch(const LevelJob *__restri
 #define ICM_PROP_F(FF) prop_lds_body<FF>(J, lds, M, tw, twShift)
     ICM_F_CASES(ICM_PROP_F)
 #undef ICM_PROP_F
+}
+
+// ---------------------------------------------------------------------------
+// Persistent ragged batch: one launch runs a q-pass's leaf build, every tree
+// level, the leaf extract and the fixed-order q reduction of all instances.
+//
+// XCD-local queues.  MI355X has eight XCDs, each with its own L2; a write is
+// seen by another XCD only after an L2 write-back there and an invalidate
+// here, which per work item cost 3.4x the whole batch (measured).  So the
+// host splits the instances into eight contiguous ranges of about equal
+// work, one item queue per XCD, and a workgroup serves the queue of the XCD
+// it runs on (hardware XCC id).  An instance's whole tree then lives in one
+// L2, and a dependency between items needs no L2 write-back: the producer
+// waits until its stores are complete (in L2) before the completion count
+// is raised, and the consumer drops its CU's L1 lines before reading.  The
+// L1 drop takes the device-scope cache invalidate (L1 and the L2's
+// non-coherent lines): the workgroup-scope one left stale L1 lines of reused
+// regions in place (wrong results in runs with small q-passes, where a
+// region fits a CU's L1 between its reuses).  Counters and tickets are
+// device-scope atomics.
+//
+// Within a queue, workgroups take items by ticket.  Items are ordered phase
+// by phase (leaves; build levels 1, 2, ..; propagate levels .., 2, 1;
+// extract; finalize), so every item's inputs come from items with smaller
+// tickets; ctr[i] counts the finished items of instance i, and an item starts
+// once its instance (every instance it spans) has finished the items of all
+// its earlier phases (`need`): an instance's next level starts as soon as its
+// own previous level is done, with no barrier between levels.  Progress: a
+// workgroup takes a ticket only while it runs, so the unfinished item with
+// the smallest ticket of a queue always has its inputs and proceeds.  Every
+// wait is further bounded by spin_limit (past it the error flag is set, the
+// item runs anyway and the host reports the error); a queue that no
+// workgroup served (an XCD without workgroups) is reported the same way by
+// the last workgroup to finish.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcc_id() {
+    int x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+    return x;
+}
+
+// This wave's global stores are complete (in its XCD's L2).
+__device__ __forceinline__ void stores_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Drop this CU's L1 lines (and the L2's non-coherent ones): later loads of
+// any wave on the CU read the XCD's L2.  Completed before returning.
+__device__ __forceinline__ void cache_invalidate() { asm volatile("buffer_inv sc1\n\ts_waitcnt vmcnt(0)" ::: "memory"); }
+
+__device__ __forceinline__ double dpp_row_shr1(double v) {
+    // lane l receives lane l - 1's value within each row of 16 lanes (row
+    // lane 0 receives 0); used within groups of 8 lanes, whose lane 0 ignores it
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(x & 0xffffffffLL), 0x111, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), 0x111, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Leaf build of the 64 global blocks gb0 .. gb0 + 63 (those below gend) for
+// every q-plane of the pass, eight lanes per block: lane r holds coefficients
+// r CPL .. r CPL + CPL - 1 (lane 7 also the top one, B).  Player i multiplies
+// by a + (1 - a) x, P[m] = a P[m] + (1 - a) P[m - 1], with the same operations
+// as the one-lane recurrence of k_leaf_build_batch (coefficients above the
+// current degree stay exactly 0); the coefficient below a lane's first comes
+// from the lane before it.  128 VGPRs suffice where one lane per block needs
+// B + 1 coefficients in registers.  The a values go through LDS (each
+// computed once, double-buffered by q-plane, pairs XOR-swizzled by block so
+// the eight blocks of a wave read distinct banks).
+template <int B>
+__device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ inst, const int *__restrict__ blk_inst,
+                                                const double *__restrict__ S, double *__restrict__ blocks, int NB,
+                                                int gend, int nq, int q0, int gb0, double *sa) {
+    constexpr int CPL = B / 8;
+    static_assert(CPL * 8 == B && 2 * 64 * B <= 2 * kTileMaxSize, "a values of two q-planes must fit the LDS");
+    const int t = body_tid<true>(), bl = t >> 3, r = t & 7, sw = bl & 7;
+    const int gb = gb0 + bl;
+    const bool has = gb < gend;
+    const BatchInst *I = has ? &inst[blk_inst[gb]] : nullptr;
+    const int b = has ? gb - I->boff : 0;
+    double Sv[CPL];
+    bool pv[CPL];
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+        const int j = b * B + r * CPL + u;
+        pv[u] = has && j < I->n;
+        Sv[u] = pv[u] ? S[I->poff + j] : 0.0;
+    }
+    const long long bplane = (long long)NB * (B + 1);
+    for (int qi = 0; qi < nq; ++qi) {
+        double *sq = sa + (qi & 1) * (64 * B) + bl * B;
+        const double lv = has ? I->logv[q0 + qi] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CPL; ++u) {
+            const int i = r * CPL + u;
+            sq[2 * ((i >> 1) ^ sw) + (i & 1)] = pv[u] ? a_of(Sv[u], lv) : 1.0;
+        }
+        __syncthreads();
+        double P[CPL], top = 0.0;
+#pragma unroll
+        for (int u = 0; u < CPL; ++u) P[u] = 0.0;
+        if (r == 0) P[0] = 1.0;
+#pragma unroll 2
+        for (int i2 = 0; i2 < B / 2; ++i2) {
+            const double2 ap = *reinterpret_cast<const double2 *>(sq + 2 * (i2 ^ sw));
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double a = h ? ap.y : ap.x;
+                const double bb = 1.0 - a;
+                const double left = dpp_row_shr1(P[CPL - 1]);
+                top = a * top + bb * P[CPL - 1];
+#pragma unroll
+                for (int u = CPL - 1; u >= 1; --u) P[u] = a * P[u] + bb * P[u - 1];
+                P[0] = (r == 0) ? P[0] * a : a * P[0] + bb * left;
+            }
+        }
+        if (has) {
+            double *dst = blocks + qi * bplane + (long long)gb * (B + 1) + r * CPL;
+#pragma unroll
+            for (int u = 0; u < CPL; ++u) dst[u] = P[u];
+            if (r == 7) dst[CPL] = top;
+        }
+    }
+}
+
+// Fixed-order q reduction of one player (the association of k_colsum: row
+// groups b mod RG, four running sums per group, groups added in order), then
+// the running total of earlier passes, and on the last pass the scatter to
+// the original order.
+__device__ __forceinline__ double colsum_group(const double *__restrict__ partial, int rows, long long n, int j,
+                                               int grp, int RG) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b = grp;
+    for (; b + 3 * RG < rows; b += 4 * RG) {
+        s0 += partial[(long long)b * n + j];
+        s1 += partial[(long long)(b + RG) * n + j];
+        s2 += partial[(long long)(b + 2 * RG) * n + j];
+        s3 += partial[(long long)(b + 3 * RG) * n + j];
+    }
+    for (; b < rows; b += RG) s0 += partial[(long long)b * n + j];
+    return (s0 + s1) + (s2 + s3);
+}
+
+__device__ __forceinline__ void persist_final(const PersistArgs &A, int j0, int jend) {
+    const int j = j0 + (int)threadIdx.x;
+    if (j >= jend) return;
+    const int rows = A.nchunk;
+    double s;
+    if (rows <= 2) {
+        s = colsum_group(A.partial, rows, A.Ntot, j, 0, 1);
+    } else {
+        s = colsum_group(A.partial, rows, A.Ntot, j, 0, 8);
+        for (int g = 1; g < 8; ++g) s += colsum_group(A.partial, rows, A.Ntot, j, g, 8);
+    }
+    if (A.accumulate) s += A.acc[j];
+    A.acc[j] = s;
+    if (A.equity) A.equity[A.perm[j]] = s;
+}
+
+// Waits until *ctr >= need.  Past `limit` polls it sets the error flag and
+// returns; once the flag is set (by any workgroup) waits return at once, so a
+// broken run drains after one bounded wait instead of one per item.
+__device__ __forceinline__ void persist_wait(int *ctr, int need, int *err, int limit) {
+    int spins = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        if (++spins > limit || ((spins & 255) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+constexpr int kPersistLds = kTileMaxSize * (int)sizeof(double2);
+
+template <int B> struct PersistCfg {
+    static constexpr int PPL = (B == 64) ? 2 : 1;             // players per lane of the extract
+    using X = LeafExN<B, PPL>;
+    static constexpr int EXB = 2 * X::BPW;                     // blocks per extract item (two halves)
+    static_assert(2 * 2 * (X::NP + X::NG) <= 2 * kTileMaxSize, "extract halves must fit the LDS");
+};
+
+// The persistent kernel's LDS: dynamic, one 64 KB array that every item body
+// addresses by name (LDS instructions also in the functions kept out of line).
+extern __shared__ double2 persist_lds[];
+
+// The leaf build and the extract as functions of their own (not inlined):
+// inlined beside the level bodies they pushed the kernel's register
+// allocation into hundreds of spills, although each fits 128 VGPRs alone.
+// A few thousand such items per pass make the call cost negligible.
+// Arguments arrive in VGPRs: the loop bounds are made wave-uniform again.
+template <int B>
+__device__ __attribute__((noinline)) void persist_leaf(const BatchInst *__restrict__ inst,
+                                                       const int *__restrict__ blk_inst,
+                                                       const double *__restrict__ S, double *__restrict__ blocks,
+                                                       int NB, int gend, int nq, int q0, int gb0) {
+    NB = __builtin_amdgcn_readfirstlane(NB);
+    gend = __builtin_amdgcn_readfirstlane(gend);
+    nq = __builtin_amdgcn_readfirstlane(nq);
+    q0 = __builtin_amdgcn_readfirstlane(q0);
+    gb0 = __builtin_amdgcn_readfirstlane(gb0);
+    leaf_build_coop<B>(inst, blk_inst, S, blocks, NB, gend, nq, q0, gb0, reinterpret_cast<double *>(persist_lds));
+}
+
+template <int B>
+__device__ __attribute__((noinline)) void persist_extract(const BatchInst *__restrict__ inst,
+                                                          const int *__restrict__ blk_inst,
+                                                          const double *__restrict__ S,
+                                                          const double *__restrict__ blocks, int NB, int gend, int nq,
+                                                          int q0, int q_per_chunk, double *partial, int Ntot, int gb0,
+                                                          int chunk) {
+    using C = PersistCfg<B>;
+    using X = typename C::X;
+    NB = __builtin_amdgcn_readfirstlane(NB);
+    gend = __builtin_amdgcn_readfirstlane(gend);
+    nq = __builtin_amdgcn_readfirstlane(nq);
+    q0 = __builtin_amdgcn_readfirstlane(q0);
+    q_per_chunk = __builtin_amdgcn_readfirstlane(q_per_chunk);
+    Ntot = __builtin_amdgcn_readfirstlane(Ntot);
+    gb0 = __builtin_amdgcn_readfirstlane(gb0);
+    chunk = __builtin_amdgcn_readfirstlane(chunk);
+    const int t = threadIdx.x;
+    double *base = reinterpret_cast<double *>(persist_lds) + (t >> 8) * 2 * (X::NP + X::NG);
+    leaf_extract_batch_n_body<B, C::PPL>(t & 255, gb0 + (t >> 8) * X::BPW, chunk, inst, blk_inst, S, blocks, NB,
+                                         gend, nq, q0, q_per_chunk, partial, Ntot,
+                                         reinterpret_cast<double(*)[X::NP]>(base),
+                                         reinterpret_cast<double(*)[X::NG]>(base + 2 * X::NP));
+}
+
+// Dynamic LDS: kPersistLds bytes (the 4096-point tile, 64 KB).  128 VGPRs
+// (four waves per SIMD): two workgroups of 512 threads per CU, as the
+// per-level batch kernels.
+template <int B>
+__global__ __launch_bounds__(kLT, 4) void k_batch_persist(PersistArgs A) {
+    double2 *lds = persist_lds;
+    __shared__ int s_tk, s_last;
+    // No branch of the item loop may be loop-invariant and divergent: the
+    // compiler would unswitch the loop on it, and a wave runs the two loop
+    // copies one after the other -- lanes without the ticket fetch (t != 0)
+    // would cycle through barriers and items on a ticket that never
+    // advances while lane 0 waits for them.  So the thread index is taken
+    // afresh in every iteration (opaque), and the ticket and item fields are
+    // made wave-uniform with readfirstlane.
+    // An item is counted done at the top of the next iteration, after the
+    // next ticket is taken: its stores drain meanwhile (and before any wait
+    // of the next item, which may need it).
+    int pend = -1, pend_i1 = 0;   // kind_inst and last instance of the item to count
+    for (;;) {
+        int t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        // the queue of this workgroup's XCD (read afresh: fewer values live across the items)
+        const int x = __builtin_amdgcn_readfirstlane(xcc_id()) % A.nqueues;
+        if (t == 0) s_tk = atomicAdd(&A.sched[x], 1);
+        __syncthreads();
+        const int tk = __builtin_amdgcn_readfirstlane(s_tk);
+        __syncthreads();
+        if (pend >= 0) {   // outputs complete in this XCD's L2, then count the item done
+            stores_complete();
+            __syncthreads();
+            const int pk = pend & 7, p0 = pend >> 3;
+            if (pk == kPLeaf || pk == kPExtract) {
+                if (p0 + t <= pend_i1)
+                    __hip_atomic_fetch_add(&A.ctr[p0 + t], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else if (t == 0) {
+                __hip_atomic_fetch_add(&A.ctr[p0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            pend = -1;
+        }
+        if (tk >= A.qoff[x + 1] - A.qoff[x]) break;
+        PItem it = A.items[A.qoff[x] + tk];
+        it.kind_inst = __builtin_amdgcn_readfirstlane(it.kind_inst);
+        it.a = __builtin_amdgcn_readfirstlane(it.a);
+        it.b = __builtin_amdgcn_readfirstlane(it.b);
+        it.need = __builtin_amdgcn_readfirstlane(it.need);
+        const int kind = it.kind_inst & 7, i0 = it.kind_inst >> 3;
+        const bool multi = kind == kPLeaf || kind == kPExtract || kind == kPFinal;
+        // inputs: wait for the instance's (instances') earlier phases (the
+        // waiting lanes are in wave 0), then read through a clean L1
+        if (kind == kPExtract || kind == kPFinal) {
+            const int i = i0 + t;
+            if (i <= it.need)
+                persist_wait(&A.ctr[i], kind == kPExtract ? A.need_ex[i] : A.need_fin[i], &A.sched[kPSchedErr],
+                             A.spin_limit);
+        } else if (!multi && t == 0 && it.need > 0) {
+            persist_wait(&A.ctr[i0], it.need, &A.sched[kPSchedErr], A.spin_limit);
+        }
+        if (kind != kPLeaf && t < 64) cache_invalidate();
+        __syncthreads();
+        switch (kind) {
+        case kPLeaf:
+            persist_leaf<B>(A.inst, A.blk_inst, A.S, A.blocks, A.NB, A.qend[2 * x], A.nq, A.q0, it.a);
+            break;
+        case kPBuildLds: {
+            const LevelJob J = A.jobs[it.a];
+            const int npair = (J.nr_parent + 1) >> 1;
+            const FlatMap M{it.b, npair, J.nq * npair};
+#define ICM_BUILD_F(FF) build_lds_body<FF, FlatMap, kLT, true>(J, lds, M, A.tw, A.twShift)
+            ICM_F_CASES(ICM_BUILD_F)
+#undef ICM_BUILD_F
+            break;
+        }
+        case kPPropLds: {
+            const LevelJob J = A.jobs[it.a];
+            const int npair = (J.np + 1) >> 1;
+            const FlatMap M{it.b, npair, J.nq * npair};
+#define ICM_PROP_F(FF) prop_lds_body<FF, FlatMap, true>(J, lds, M, A.tw, A.twShift)
+            ICM_F_CASES(ICM_PROP_F)
+#undef ICM_PROP_F
+            break;
+        }
+        case kPBuildSb: {
+            const LevelJob J = A.jobs[it.a];
+            build_sb_body<FlatMap, kLT, true>(J, reinterpret_cast<double *>(lds), FlatMap{it.b, J.nr_parent,
+                                        J.nq * J.nr_parent}, sb_ppw(sb_stride_build(J.child.count)));
+            break;
+        }
+        case kPPropSb: {
+            const LevelJob J = A.jobs[it.a];
+            prop_sb_body<FlatMap, kLT, true>(J, reinterpret_cast<double *>(lds), FlatMap{it.b, J.np, J.nq * J.np},
+                                       sb_ppw(sb_stride_prop(J.child.count, J.gchild.count)));
+            break;
+        }
+        case kPExtract:
+            persist_extract<B>(A.inst, A.blk_inst, A.S, A.blocks, A.NB, A.qend[2 * x], A.nq, A.q0, A.q_per_chunk,
+                               A.partial, A.Ntot, it.a, it.b);
+            break;
+        case kPFinal:
+            persist_final(A, it.a, A.qend[2 * x + 1]);
+            break;
+        default:
+            break;
+        }
+        if (kind != kPFinal) {   // nothing waits for finalize items
+            pend = it.kind_inst;
+            pend_i1 = it.need;
+        }
+    }
+    // the last workgroup to leave checks that every queue was served and
+    // clears the counters for the next launch
+    const int t = threadIdx.x;
+    if (t == 0) {
+        const int e = __hip_atomic_fetch_add(&A.sched[kPSchedDone], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (e == (int)gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(s_last)) {
+        if (t < A.nqueues) {
+            const int served = __hip_atomic_load(&A.sched[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (served < A.qoff[t + 1] - A.qoff[t])
+                __hip_atomic_store(&A.sched[kPSchedErr], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        for (int i = t; i < A.ninst; i += kLT) A.ctr[i] = 0;
+        __syncthreads();
+        if (t < A.nqueues) A.sched[t] = 0;
+        if (t == 0) A.sched[kPSchedDone] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3776,6 +4166,38 @@ hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *i
     else k_prop_sb_batch<<<nitems, kSbT, 0, s>>>(jobs, items);
     hipError_t e = hipGetLastError();
     PF_END(build ? "k_build_sb_batch" : "k_prop_sb_batch", bytes, flops);
+    return e;
+}
+
+// The persistent kernel is built for the batch's tuned block size, 64; other
+// block sizes run the per-level launches.
+int persist_extract_blocks(int B) { return B == 64 ? PersistCfg<64>::EXB : 0; }
+
+template <int B> static int persist_grid_b() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(k_batch_persist<B>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kPersistLds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_persist<B>, kLT, kPersistLds) != hipSuccess)
+        return 0;
+    return cus * per;
+}
+
+int batch_persist_grid(int B) { return B == 64 ? persist_grid_b<64>() : 0; }
+
+hipError_t launch_batch_persist(int B, const PersistArgs &A, int grid, hipStream_t s, Prof *pf, double bytes,
+                                double flops) {
+    if (A.nitems < 1) return hipSuccess;
+    if (grid < 1) return hipErrorInvalidValue;
+    PF_BEGIN();
+    if (B != 64) return hipErrorInvalidValue;
+    k_batch_persist<64><<<grid, kLT, kPersistLds, s>>>(A);
+    hipError_t e = hipGetLastError();
+    PF_END("k_batch_persist", bytes, flops);
     return e;
 }
 

@@ -233,6 +233,74 @@ hipError_t launch_prop_sb(const LevelJob &J, hipStream_t s, Prof *pf);
 hipError_t launch_level_sb_batch(bool build, const LevelJob *jobs, const int2 *items, int nitems, hipStream_t s,
                                  Prof *pf, double bytes, double flops);
 
+// Persistent ragged batch: one launch per q-pass runs every phase (leaf
+// build, build and propagate levels, extract, finalize) as ticketed lists of
+// work items with per-instance completion counters (no barrier between
+// levels), one list per XCD: instance ranges [qinst[x], qinst[x+1]) are
+// served only by workgroups on XCD x, so an instance's data stays in one L2.
+// Item kinds and fields:
+//   kPLeaf      a = first global block (kPLeafBlocks blocks, every q-plane)
+//   kPBuildLds, kPBuildSb, kPPropLds, kPPropSb
+//               a = job (LevelJob index), b = first tile item (as the
+//               per-level launches), need = finished items of the instance
+//               required first
+//   kPExtract   a = first global block (persist_extract_blocks(B)), b = q-chunk
+//   kPFinal     a = first player (sorted order, kPFinalPlayers players)
+// kind_inst = kind | (first instance << 3); leaf, extract and final items
+// span instances first .. need (the need counts of extract and final come
+// from need_ex / need_fin per instance).
+enum PersistKind { kPLeaf = 0, kPBuildLds = 1, kPBuildSb = 2, kPPropLds = 3, kPPropSb = 4, kPExtract = 5, kPFinal = 6 };
+struct PItem {
+    int kind_inst;
+    int a;
+    int b;
+    int need;
+};
+constexpr int kPLeafBlocks = 64;
+constexpr int kPFinalPlayers = 512;
+// Queues (one per XCD of MI355X) and the scheduler array: sched[x] ticket of
+// queue x, sched[kPSchedDone] workgroups finished, sched[kPSchedErr] error
+// (1: a wait passed spin_limit, 2: a queue no workgroup served).
+constexpr int kPMaxQueues = 8;
+constexpr int kPSchedDone = kPMaxQueues;
+constexpr int kPSchedErr = kPMaxQueues + 1;
+constexpr int kPSchedSize = 16;
+struct PersistArgs {
+    const PItem *items;       // queue x: items[qoff[x] .. qoff[x+1])
+    int nitems;
+    const int *qoff;          // [nqueues + 1]
+    const int *qend;          // [nqueues][2]: end of queue x's global blocks, of its players
+    int nqueues;
+    const LevelJob *jobs;
+    const BatchInst *inst;
+    const int *blk_inst;
+    const double *S;          // sorted stacks, concatenated
+    double *blocks;           // [nq][NB][B+1]
+    int NB, nq, q0, ninst;
+    const double2 *tw;
+    int twShift;
+    int nchunk, q_per_chunk;
+    double *partial;          // [nchunk][Ntot]
+    int Ntot;
+    double *acc;              // [Ntot] running total over passes (sorted order)
+    int accumulate;           // add acc (passes after the first)
+    const int *perm;          // sorted position -> flat original index
+    double *equity;           // last pass: equity[perm[j]] = acc[j]; null otherwise
+    const int *need_ex;       // [ninst] finished items before an instance's extract
+    const int *need_fin;      // [ninst] finished items before its finalize
+    int *ctr;                 // [ninst] finished items (zero at launch; cleared by the last workgroup)
+    int *sched;               // [kPSchedSize], layout above
+    int spin_limit;
+};
+// Blocks per extract item of the persistent kernel (two groups of 256
+// threads); 0 for a block size the kernel is not built for (only 64 is).
+int persist_extract_blocks(int B);
+// Workgroups of the persistent kernel resident at once on the current device
+// (0 on error); the launch uses at most this many.
+int batch_persist_grid(int B);
+hipError_t launch_batch_persist(int B, const PersistArgs &A, int grid, hipStream_t s, Prof *pf, double bytes,
+                                double flops);
+
 // Small instances in one launch: one workgroup per quadrature node (a
 // workgroup loops over nodes b, b + G, ... when there are more nodes than
 // workgroups), the whole tree with blocks of one player in LDS (product

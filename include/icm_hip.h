@@ -355,6 +355,19 @@ int icm_hip_batch_summary(const IcmHipBatchPlan *plan, int *n_inst, int *n_playe
 /* Schoolbook / transform crossover of the batch's forward levels (as
  * icm_hip_plan_set_schoolbook_max; the adjoint keeps transforms). */
 int icm_hip_batch_set_schoolbook_max(IcmHipBatchPlan *plan, int cmax);
+/* Forward batch as ONE persistent launch per q-pass (k_batch_persist) instead
+ * of one launch per phase and tree level: workgroups take leaf, level,
+ * extract and reduction work items by ticket, and each instance starts its
+ * next level as soon as its own previous level is done (per-instance
+ * completion counters, no barrier between levels).  Same equities as the
+ * per-level launches.  Needs block size 64 (the tuned default; -1 otherwise);
+ * the default comes from ICM_HIP_BATCH_PERSISTENT (0/1).  The captured graph
+ * then holds that one kernel per pass.  The adjoint keeps per-level launches.
+ * set_spin_limit bounds each item's wait for its inputs (default 2^20 polls,
+ * under a second): past it the run fails with an error instead of waiting on. */
+int icm_hip_batch_set_persistent(IcmHipBatchPlan *plan, int on);
+int icm_hip_batch_is_persistent(const IcmHipBatchPlan *plan);
+int icm_hip_batch_set_spin_limit(IcmHipBatchPlan *plan, int spins);
 /* Per-kernel timing of batch runs (runs eagerly, without the graph, while on). */
 int icm_hip_batch_set_profiling(IcmHipBatchPlan *plan, int on);
 int icm_hip_batch_kernel_count(const IcmHipBatchPlan *plan);
