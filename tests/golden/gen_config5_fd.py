@@ -1,11 +1,11 @@
 """Writes tests/golden/config5_fd.npz: central finite differences of the CPU
 oracle's v . E at the adjoint workload (n = 131072, k = 1000, Q = 256,
 stacks lognormal(0, 1.5), cotangent v ~ N(0, 1), linear payout k - j
-normalised), on 64 random coordinates at h = 1e-6 S_i and the first 16 of
+normalised), on 256 random coordinates at h = 1e-6 S_i and the first 16 of
 them at h = 1e-5 S_i.  The largest stack is excluded: the quadrature nodes
 depend on max S, which the adjoint holds fixed.
 
-Run from the repository root (about 12 minutes on 8 threads):
+Run from the repository root (about 40 minutes on 8 threads):
     python tests/golden/gen_config5_fd.py [threads]
 """
 import os
@@ -28,7 +28,7 @@ def workload():
     p = np.arange(K, 0, -1, dtype=np.float64)
     p /= p.sum()
     imax = int(np.argmax(S))
-    coords = [int(i) for i in rng.choice(N, 80, replace=False) if i != imax][:64]
+    coords = [int(i) for i in rng.choice(N, 264, replace=False) if i != imax][:256]
     return S, v, p, np.array(coords, dtype=np.int64)
 
 

@@ -15,8 +15,9 @@
     seed 4): every instance against the oracle run here;
   * adjoint at n = 131072, k = 1000 (stacks lognormal(0, 1.5), v ~ N(0, 1)):
     dS against central differences of the oracle's v . E at h = 1e-6 S_i on
-    64 coordinates and h = 1e-5 S_i on 16 (tests/golden/gen_config5_fd.py,
-    config5_fd.npz);
+    256 coordinates and h = 1e-5 S_i on 16 (tests/golden/gen_config5_fd.py,
+    config5_fd.npz), each coordinate within 2e-7 max|dS| plus the difference
+    quotient's rounding noise eps sum|v E| / h_i;
   * the analytic gradient of the quadratic closed form at n = 64 and 96.
 """
 import math
@@ -153,11 +154,19 @@ def test_config5_vjp_vs_oracle_finite_differences():
     assert abs(float(np.dot(v, eq)) - float(f["vE"])) <= 1e-11 * float(np.dot(np.abs(v), eq))
     c = f["coords"]
     scale = float(np.max(np.abs(ds)))
-    e6 = float(np.max(np.abs(ds[c] - f["fd6"]))) / scale
-    e5 = float(np.max(np.abs(ds[c[:len(f["fd5"])]] - f["fd5"]))) / scale
-    say(f"config5 adjoint: max |dS - FD| / max|dS| = {e6:.2e} (h=1e-6 S, {len(c)} coords), "
-        f"{e5:.2e} (h=1e-5 S, {len(f['fd5'])} coords)")
-    assert e6 < 2e-5 and e5 < 2e-6
+    # Per coordinate: 2e-7 max|dS| (the adjoint's accuracy and the O(h^2)
+    # truncation) plus the rounding noise of a central difference of the
+    # oracle's v . E, eps sum|v E| / h_i (h_i = rel S_i: small stacks have
+    # tiny steps; observed noise at most 0.1 of this term).
+    noise = np.finfo(np.float64).eps * float(np.sum(np.abs(v * eq)))
+    worst = 0.0
+    for rel, fd, cc in ((1e-6, f["fd6"], c), (1e-5, f["fd5"], c[:len(f["fd5"])])):
+        err = np.abs(ds[cc] - fd)
+        bound = 2e-7 * scale + noise / (rel * S[cc])
+        worst = max(worst, float(np.max(err / bound)))
+        say(f"config5 adjoint, h={rel:g} S on {len(cc)} coords: max |dS - FD| / max|dS| = "
+            f"{float(np.max(err)) / scale:.2e}, max error / bound = {float(np.max(err / bound)):.3f}")
+    assert worst <= 1.0
 
 
 def _quadratic_gradient(S, v):
