@@ -13,6 +13,7 @@
 #include "fft_lds.hpp"
 #include "icm_kernels.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1984,7 +1985,7 @@ __device__ __forceinline__ double dpp_row_shr1(double v) {
 }
 
 // Leaf build of the 64 global blocks gb0 .. gb0 + 63 (those below gend) for
-// every q-plane of the pass, eight lanes per block: lane r holds coefficients
+// q-planes qa .. qe - 1 of the pass, eight lanes per block (512 threads): lane r holds coefficients
 // r CPL .. r CPL + CPL - 1 (lane 7 also the top one, B).  Player i multiplies
 // by a + (1 - a) x, P[m] = a P[m] + (1 - a) P[m - 1], with the same operations
 // as the one-lane recurrence of k_leaf_build_batch (coefficients above the
@@ -1996,7 +1997,7 @@ __device__ __forceinline__ double dpp_row_shr1(double v) {
 template <int B>
 __device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ inst, const int *__restrict__ blk_inst,
                                                 const double *__restrict__ S, double *__restrict__ blocks, int NB,
-                                                int gend, int nq, int q0, int gb0, double *sa) {
+                                                int gend, int qa, int qe, int q0, int gb0, double *sa) {
     constexpr int CPL = B / 8;
     static_assert(CPL * 8 == B && 2 * 64 * B <= 2 * kTileMaxSize, "a values of two q-planes must fit the LDS");
     const int t = body_tid<true>(), bl = t >> 3, r = t & 7, sw = bl & 7;
@@ -2013,7 +2014,7 @@ __device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ in
         Sv[u] = pv[u] ? S[I->poff + j] : 0.0;
     }
     const long long bplane = (long long)NB * (B + 1);
-    for (int qi = 0; qi < nq; ++qi) {
+    for (int qi = qa; qi < qe; ++qi) {
         double *sq = sa + (qi & 1) * (64 * B) + bl * B;
         const double lv = has ? I->logv[q0 + qi] : 0.0;
 #pragma unroll
@@ -2083,6 +2084,19 @@ __device__ __forceinline__ void persist_final(const PersistArgs &A, int j0, int 
     if (A.equity) A.equity[A.perm[j]] = s;
 }
 
+// Batch leaf build with eight lanes per block (the per-level launches' leaf
+// phase at B = 64): 64 blocks per workgroup, q-planes blockIdx.y * qper ..
+// + qper - 1.  The same products as k_leaf_build_batch.
+template <int B>
+__global__ __launch_bounds__(kLT) void k_leaf_build_batch_coop(const BatchInst *__restrict__ inst,
+                                                               const int *__restrict__ blk_inst,
+                                                               const double *__restrict__ S, double *blocks, int NB,
+                                                               int nq, int q0, int qper) {
+    __shared__ double sa[2 * 64 * B];
+    const int qa = blockIdx.y * qper;
+    leaf_build_coop<B>(inst, blk_inst, S, blocks, NB, NB, qa, min(nq, qa + qper), q0, blockIdx.x * 64, sa);
+}
+
 // Waits until *ctr >= need.  Past `limit` polls it sets the error flag and
 // returns; once the flag is set (by any workgroup) waits return at once, so a
 // broken run drains after one bounded wait instead of one per item.
@@ -2126,7 +2140,7 @@ __device__ __attribute__((noinline)) void persist_leaf(const BatchInst *__restri
     nq = __builtin_amdgcn_readfirstlane(nq);
     q0 = __builtin_amdgcn_readfirstlane(q0);
     gb0 = __builtin_amdgcn_readfirstlane(gb0);
-    leaf_build_coop<B>(inst, blk_inst, S, blocks, NB, gend, nq, q0, gb0, reinterpret_cast<double *>(persist_lds));
+    leaf_build_coop<B>(inst, blk_inst, S, blocks, NB, gend, 0, nq, q0, gb0, reinterpret_cast<double *>(persist_lds));
 }
 
 template <int B>
@@ -4019,10 +4033,29 @@ hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const do
 // ---------------------------------------------------------------------------
 // Ragged-batch launchers
 // ---------------------------------------------------------------------------
+// Leaf build of the batch at B = 64: eight lanes per block (94 VGPRs, four
+// waves per SIMD) unless ICM_HIP_LEAF_COOP=0 (read at each launch; a captured
+// graph keeps the choice of its capture) selects one lane per block.
+static bool leaf_coop() {
+    const char *e = getenv("ICM_HIP_LEAF_COOP");
+    return !(e && atoi(e) == 0);
+}
+
 hipError_t launch_leaf_build_batch(int B, const BatchInst *inst, const int *blk_inst, const double *S,
                                    double *blocks, int NB, int nq, int q0, hipStream_t s, Prof *pf, double bytes) {
     if (NB < 1 || nq < 1) return hipSuccess;
     PF_BEGIN();
+    if (B == 64 && leaf_coop()) {
+        // about 4096 workgroups: columns of 64 blocks, rows of q-planes
+        const int cols = (NB + 63) / 64;
+        const int rows = std::max(1, std::min(nq, (4096 + cols - 1) / cols));
+        const int qper = (nq + rows - 1) / rows;
+        k_leaf_build_batch_coop<64><<<dim3(cols, (nq + qper - 1) / qper), kLT, 0, s>>>(inst, blk_inst, S, blocks, NB,
+                                                                                      nq, q0, qper);
+        hipError_t e = hipGetLastError();
+        PF_END("k_leaf_build_batch_coop", bytes, 0.0);
+        return e;
+    }
     dim3 grid((NB + 63) / 64, nq);
     switch (B) {
     case 16: k_leaf_build_batch<16><<<grid, 64, 0, s>>>(inst, blk_inst, S, blocks, NB, q0); break;

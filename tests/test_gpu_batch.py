@@ -189,3 +189,24 @@ def test_batch_leaf_extract_players_per_lane_bit_identical(monkeypatch):
             bp.close()
     say(f"batch leaf extract per-lane players: {len(stacks)} instances")
     assert np.array_equal(out["1"], out["2"])
+
+
+def test_batch_leaf_build_eight_lanes_vs_one_lane(monkeypatch):
+    """The batch leaf build with eight lanes per block (B = 64) against one
+    lane per block (ICM_HIP_LEAF_COOP=0): the same recurrence; equal to
+    rounding (the two kernels may form their multiply-adds differently),
+    and both runs of each kernel deterministic."""
+    stacks, payouts = ragged_instances(400, seed=23, n_max=4096)
+    out = {}
+    for coop in ("0", "1"):
+        monkeypatch.setenv("ICM_HIP_LEAF_COOP", coop)
+        bp = icm_amd.BatchPlan(stacks, [len(p) for p in payouts])
+        try:
+            out[coop] = np.concatenate(bp.run(payouts, 256))
+            assert np.array_equal(out[coop], np.concatenate(bp.run(payouts, 256)))
+        finally:
+            bp.close()
+    d = float(np.max(np.abs(out["1"] - out["0"])))
+    say(f"batch leaf build, eight lanes vs one lane per block: max |diff| {d:.2e}, "
+        f"identical: {np.array_equal(out['0'], out['1'])}")
+    assert d <= 1e-14 * float(np.max(np.abs(out["0"])))
