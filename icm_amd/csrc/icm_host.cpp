@@ -777,11 +777,13 @@ bool ensure_vjp(IcmHipPlan *p) {
         if (best && planes(bper) > qv) { m = best; qv = planes(bper); }
     }
     // the tangent spectrum cache, beside the forward's, while the q-batch
-    // keeps at least min(qv, 32) planes (not with checkpointing: recomputed
-    // segments would have to rebuild it)
+    // keeps at least min(qv, 32) planes; also with checkpointing: a tangent
+    // build writes its level's entry whenever it runs (recomputed segments
+    // rewrite the same values), and with it the four-step levels' down-sweep
+    // reads only spectra, so their tangent products need no recompute
     bool dspec = false;
-    if (m == 0 && p->d_spec && p->spec_elems > 0) {
-        const int qd = planes(per_of(0) + 16 * p->spec_elems);
+    if (p->d_spec && p->spec_elems > 0) {
+        const int qd = planes(per_of(m) + 16 * p->spec_elems);
         if (qd >= std::min(qv, 32)) {
             qv = qd;
             dspec = true;
@@ -939,7 +941,10 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         int ca = 0, cb = 0;  // current A / B buffers (A in d_g, B in d_bv)
         for (int l = L - 1; l >= 1; --l) {
             const bool root = (l == L - 1);
-            if (m >= 2 && l - 1 >= 1 && !ckpt_keeps(m, l - 1) && (l - 1) / m != resident) {
+            // this level reads the children's tangent products as coefficients
+            // (LDS levels, or four-step levels without the tangent spectra)
+            const bool needs_dp = g.F[l] <= icm::kMaxLdsF || !dspec_of(l) || root;
+            if (m >= 2 && needs_dp && l - 1 >= 1 && !ckpt_keeps(m, l - 1) && (l - 1) / m != resident) {
                 for (int j = (l - 1) / m * m + 1; j <= l - 1; ++j)
                     if (!build_level(j, true)) return -1;
                 resident = (l - 1) / m;

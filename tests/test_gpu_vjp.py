@@ -213,12 +213,12 @@ def test_vjp_spectrum_cache_matches_recompute(n, k, q_batch, Q):
 @pytest.mark.parametrize("n,k,q_batch,Q", [(65536, 65536, 0, 16), (20000, 20000, 2, 5), (131072, 1000, 0, 16)])
 def test_vjp_checkpointing_matches_keeping_every_level(n, k, q_batch, Q):
     """Checkpointing keeps the tangent products of every m-th tree level and
-    rebuilds each segment from its lower checkpoint in the reverse sweep.
-    The rebuilt levels come from the same kernels on the same inputs, so every
-    interval gives the same bits as keeping every level without the tangent
-    spectrum cache (the cache-off plan), and agrees with the default
-    (cached) adjoint to rounding; the oracle's finite differences pin dS at a
-    few coordinates."""
+    rebuilds a segment from its lower checkpoint when the reverse sweep needs
+    its coefficients (levels that read the tangent pairs' cached spectra need
+    none).  The rebuilt levels come from the same kernels on the same inputs
+    and the tangent spectrum cache is kept beside the checkpoints, so every
+    interval gives the same bits as the default adjoint that keeps every
+    level; the plan without spectrum caches agrees to rounding."""
     rng = np.random.default_rng(n + k + 11)
     S = np.exp(rng.uniform(0.0, math.log(1e4), n))
     r = np.arange(k, dtype=float)
@@ -241,10 +241,10 @@ def test_vjp_checkpointing_matches_keeping_every_level(n, k, q_batch, Q):
     e0, d0 = plan.vjp(p, v, Q=Q)
     plan.close()
     for m, (e, d) in runs.items():
-        assert np.array_equal(e, e0) and np.array_equal(d, d0), \
-            f"interval {m}: max |dS diff| {np.max(np.abs(d - d0)):.3e}"
+        assert np.array_equal(e, e_def) and np.array_equal(d, d_def), \
+            f"interval {m}: max |dS diff| {np.max(np.abs(d - d_def)):.3e}"
     err = np.max(np.abs(d0 - d_def)) / np.max(np.abs(d_def))
-    say(f"  checkpointed vs cached adjoint: {err:.2e} of max|dS|")
+    say(f"  without spectrum caches vs the cached adjoint: {err:.2e} of max|dS|")
     assert err < 1e-11 and np.max(np.abs(e0 - e_def)) <= 1e-12 * np.max(np.abs(e_def))
 
 
