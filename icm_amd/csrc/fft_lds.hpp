@@ -233,15 +233,30 @@ template <int R, bool INV> __device__ __forceinline__ void dft(double2 *a) {
 // and w^8 come from the table (correctly rounded); the other powers are
 // products of at most three of them, so each twiddle carries a few ulp at
 // most.  Loading 4 instead of 15 entries keeps the register footprint small.
-// The table pointer is deliberately not __restrict__: its loads then stay
-// below the preceding barrier instead of being hoisted across passes.
+// The passes load the entries (twiddle_load) before their LDS reads and
+// barrier and apply them after (twiddle_apply), so the table's latency
+// overlaps the reads and the wait at the barrier.  The table pointer is
+// deliberately not __restrict__: its loads then stay below the preceding
+// pass's barrier instead of being hoisted across passes.
+template <int R> struct TwRaw {
+    double2 e[R >= 16 ? 4 : R >= 8 ? 3 : R >= 4 ? 2 : 1];   // w, w^2, w^4, w^8
+};
+template <int R>
+__device__ __forceinline__ TwRaw<R> twiddle_load(const double2 *tw, int k, int twSh) {
+    TwRaw<R> r;
+    r.e[0] = tw[k << twSh];
+    if constexpr (R >= 4) r.e[1] = tw[(2 * k) << twSh];
+    if constexpr (R >= 8) r.e[2] = tw[(4 * k) << twSh];
+    if constexpr (R >= 16) r.e[3] = tw[(8 * k) << twSh];
+    return r;
+}
 template <int R, bool INV>
-__device__ __forceinline__ void twiddle(double2 *v, const double2 *tw, int k, int twSh) {
+__device__ __forceinline__ void twiddle_apply(double2 *v, const TwRaw<R> &r) {
     double2 w[R];
-    w[1] = tw[k << twSh];
-    if constexpr (R >= 4) w[2] = tw[(2 * k) << twSh];
-    if constexpr (R >= 8) w[4] = tw[(4 * k) << twSh];
-    if constexpr (R >= 16) w[8] = tw[(8 * k) << twSh];
+    w[1] = r.e[0];
+    if constexpr (R >= 4) w[2] = r.e[1];
+    if constexpr (R >= 8) w[4] = r.e[2];
+    if constexpr (R >= 16) w[8] = r.e[3];
     if constexpr (R >= 4) w[3] = c_mul(w[1], w[2]);
     if constexpr (R >= 8) {
         w[5] = c_mul(w[1], w[4]);
@@ -268,6 +283,8 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
     const int Ns = 1 << logNs;
     double2 v[PER][R];
     int item[PER], jj[PER];
+    TwRaw<R> w[PER];
+    const int twSh = 12 - (logNs + ilog2c(R));   // into the 4096-entry part
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int bj = t + u * NT;
@@ -279,15 +296,15 @@ __device__ __forceinline__ void fft_pass(double2 *lds, int logNs, const double2 
             item[u] = bj >> LOG_NBI;
             jj[u] = bj & (NBI - 1);
         }
+        if (logNs > 0) w[u] = twiddle_load<R>(tw, jj[u] & (Ns - 1), twSh);
 #pragma unroll
         for (int s = 0; s < R; ++s) v[u][s] = lds[TL::at(item[u], jj[u] + s * NBI)];
     }
     __syncthreads();
-    const int twSh = 12 - (logNs + ilog2c(R));   // into the 4096-entry part
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int k = jj[u] & (Ns - 1);
-        if (logNs > 0) twiddle<R, INV>(v[u], tw, k, twSh);
+        if (logNs > 0) twiddle_apply<R, INV>(v[u], w[u]);
         dft<R, INV>(v[u]);
         const int d0 = (jj[u] - k) * R + k;
 #pragma unroll
@@ -315,6 +332,8 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
     const int Ns = 1 << logNs;
     double2 v[PER][R];
     int item[PER], jj[PER];
+    TwRaw<R> w[PER];
+    const int twSh = 12 - (logNs + ilog2c(R));
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int bj = t + u * NT;
@@ -326,6 +345,7 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
             item[u] = bj >> LOG_NBI;
             jj[u] = bj & (NBI - 1);
         }
+        if (logNs > 0) w[u] = twiddle_load<R>(tw, jj[u] & (Ns - 1), twSh);
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             if constexpr (LOAD) v[u][s] = ld(item[u], jj[u] + s * NBI);
@@ -335,11 +355,10 @@ __device__ __forceinline__ void fft_pass_io(double2 *lds, int logNs, const doubl
     // loads: the caller's earlier LDS reads must finish before this pass
     // writes; LDS reads: every read of this pass before anyone writes
     __syncthreads();
-    const int twSh = 12 - (logNs + ilog2c(R));
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
         const int k = jj[u] & (Ns - 1);
-        if (logNs > 0) twiddle<R, INV>(v[u], tw, k, twSh);
+        if (logNs > 0) twiddle_apply<R, INV>(v[u], w[u]);
         dft<R, INV>(v[u]);
         const int d0 = (jj[u] - k) * R + k;
 #pragma unroll

@@ -1427,24 +1427,29 @@ __device__ __forceinline__ int slot_parent(const LevelJob &J, int i) {
     return J.plist ? J.plist[i] : i;
 }
 
+// 0 and 1 that a loader reads in place of a coefficient past a row's end,
+// of a missing row or of the identity polynomial.  Such loaders choose an
+// address and load unconditionally: with the loads under per-lane branches
+// and the loaded value merged with a constant afterwards, the compiler
+// waited for all earlier loads at every merge, one memory latency per point.
+// (Not const: a const table would sit in the constant address space and
+// turn the selected loads into flat loads.)
+__device__ double kZeroOne[2] = {0.0, 1.0};
+
 // Child pair of parent p packed as L + i R (q-plane qi); a missing right
 // child is the identity polynomial (or 0 with r_missing_zero); p < 0 gives
 // zeros.
 __device__ __forceinline__ double2 load_child_pair(const LevelJob &J, int qi, int p, int m) {
-    double2 z = make_double2(0.0, 0.0);
-    if (p >= 0 && p < J.nr_parent) {
-        const int cc = J.child.count;
-        const bool has_r = (2 * p + 1) < J.nr_child;
-        if (m < cc) {
-            z.x = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m];
-            if (has_r) {
-                const CoefView &R = J.child_r.base ? J.child_r : J.child;
-                z.y = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m];
-            }
-        }
-        if (!has_r && m == 0 && !J.r_missing_zero) z.y = 1.0;
-    }
-    return z;
+    const bool ok = p >= 0 && p < J.nr_parent;
+    const bool in = ok && m < J.child.count;
+    const bool has_r = (2 * p + 1) < J.nr_child;
+    const CoefView &R = J.child_r.base ? J.child_r : J.child;
+    const double *lx = J.child.base + (long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m;
+    const double *ry = R.base + (long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m;
+    const bool one = ok && !has_r && m == 0 && !J.r_missing_zero;
+    const double *px = in ? lx : kZeroOne;
+    const double *py = (in && has_r) ? ry : kZeroOne + (one ? 1 : 0);
+    return make_double2(*px, *py);
 }
 
 // Exact product term L[m] R[m] of parent p (m = 0 or m = c-1).
@@ -1631,20 +1636,37 @@ __global__ __launch_bounds__(lds_build_threads<F>(), lds_build_threads<F>() == 2
 // memory and the inverses store straight to the children's g (two LDS round
 // trips fewer per transform); with per-pass LDS addresses this fits 128
 // VGPRs at 512 threads (two workgroups per CU) without spills.
-template <int F, class Map, bool OPQ = false>
+//
+// Slot parents: without a parent list (PL = false) slot i is parent i, found
+// by arithmetic; with one (PL = true) the launch fills spar[2 item + w] with
+// the parent of the item's slot w before the body runs, so the per-point
+// loaders read LDS: a list entry loaded from global memory inside a loader
+// made every point wait for all earlier loads (in-order vector memory
+// counters), one memory latency per point.
+template <int F, class Map, bool OPQ = false, bool PL = false>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
-                                              int twShift) {
+                                              int twShift, const int *spar = nullptr) {
     using TL = SwzLayout<F>;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
     const int t = body_tid<OPQ>();
     const int gs = J.gpar.stride, gcnt = J.gpar.count;
+    auto parent = [&](int item, int pr, int w) -> int {
+        if constexpr (PL) {
+            (void)pr;
+            return spar[2 * item + w];
+        } else {
+            (void)item;
+            const int i = 2 * pr + w;
+            return i < J.np ? i : -1;
+        }
+    };
 
     auto ldg = [&](int item, int m) {
         int qi, pr;
         M.at(item, qi, pr);
         const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
-        const int pa = slot_parent(J, 2 * pr), pb = slot_parent(J, 2 * pr + 1);
+        const int pa = parent(item, pr, 0), pb = parent(item, pr, 1);
         double2 z = make_double2(0.0, 0.0);
         if (m < gcnt) {
             if (pa >= 0) z.x = gb[(long long)pa * gs + m];
@@ -1674,7 +1696,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         auto ldc = [&](int item, int m) {
             int qi, pr;
             M.at(item, qi, pr);
-            return load_child_pair(J, qi, slot_parent(J, 2 * pr + which), m);
+            return load_child_pair(J, qi, parent(item, pr, which), m);
         };
         fft_tile_io<F, false, kLT, TL>(lds, tw, ldc, NoIO{});
 #pragma unroll
@@ -1697,7 +1719,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         auto st = [&](int item, int m, double2 v) {
             int qi, pr;
             M.at(item, qi, pr);
-            const int p = slot_parent(J, 2 * pr + which);
+            const int p = parent(item, pr, which);
             if (p >= 0 && m < gcc) {
                 double *gcb = J.gchild.base + (long long)qi * J.gchild.plane;
                 double vl = v.x * invF, vr = v.y * invF;
@@ -1713,10 +1735,21 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
     }
 }
 
-template <int F>
+template <int F, bool PL>
 __global__ __launch_bounds__(kLT, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
+    constexpr int ITEMS = kTile / F;
     __shared__ double2 lds[kTile];
-    prop_lds_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+    __shared__ int spar[PL ? 2 * ITEMS : 1];
+    const PlainMap M{(int)blockIdx.x * ITEMS, (int)blockIdx.y};
+    if constexpr (PL) {
+        for (int e = threadIdx.x; e < 2 * ITEMS; e += kLT) {
+            int qi, pr;
+            M.at(e >> 1, qi, pr);
+            spar[e] = slot_parent(J, 2 * pr + (e & 1));
+        }
+        __syncthreads();
+    }
+    prop_lds_body<F, PlainMap, false, PL>(J, lds, M, tw, twShift, spar);
 }
 
 // ---------------------------------------------------------------------------
@@ -2600,13 +2633,12 @@ constexpr int kF2 = kTile;
 struct RowPair {
     const double *x, *y;
     int cx, cy;
-    double one;
+    double one;   // 1: the y row is the identity polynomial (only without a y row)
+    // branch-free, as load_child_pair
     __device__ __forceinline__ double2 at(int m) const {
-        double2 z = make_double2(0.0, 0.0);
-        if (m < cx) z.x = x[m];
-        if (m < cy) z.y = y[m];
-        if (m == 0 && one != 0.0) z.y = one;   // only without a y row
-        return z;
+        const double *px = m < cx ? x + m : kZeroOne;
+        const double *py = m < cy ? y + m : kZeroOne + ((m == 0 && one != 0.0) ? 1 : 0);
+        return make_double2(*px, *py);
     }
 };
 // Child pair of slot a: left child (x) and right child (y) of its parent, as
@@ -3693,7 +3725,8 @@ static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, 
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.np + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    k_prop_lds<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    if (J.plist) k_prop_lds<F, true><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    else k_prop_lds<F, false><<<grid, kLT, 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
