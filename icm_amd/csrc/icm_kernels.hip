@@ -2342,18 +2342,16 @@ __global__ __launch_bounds__(kLT, 4) void k_batch_persist(PersistArgs A) {
 // dP = 0.  Coefficient fixes as the forward kernels: exact 0 / top terms in
 // the build, the one wrapped B coefficient in the propagate.
 // ---------------------------------------------------------------------------
+// Branch-free, as load_child_pair.
 __device__ __forceinline__ double2 load_view_pair(const CoefView &V, int qi, int p, int m, int nr_child,
                                                   bool missing_is_one) {
-    double2 z = make_double2(0.0, 0.0);
-    if (p < 0) return z;
     const bool has_r = (2 * p + 1) < nr_child;
-    if (m < V.count) {
-        const double *b = V.base + (long long)qi * V.plane + (long long)(2 * p) * V.stride;
-        z.x = b[m];
-        if (has_r) z.y = b[V.stride + m];
-    }
-    if (!has_r && m == 0 && missing_is_one) z.y = 1.0;
-    return z;
+    const bool in = p >= 0 && m < V.count;
+    const bool one = p >= 0 && !has_r && m == 0 && missing_is_one;
+    const double *b = V.base + (long long)qi * V.plane + (long long)(2 * p) * V.stride + m;
+    const double *px = in ? b : kZeroOne;
+    const double *py = (in && has_r) ? b + V.stride : kZeroOne + (one ? 1 : 0);
+    return make_double2(*px, *py);
 }
 
 // Item maps as the forward kernels, with one parent per item ("pair" = parent).
@@ -3152,6 +3150,7 @@ using RowBuild = RowR16<true>;
 using RowProp = RowR16<true>;
 using RowBuildT = RowR16<false>;
 using RowPropFused = RowR16<false>;   // F = 8192 propagate: its loaders hold more state (33 spills with KEEP)
+using RowBuildFused = RowR16<false>;  // F = 8192 build, likewise
 using RowPropB = RowR16<false>;       // adjoint B propagate: four g spectra live
 
 template <class TL> __host__ __device__ __forceinline__ int row_at(int k2) { return TL::at(0, k2); }
@@ -3257,6 +3256,22 @@ struct RowDst {
 // inverse rows V0, V1 give x[n2] = V0 + V1 w_F^-n2 and x[n2 + 4096] =
 // V0 - V1 w_F^-n2 for the coefficient storer (Col2Dst, which scales by 1/F).
 // No column kernels and no column arrays for such a level.
+// exp(-2 pi i j / 32) for a constant j (0 <= j < 32), correctly rounded.
+__device__ __forceinline__ double2 w32(int j) {
+    constexpr double c[32] = {
+        1.0, 0.9807852804032304, 0.9238795325112867, 0.8314696123025452, 0.7071067811865476, 0.5555702330196022,
+        0.3826834323650898, 0.19509032201612828, 0.0, -0.19509032201612828, -0.3826834323650898, -0.5555702330196022,
+        -0.7071067811865476, -0.8314696123025452, -0.9238795325112867, -0.9807852804032304, -1.0, -0.9807852804032304,
+        -0.9238795325112867, -0.8314696123025452, -0.7071067811865476, -0.5555702330196022, -0.3826834323650898,
+        -0.19509032201612828, 0.0, 0.19509032201612828, 0.3826834323650898, 0.5555702330196022, 0.7071067811865476,
+        0.8314696123025452, 0.9238795325112867, 0.9807852804032304};
+    return make_double2(c[j & 31], -c[(j + 24) & 31]);   // sin(2 pi j / 32) = cos(2 pi (j - 8) / 32)
+}
+// The loader's points n2 = c + 256 s (s < 16) of one thread share c, so
+// w_8192^n2 = w_8192^c w_32^s: one table twiddle per thread and a constant
+// per point.  No table loads per point and no branch: a thread's loads then
+// issue back to back (per-point twiddle loads under the tile's branch made
+// every point wait for the previous one's loads).
 template <class Rows> struct Col2Src {
     Rows src;
     int twShift;
@@ -3267,24 +3282,29 @@ template <class Rows> struct Col2Src {
             T, tw,
             [&](int h, int n2) {
                 const double2 x0 = r.at(n2), x1 = r.at(n2 + kF2);
-                return h ? c_mul(c_sub(x0, x1), tw_big(tw, twShift, n2, 13)) : c_add(x0, x1);
+                const double2 w = c_mul(tw_big2(tw, twShift, n2 & 255, 13), w32(n2 >> 8));
+                const double2 d = c_mul(c_sub(x0, x1), w), sm = c_add(x0, x1);
+                return h ? d : sm;
             },
             NoIO{});
     }
 };
+// The storer's n2 = t + 512 u: w_8192^n2 = w_8192^t w_32^(2u).
 template <class St> struct Col2Dst {
     St st;
     int twShift;
     template <class RS>
     __device__ __forceinline__ void ifft_store(double2 *T, int a, int qi, int, int, const double2 *tw) const {
         using TL = typename RS::TL;
+        static_assert(kF2 / RS::NTH <= 16, "w_32 steps");
         fft_tile_io<kF2, true, RS::NT, TL, RS::PTS, RS::KEEP>(T, tw, NoIO{}, NoIO{});
         const OutRows o = st.rows(a, qi);
+        const double2 wt = tw_big2(tw, twShift, (int)threadIdx.x, 13);
 #pragma unroll
         for (int u = 0; u < kF2 / RS::NTH; ++u) {
             const int n2 = (int)threadIdx.x + u * RS::NTH;
             const double2 v0 = T[TL::at(0, n2)];
-            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], tw_big(tw, twShift, n2, 13));
+            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], c_mul(wt, w32(2 * u)));
             st.store(o, qi, n2, c_add(v0, v1));
             st.store(o, qi, n2 + kF2, c_sub(v0, v1));
         }
@@ -3326,6 +3346,12 @@ struct Col2DstNext {
         const double invF = 1.0 / (double)F;
         double *pb = J.parent.base + (long long)qi * J.parent.plane;
         double2 *o = out + ((long long)qi * nArr + a) * (2 * F);
+        // twiddles of this level's storer (w_8192^t) and of the level
+        // above's column transform (w_16384^(t r)), per thread
+        const int t = (int)threadIdx.x;
+        const double2 wt = tw_big2(tw, twShift, t, 13);
+        const double2 wr1 = tw_big2(tw, twShift, t, 14), wr2 = tw_big2(tw, twShift, 2 * t, 14),
+                      wr3 = tw_big2(tw, twShift, 3 * t, 14);
         double2 top = make_double2(0.0, 0.0);
         if (pc > F) {
             const int tc = J.child.count - 1;
@@ -3336,7 +3362,7 @@ struct Col2DstNext {
         for (int u = 0; u < kF2 / RS::NTH; ++u) {
             const int n2 = (int)threadIdx.x + u * RS::NTH;
             const double2 v0 = T[TL::at(0, n2)];
-            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], tw_big(tw, twShift, n2, 13));
+            const double2 v1 = c_mulc(T[TL::IS + TL::at(0, n2)], c_mul(wt, w32(2 * u)));
             double2 a0 = coef(n2, c_add(v0, v1), hx, hy, qi, p, invF);
             double2 a1 = coef(n2 + kF2, c_sub(v0, v1), hx, hy, qi, p, invF);
             double2 a2 = n2 == 0 ? top : make_double2(0.0, 0.0);
@@ -3356,10 +3382,11 @@ struct Col2DstNext {
                 if (hy) pb[(long long)(p + 1) * J.parent.stride + F] = top.y;
             }
             dft4<false>(a0, a1, a2, a3);
-            const double2 X[4] = {a0, a1, a2, a3};
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                o[r * kF2 + n2] = c_mul(X[r], tw_big(tw, twShift, (n2 * r) & (2 * F - 1), 14));
+            // w_16384^(n2 r) = w_16384^(t r) w_32^(u r)
+            o[n2] = a0;
+            o[kF2 + n2] = c_mul(a1, c_mul(wr1, w32(u)));
+            o[2 * kF2 + n2] = c_mul(a2, c_mul(wr2, w32(2 * u)));
+            o[3 * kF2 + n2] = c_mul(a3, c_mul(wr3, w32(3 * u)));
         }
         __syncthreads();   // the tiles are free for writing on return
     }
@@ -3884,8 +3911,8 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
     if (col_fused(J.F) && next) {   // column stage inside the row kernel, the level above's columns after it
         PF_BEGIN();
-        k_rowpair_build<RowBuild, Col2Src<ChildRows>, Col2DstNext>
-            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
+        k_rowpair_build<RowBuildFused, Col2Src<ChildRows>, Col2DstNext>
+            <<<dim3(1, npairs, J.nq), RowBuildFused::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
                                                             Col2DstNext{J, next_spec, npairs, twShift}, nrp,
                                                             npairs, J.logF, tw, twShift);
         ICM_TRY(hipGetLastError());
@@ -3896,8 +3923,8 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
     }
     if (col_fused(J.F)) {   // column stage inside the row kernel; nothing for the spectrum cache
         PF_BEGIN();
-        k_rowpair_build<RowBuild, Col2Src<ChildRows>, Col2Dst<StParentPair>>
-            <<<dim3(1, npairs, J.nq), RowBuild::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
+        k_rowpair_build<RowBuildFused, Col2Src<ChildRows>, Col2Dst<StParentPair>>
+            <<<dim3(1, npairs, J.nq), RowBuildFused::NTH, 0, s>>>(Col2Src<ChildRows>{ChildRows{J}, twShift},
                                                             Col2Dst<StParentPair>{StParentPair{J}, twShift}, nrp,
                                                             npairs, J.logF, tw, twShift);
         ICM_TRY(hipGetLastError());
