@@ -1502,15 +1502,17 @@ __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int 
 // ---------------------------------------------------------------------------
 // LDS-level kernels: LT threads per 4096-point tile, 4096 / LT points per
 // thread: 512 (radix-8 passes) or 256 (radix-16 passes, two workgroups and
-// 256 VGPRs per CU).  The plain build takes 256 threads at F = 128 and 256,
-// where that measured faster (1.1-1.2x); every propagate, the build at
-// F >= 512, the fused adjoint kernels (which spill at 256 VGPRs) and the
-// ragged-batch kernels (which switch over F) run 512 threads.
+// 256 VGPRs per CU).  The plain build and propagate take 256 threads where
+// radix 16 needs fewer passes than radix 8 (F = 128, 256, 1024, 2048, 4096:
+// one LDS round trip fewer per transform; measured 4-9% faster at
+// n = 2^20), 512 at F = 512 (three passes either way; radix 16 measured 6%
+// slower) and below 128; the fused adjoint kernels (which spill at 256
+// VGPRs) and the ragged-batch kernels (which switch over F) run 512 threads.
 constexpr int kLT = 512;
 constexpr int kLP = kTile / kLT;
 template <int F, int LT> struct LdsTL { using type = SwzLayout<F>; };
 template <int F> struct LdsTL<F, 256> { using type = Swz16Layout<F>; };
-template <int F> constexpr int lds_build_threads() { return (F == 128 || F == 256) ? 256 : 512; }
+template <int F> constexpr int lds_build_threads() { return (F >= 128 && F != 512) ? 256 : 512; }
 // Largest padded tile (F = 16): LDS of the kernels that switch over F.
 constexpr int kTileMaxSize = SwzLayout<16>::SIZE;
 
@@ -1643,10 +1645,11 @@ __global__ __launch_bounds__(lds_build_threads<F>(), lds_build_threads<F>() == 2
 // loaders read LDS: a list entry loaded from global memory inside a loader
 // made every point wait for all earlier loads (in-order vector memory
 // counters), one memory latency per point.
-template <int F, class Map, bool OPQ = false, bool PL = false>
+template <int F, class Map, bool OPQ = false, bool PL = false, int LT = kLT>
 __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                               int twShift, const int *spar = nullptr) {
-    using TL = SwzLayout<F>;
+    using TL = typename LdsTL<F, LT>::type;
+    constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
     const int t = body_tid<OPQ>();
@@ -1674,11 +1677,11 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
         }
         return z;
     };
-    fft_tile_io<F, false, kLT, TL>(lds, tw, ldg, NoIO{});
-    double2 G1[kLP / 2], G2[kLP / 2];
+    fft_tile_io<F, false, LT, TL>(lds, tw, ldg, NoIO{});
+    double2 G1[LP / 2], G2[LP / 2];
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) {
@@ -1698,10 +1701,10 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
             M.at(item, qi, pr);
             return load_child_pair(J, qi, parent(item, pr, which), m);
         };
-        fft_tile_io<F, false, kLT, TL>(lds, tw, ldc, NoIO{});
+        fft_tile_io<F, false, LT, TL>(lds, tw, ldc, NoIO{});
 #pragma unroll
-        for (int u = 0; u < kLP / 2; ++u) {
-            const int ps = t + u * kLT;
+        for (int u = 0; u < LP / 2; ++u) {
+            const int ps = t + u * LT;
             const int item = ps >> LOGH, jb = ps & (HALF - 1);
             const double2 g = which == 0 ? G1[u] : G2[u];
             const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
@@ -1731,25 +1734,27 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
                 if (2 * p + 1 < J.nr_child) put(&gcb[(long long)(2 * p + 1) * gcs + m], vr, J.acc_out);
             }
         };
-        fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, st);
+        fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, st);
     }
 }
 
+template <int F> constexpr int lds_prop_threads() { return lds_build_threads<F>(); }
 template <int F, bool PL>
-__global__ __launch_bounds__(kLT, 4) void k_prop_lds(LevelJob J, const double2 *tw, int twShift) {
-    constexpr int ITEMS = kTile / F;
+__global__ __launch_bounds__(lds_prop_threads<F>(), lds_prop_threads<F>() == 256 ? 2 : 4) void k_prop_lds(
+    LevelJob J, const double2 *tw, int twShift) {
+    constexpr int ITEMS = kTile / F, LT = lds_prop_threads<F>();
     __shared__ double2 lds[kTile];
     __shared__ int spar[PL ? 2 * ITEMS : 1];
     const PlainMap M{(int)blockIdx.x * ITEMS, (int)blockIdx.y};
     if constexpr (PL) {
-        for (int e = threadIdx.x; e < 2 * ITEMS; e += kLT) {
+        for (int e = threadIdx.x; e < 2 * ITEMS; e += LT) {
             int qi, pr;
             M.at(e >> 1, qi, pr);
             spar[e] = slot_parent(J, 2 * pr + (e & 1));
         }
         __syncthreads();
     }
-    prop_lds_body<F, PlainMap, false, PL>(J, lds, M, tw, twShift, spar);
+    prop_lds_body<F, PlainMap, false, PL, LT>(J, lds, M, tw, twShift, spar);
 }
 
 // ---------------------------------------------------------------------------
@@ -3752,8 +3757,8 @@ static hipError_t prop_lds_f(const LevelJob &J, const double2 *tw, int twShift, 
     constexpr int ITEMS = kTile / F;
     const int npairs = (J.np + 1) / 2;
     dim3 grid((npairs + ITEMS - 1) / ITEMS, J.nq);
-    if (J.plist) k_prop_lds<F, true><<<grid, kLT, 0, s>>>(J, tw, twShift);
-    else k_prop_lds<F, false><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    if (J.plist) k_prop_lds<F, true><<<grid, lds_prop_threads<F>(), 0, s>>>(J, tw, twShift);
+    else k_prop_lds<F, false><<<grid, lds_prop_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
