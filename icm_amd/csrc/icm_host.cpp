@@ -901,18 +901,45 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         if (!hip_ok(icm::launch_leaf_build_vjp(V, s, &p->prof), "leaf_build_vjp")) return -1;
         // P_X and dP_X of level l (recompute: dP_X only; P is kept for every
         // level, and a fused LDS level rewrites it with the same values)
-        auto build_level = [&](int l, bool recompute) -> bool {
+        auto pjob = [&](int l) {   // P_X of level l from its children's P
             icm::LevelJob J{};
             J.nr_child = g.n_real[l - 1]; J.nr_parent = g.n_real[l]; J.np = g.n_real[l];
             J.F = g.F[l]; J.logF = ilog2i(g.F[l]); J.nq = nq;
             J.child = view(l - 1, false);
             J.parent = icm::CoefOut{p->d_coef[l], (long long)g.n_real[l] * g.c[l], g.c[l], g.c[l]};
+            return J;
+        };
+        // Up-sweep column fusion: a four-step P build hands the level above
+        // its child-pair column transforms (the spectrum cache entry) when
+        // every later reader of this level's P takes those spectra, i.e. the
+        // level above keeps both spectrum caches (its dP build and its A and
+        // B propagates then read spectra, and the exact terms read the end
+        // coefficients, which the fused kernels store).  p_ready: the level
+        // below handed this level's entry over.
+        bool p_ready = false;
+        auto build_level = [&](int l, bool recompute) -> bool {
+            icm::LevelJob J = pjob(l);
             if (J.F <= icm::kMaxLdsF) {                           // P_X and dP_X in one pass
                 J.tchild = view(l - 1, true);
                 J.tparent = icm::CoefOut{dcoef_of(p, l), J.parent.plane, J.parent.stride, J.parent.count};
+                p_ready = false;
                 return hip_ok(icm::launch_build_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp build level");
             }
-            if (!recompute && !level(J, true, spec_of(l))) return false;   // P_X = P_L P_R
+            if (!recompute) {                                     // P_X = P_L P_R
+                icm::LevelJob N{};
+                bool fuse = false;
+                if (l + 1 <= L - 2 && spec_of(l + 1) && dspec_of(l + 1) &&
+                    (p->fuse_columns & (icm::col_fused(J.F) ? 8 : 2))) {
+                    N = pjob(l + 1);
+                    fuse = icm::build_fusable(J, N);
+                }
+                if (!hip_ok(icm::launch_build_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
+                                                    spec_of(l), p_ready && spec_of(l), fuse ? &N : nullptr,
+                                                    fuse ? spec_of(l + 1) : nullptr),
+                            "vjp build level"))
+                    return false;
+                p_ready = fuse;
+            }
             J.parent.base = dcoef_of(p, l);
             if (dspec_of(l)) {                                    // dP_X from both pairs' spectra
                 J.tchild = view(l - 1, true);

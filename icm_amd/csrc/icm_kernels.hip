@@ -2360,17 +2360,19 @@ __device__ __forceinline__ double2 load_view_pair(const CoefView &V, int qi, int
 }
 
 // Item maps as the forward kernels, with one parent per item ("pair" = parent).
-template <int F, class Map>
+// LT threads per tile as the forward LDS kernels (512: radix 8, 256: radix 16).
+template <int F, class Map, int LT = kLT>
 __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                  int twShift) {
-    using TL = SwzLayout<F>;
+    using TL = typename LdsTL<F, LT>::type;
+    constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
     (void)twShift;
     // forward transforms take their first pass straight from global memory,
     // the inverse stores its last straight to P_X and dP_X
-    fft_tile_io<F, false, kLT, TL>(
+    fft_tile_io<F, false, LT, TL>(
         lds, tw,
         [&](int item, int m) {
             int qi, pr;
@@ -2378,17 +2380,17 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
             return load_view_pair(J.child, qi, (pr < J.nr_parent) ? pr : -1, m, J.nr_child, true);
         },
         NoIO{});
-    double2 XL[kLP / 2], XR[kLP / 2];
+    double2 XL[LP / 2], XR[LP / 2];
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) { XL[u] = make_double2(za.x, zb.x); XR[u] = make_double2(za.y, zb.y); }
         else { XL[u] = split_x(za, zb); XR[u] = split_y(za, zb); }
     }
     __syncthreads();
-    fft_tile_io<F, false, kLT, TL>(
+    fft_tile_io<F, false, LT, TL>(
         lds, tw,
         [&](int item, int m) {
             int qi, pr;
@@ -2397,8 +2399,8 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
         },
         NoIO{});
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
         const double2 za = lds[ia], zb = lds[ib];
@@ -2415,7 +2417,7 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
     }
     const double invF = 1.0 / (double)F;
     const int pc = J.parent.count, cc = J.child.count;
-    fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
+    fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
         int qi, p;
         M.at(item, qi, p);
         if (p >= J.nr_parent) return;
@@ -2449,17 +2451,27 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
     });
 }
 
-template <int F, class Map>
+// Propagate order: (1) A + iB of the parent, whose spectra GA, GB stay in
+// registers; (2) P_L + iP_R: the A outputs' spectra go to the tile and their
+// inverse runs at once, while the B outputs keep their first terms
+// B conj(P_R), B conj(P_L); (3) dP_L + i dP_R adds A conj(dP_R), A conj(dP_L)
+// and the B inverse runs.  Three spectra per representative stay live (GA
+// and the two B partial sums) instead of six, which lets the kernel run
+// radix-16 passes at 256 threads without spills.  Each B coefficient is the
+// same two products summed in the same order as computing both at once.
+template <int F, class Map, int LT = kLT>
 __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds, const Map &M, const double2 *tw,
                                                 int twShift) {
-    using TL = SwzLayout<F>;
+    using TL = typename LdsTL<F, LT>::type;
+    constexpr int LP = kTile / LT;
     constexpr int HALF = F / 2;
     constexpr int LOGH = ilog2c(HALF);
     const int t = threadIdx.x;
+    (void)twShift;
     // (1) A + iB of the parent (every forward transform takes its first pass
     // straight from global memory, every inverse stores its last straight to
     // the outputs)
-    fft_tile_io<F, false, kLT, TL>(
+    fft_tile_io<F, false, LT, TL>(
         lds, tw,
         [&](int item, int m) {
             int qi, p;
@@ -2472,18 +2484,20 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
             return z;
         },
         NoIO{});
-    double2 GA[kLP / 2], GB[kLP / 2];
+    double2 GA[LP / 2], GB[LP / 2];
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
         if (jb == 0) { GA[u] = make_double2(za.x, zb.x); GB[u] = make_double2(za.y, zb.y); }
         else { GA[u] = split_x(za, zb); GB[u] = split_y(za, zb); }
     }
     __syncthreads();
-    // (2) P_L + iP_R
-    fft_tile_io<F, false, kLT, TL>(
+    // (2) P_L + iP_R: A outputs to the tile, first terms of the B outputs
+    // (bins 0 and HALF, real: BL1 = (B conj(R), B conj(L)) at bin 0 and BR1
+    // at bin HALF; other bins: BL1 = B conj(R), BR1 = B conj(L))
+    fft_tile_io<F, false, LT, TL>(
         lds, tw,
         [&](int item, int m) {
             int qi, pr;
@@ -2491,52 +2505,31 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
             return load_view_pair(J.child, qi, (pr < J.nr_parent) ? pr : -1, m, J.nr_child, true);
         },
         NoIO{});
-    double2 XL[kLP / 2], XR[kLP / 2];
+    double2 BL1[LP / 2], BR1[LP / 2];
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
-        const int item = ps >> LOGH, jb = ps & (HALF - 1);
-        const double2 za = lds[TL::at(item, jb)], zb = lds[TL::at(item, jb == 0 ? HALF : F - jb)];
-        if (jb == 0) { XL[u] = make_double2(za.x, zb.x); XR[u] = make_double2(za.y, zb.y); }
-        else { XL[u] = split_x(za, zb); XR[u] = split_y(za, zb); }
-    }
-    __syncthreads();
-    // (3) dP_L + i dP_R; spectra of the outputs
-    fft_tile_io<F, false, kLT, TL>(
-        lds, tw,
-        [&](int item, int m) {
-            int qi, pr;
-            M.at(item, qi, pr);
-            return load_view_pair(J.tchild, qi, (pr < J.nr_parent) ? pr : -1, m, J.nr_child, false);
-        },
-        NoIO{});
-    double2 U2a[kLP / 2], U2b[kLP / 2];   // B_L + i B_R at bins jb and F - jb
-#pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
         const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
         const double2 za = lds[ia], zb = lds[ib];
         if (jb == 0) {
             // bins 0 and HALF carry real values (x: bin 0, y: bin HALF)
-            const double dl0 = za.x, dr0 = za.y, dlh = zb.x, drh = zb.y;
-            lds[ia] = make_double2(GA[u].x * XR[u].x, GA[u].x * XL[u].x);
-            lds[ib] = make_double2(GA[u].y * XR[u].y, GA[u].y * XL[u].y);
-            U2a[u] = make_double2(GB[u].x * XR[u].x + GA[u].x * dr0, GB[u].x * XL[u].x + GA[u].x * dl0);
-            U2b[u] = make_double2(GB[u].y * XR[u].y + GA[u].y * drh, GB[u].y * XL[u].y + GA[u].y * dlh);
+            const double2 XL = make_double2(za.x, zb.x), XR = make_double2(za.y, zb.y);
+            lds[ia] = make_double2(GA[u].x * XR.x, GA[u].x * XL.x);
+            lds[ib] = make_double2(GA[u].y * XR.y, GA[u].y * XL.y);
+            BL1[u] = make_double2(GB[u].x * XR.x, GB[u].x * XL.x);
+            BR1[u] = make_double2(GB[u].y * XR.y, GB[u].y * XL.y);
         } else {
-            const double2 DL = split_x(za, zb), DR = split_y(za, zb);
-            const double2 AL = c_mulc(GA[u], XR[u]), AR = c_mulc(GA[u], XL[u]);
-            const double2 BL = c_add(c_mulc(GB[u], XR[u]), c_mulc(GA[u], DR));
-            const double2 BR = c_add(c_mulc(GB[u], XL[u]), c_mulc(GA[u], DL));
+            const double2 XL = split_x(za, zb), XR = split_y(za, zb);
+            const double2 AL = c_mulc(GA[u], XR), AR = c_mulc(GA[u], XL);
             lds[ia] = pack_ai(AL, AR);
             lds[ib] = pack_ai(c_conj(AL), c_conj(AR));
-            U2a[u] = pack_ai(BL, BR);
-            U2b[u] = pack_ai(c_conj(BL), c_conj(BR));
+            BL1[u] = c_mulc(GB[u], XR);
+            BR1[u] = c_mulc(GB[u], XL);
         }
     }
     const double invF = 1.0 / (double)F;
-    fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
+    fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
         int qi, p;
         M.at(item, qi, p);
         if (p < J.nr_parent && m < J.gchild.count) {
@@ -2545,15 +2538,35 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
             if (2 * p + 1 < J.nr_child) g[J.gchild.stride + m] = v.y * invF;
         }
     });
+    // (3) dP_L + i dP_R: second terms of the B outputs, their inverse
+    fft_tile_io<F, false, LT, TL>(
+        lds, tw,
+        [&](int item, int m) {
+            int qi, pr;
+            M.at(item, qi, pr);
+            return load_view_pair(J.tchild, qi, (pr < J.nr_parent) ? pr : -1, m, J.nr_child, false);
+        },
+        NoIO{});
 #pragma unroll
-    for (int u = 0; u < kLP / 2; ++u) {
-        const int ps = t + u * kLT;
+    for (int u = 0; u < LP / 2; ++u) {
+        const int ps = t + u * LT;
         const int item = ps >> LOGH, jb = ps & (HALF - 1);
-        lds[TL::at(item, jb)] = U2a[u];
-        lds[TL::at(item, jb == 0 ? HALF : F - jb)] = U2b[u];
+        const int ia = TL::at(item, jb), ib = TL::at(item, jb == 0 ? HALF : F - jb);
+        const double2 za = lds[ia], zb = lds[ib];
+        if (jb == 0) {
+            const double dl0 = za.x, dr0 = za.y, dlh = zb.x, drh = zb.y;
+            lds[ia] = make_double2(BL1[u].x + GA[u].x * dr0, BL1[u].y + GA[u].x * dl0);
+            lds[ib] = make_double2(BR1[u].x + GA[u].y * drh, BR1[u].y + GA[u].y * dlh);
+        } else {
+            const double2 DL = split_x(za, zb), DR = split_y(za, zb);
+            const double2 BL = c_add(BL1[u], c_mulc(GA[u], DR));
+            const double2 BR = c_add(BR1[u], c_mulc(GA[u], DL));
+            lds[ia] = pack_ai(BL, BR);
+            lds[ib] = pack_ai(c_conj(BL), c_conj(BR));
+        }
     }
     const int cc = J.child.count;
-    fft_tile_io<F, true, kLT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
+    fft_tile_io<F, true, LT, TL>(lds, tw, NoIO{}, [&](int item, int m, double2 v) {
         int qi, p;
         M.at(item, qi, p);
         if (p < J.nr_parent && m < J.bchild.count) {
@@ -2579,18 +2592,23 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
 }
 
 
+// Threads per tile as the forward LDS kernels; two workgroups per CU.
+template <int F> constexpr int lds_t_threads() { return lds_build_threads<F>(); }
 template <int F>
-__global__ __launch_bounds__(kLT) void k_build_lds_t(LevelJob J, const double2 *tw, int twShift) {
+__global__ __launch_bounds__(lds_t_threads<F>(), lds_t_threads<F>() == 256 ? 2 : 4) void k_build_lds_t(
+    LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[kTile];
-    build_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+    build_lds_t_body<F, PlainMap, lds_t_threads<F>()>(
+        J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
-// Capped at 128 VGPRs: two workgroups per CU with ~20 spilled registers
-// measured faster (34 against 42 ms at n = 2^20) than one with none.
+template <int F> constexpr int lds_prop_t_threads() { return lds_build_threads<F>(); }
 template <int F>
-__global__ __launch_bounds__(kLT, 4) void k_prop_lds_t(LevelJob J, const double2 *tw, int twShift) {
+__global__ __launch_bounds__(lds_prop_t_threads<F>(), lds_prop_t_threads<F>() == 256 ? 2 : 4) void k_prop_lds_t(
+    LevelJob J, const double2 *tw, int twShift) {
     __shared__ double2 lds[kTile];
-    prop_lds_t_body<F>(J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
+    prop_lds_t_body<F, PlainMap, lds_prop_t_threads<F>()>(
+        J, lds, PlainMap{(int)blockIdx.x * (kTile / F), (int)blockIdx.y}, tw, twShift);
 }
 
 // Ragged batches, adjoint levels: work item = (job, first parent item).
@@ -3788,14 +3806,14 @@ template <int F>
 static hipError_t build_lds_t_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
     constexpr int ITEMS = kTile / F;
     dim3 grid((J.nr_parent + ITEMS - 1) / ITEMS, J.nq);
-    k_build_lds_t<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    k_build_lds_t<F><<<grid, lds_t_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 template <int F>
 static hipError_t prop_lds_t_f(const LevelJob &J, const double2 *tw, int twShift, hipStream_t s) {
     constexpr int ITEMS = kTile / F;
     dim3 grid((J.nr_parent + ITEMS - 1) / ITEMS, J.nq);
-    k_prop_lds_t<F><<<grid, kLT, 0, s>>>(J, tw, twShift);
+    k_prop_lds_t<F><<<grid, lds_prop_t_threads<F>(), 0, s>>>(J, tw, twShift);
     return hipGetLastError();
 }
 
