@@ -966,6 +966,17 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         for (int l = L - 2; m >= 2 && l >= 1; --l)
             if (!ckpt_keeps(m, l)) { resident = l / m; break; }
         int ca = 0, cb = 0;  // current A / B buffers (A in d_g, B in d_bv)
+        // Down-sweep column fusion: a four-step level whose B propagate takes
+        // the one-pass form (both spectrum caches) runs it before its A
+        // propagate, whose column inverse may then write the level below's A
+        // g-pair column transforms straight into bufA (where this level's
+        // were) when the level below takes the same form; that level's A
+        // coefficients are then never stored except coefficient 0, which its
+        // B propagate's wrapped term reads.  a_ready: bufA holds this level's.
+        bool a_ready = false;
+        auto one_pass_b = [&](int l) {
+            return l < L - 1 && g.F[l] > icm::kMaxLdsF && !icm::col_fused(g.F[l]) && dspec_of(l) != nullptr;
+        };
         for (int l = L - 1; l >= 1; --l) {
             const bool root = (l == L - 1);
             // this level reads the children's tangent products as coefficients
@@ -1000,25 +1011,46 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 cb = nbv;
                 continue;
             }
-            if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
             icm::LevelJob K = J;                                  // B_L = A_X * dR ...
             K.child = view(l - 1, true);
             K.r_missing_zero = 1;
             K.gchild = Bout;
             K.gfix = gfix;
-            if (!root && dspec_of(l)) {                           // ... + B_X * R in one pass
+            if (one_pass_b(l)) {                                  // ... + B_X * R in one pass, then A
                 icm::LevelJob KB = J;
                 KB.gpar = Bpar;
                 KB.gchild = Bout;
                 KB.gfix = gfix;
+                if (!a_ready && !hip_ok(icm::launch_g_pairs(J, p->d_tw, p->twShift, p->d_bufA, s, &p->prof),
+                                        "vjp propagate level"))
+                    return -1;
                 if (!hip_ok(icm::launch_prop_large_b(K, KB, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, spec_of(l),
                                                      dspec_of(l), s, &p->prof),
                             "vjp propagate level"))
                     return -1;
+                icm::LevelJob N{};                                // the level below's A propagate
+                bool fuse = false;
+                if (l - 1 >= 1 && one_pass_b(l - 1) && (p->fuse_columns & 1)) {
+                    N.nr_child = g.n_real[l - 2]; N.nr_parent = g.n_real[l - 1]; N.np = g.n_real[l - 1];
+                    N.F = g.F[l - 1]; N.logF = ilog2i(g.F[l - 1]); N.nq = nq;
+                    N.child = view(l - 2, false);
+                    N.gpar = icm::CoefView{p->d_g[na], (long long)g.n_real[l - 1] * g.G[l - 1], g.G[l - 1], g.G[l - 1]};
+                    N.gchild = icm::CoefOut{p->d_g[na ^ 1], (long long)g.n_real[l - 2] * g.G[l - 2], g.G[l - 2],
+                                            g.G[l - 2]};
+                    // the wrapped term reads coefficients 0 and F of the level below's A; only 0 is kept
+                    fuse = icm::prop_fusable(J, N) && N.gpar.count <= N.F;
+                }
+                if (!hip_ok(icm::launch_prop_large(J, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, s, &p->prof,
+                                                   spec_of(l), true, fuse ? &N : nullptr, fuse),
+                            "vjp propagate level"))   // A_L = A_X * R
+                    return -1;
+                a_ready = fuse;
                 ca = na;
                 cb = nbv;
                 continue;
             }
+            a_ready = false;
+            if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
             // same g pair as the pass above: its column transforms are still in bufA
             if (!level(K, false, dspec_of(l), true)) return -1;
             if (!root) {

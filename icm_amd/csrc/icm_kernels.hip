@@ -2907,10 +2907,12 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
 // (same values: the scaled coefficients, zero past the count and for a
 // missing right child, then the same passes and twiddles).  Level l-1's g
 // coefficients are never stored.
+// keep_g0: coefficient 0 of every child's g is stored as well (the adjoint's
+// B propagate of the level below reads it for its wrapped term).
 template <int F1>
 __global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__ in, LevelJob J,
                                                      double2 *__restrict__ out, int nArr, int logF,
-                                                     const double2 *tw, int twShift) {
+                                                     const double2 *tw, int twShift, int keep_g0) {
     static_assert(F1 >= 8, "the next level must be a four-step level with column kernels");
     using TL = ColLayout<F1>;
     constexpr int H = F1 / 2;
@@ -2960,6 +2962,11 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__
             const double2 v = lds[TL::at(c, r)];
             held[h][u] = make_double2((has_l && n < gcount) ? v.x * invF : 0.0,
                                       (has_r && n < gcount) ? v.y * invF : 0.0);
+            if (keep_g0 && n == 0 && has_l) {
+                double *gl = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+                gl[0] = held[h][u].x;
+                if (has_r) gl[J.gchild.stride] = held[h][u].y;
+            }
         }
         __syncthreads();                          // reads done before the tile is reused
     }
@@ -4007,9 +4014,9 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
     return hipSuccess;
 }
 static hipError_t col_inv_fwd(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
-                              int logF, const double2 *tw, int twShift, hipStream_t s) {
+                              int logF, const double2 *tw, int twShift, hipStream_t s, bool keep_g0) {
     dim3 grid(F1 / 2, nArr, nq);
-#define ICM_CIF(V) case V: k_col_inv_fwd<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift); break;
+#define ICM_CIF(V) case V: k_col_inv_fwd<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift, keep_g0 ? 1 : 0); break;
     switch (F1) {
         ICM_CIF(8) ICM_CIF(16) ICM_CIF(32) ICM_CIF(64) ICM_CIF(128) ICM_CIF(256) ICM_CIF(512) ICM_CIF(1024)
         ICM_CIF(2048) ICM_CIF(4096)
@@ -4025,8 +4032,23 @@ bool prop_fusable(const LevelJob &J, const LevelJob &Jn) {
            Jn.gpar.count == J.gchild.count && Jn.nq == J.nq && J.np == J.nr_parent;
 }
 
+hipError_t launch_g_pairs(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, hipStream_t s, Prof *pf) {
+    if (J.F <= kMaxLdsF || col_fused(J.F)) return hipErrorInvalidValue;
+    const int F1 = J.F >> 12;
+    const int nrp = J.np, npairs = (nrp + 1) / 2;
+    const long long r1 = (long long)J.nq * nrp * J.F;
+    const int gq = J.gpar.plane == 0 ? 1 : J.nq;
+    const double F = J.F, colFl = kF2 * fft_flops(F1) + 6.0 * F;
+    const double gin = J.gpar.plane == 0 ? 8.0 * J.gpar.count : 8.0 * J.nq * (double)nrp * J.gpar.count;
+    PF_BEGIN();
+    ICM_TRY(col_fwd(F1, GRows{J}, bufA + r1, npairs, gq, J.logF, tw, twShift, s));
+    PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * gq * npairs * F, gq * npairs * colFl);
+    return hipSuccess;
+}
+
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, double2 *bufB,
-                             hipStream_t s, Prof *pf, const double2 *spec, bool g_ready, const LevelJob *next) {
+                             hipStream_t s, Prof *pf, const double2 *spec, bool g_ready, const LevelJob *next,
+                             bool keep_g0) {
     if (next && !prop_fusable(J, *next)) return hipErrorInvalidValue;
     const int F1 = J.F >> 12;
     const int nrp = J.np, npairs = (nrp + 1) / 2;   // slots (all parents without pruning)
@@ -4076,7 +4098,7 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     if (next) {   // the next level's g pairs straight from this level's inverse columns
         double2 *gout = bufA + (long long)next->nq * next->np * next->F;
         PF_BEGIN();
-        ICM_TRY(col_inv_fwd(F1, bufB, J, gout, nrp, J.nq, J.logF, tw, twShift, s));
+        ICM_TRY(col_inv_fwd(F1, bufB, J, gout, nrp, J.nq, J.logF, tw, twShift, s, keep_g0));
         PF_END(kname("k_col_inv_fwd<%d>", F1), 16.0 * nq * nrp * F + 8.0 * nq * nrp * F,
                nq * nrp * kF2 * (fft_flops(F1) + fft_flops(F1 / 2)));
         return hipSuccess;

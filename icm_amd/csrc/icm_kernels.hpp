@@ -171,10 +171,15 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
 // level with column kernels) is fused in: this level's column inverse writes
 // next's g-pair column transforms (at bufA + next.nq * next.np * next.F,
 // where next's launch reads them with g_ready) instead of the children's g
-// coefficients, which are then never stored.  Only where prop_fusable.
+// coefficients, which are then never stored (keep_g0: except each child's
+// coefficient 0).  Only where prop_fusable.
 hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA,
                              double2 *bufB, hipStream_t s, Prof *pf, const double2 *spec = nullptr,
-                             bool g_ready = false, const LevelJob *next = nullptr);
+                             bool g_ready = false, const LevelJob *next = nullptr, bool keep_g0 = false);
+// The g-pair column transforms of a four-step propagate J into bufA (where
+// launch_prop_large with g_ready and launch_prop_large_b read them).
+hipError_t launch_g_pairs(const LevelJob &J, const double2 *tw, int twShift, double2 *bufA, hipStream_t s,
+                          Prof *pf);
 bool prop_fusable(const LevelJob &J, const LevelJob &next);
 
 // Adjoint B propagate of a four-step level in one pass: B children =
