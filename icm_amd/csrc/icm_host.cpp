@@ -916,6 +916,14 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         // B propagates then read spectra, and the exact terms read the end
         // coefficients, which the fused kernels store).  p_ready: the level
         // below handed this level's entry over.
+        // d_ready: likewise for the tangent builds (the level below's dP
+        // column inverse wrote this level's dP-pair transforms into its cache
+        // entry).  partial[l]: level l's P or dP coefficients were handed over
+        // and only their end coefficients stored; nothing may read them whole
+        // (a rebuild from them is refused -- the fusions are chosen so that
+        // none is needed: a level with both caches is never rebuilt).
+        bool d_ready = false;
+        std::vector<char> partial(L + 1, 0);
         bool p_ready = false;
         auto build_level = [&](int l, bool recompute) -> bool {
             icm::LevelJob J = pjob(l);
@@ -923,6 +931,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 J.tchild = view(l - 1, true);
                 J.tparent = icm::CoefOut{dcoef_of(p, l), J.parent.plane, J.parent.stride, J.parent.count};
                 p_ready = false;
+                d_ready = false;
                 return hip_ok(icm::launch_build_lds_t(J, p->d_tw, p->twShift, s, &p->prof), "vjp build level");
             }
             if (!recompute) {                                     // P_X = P_L P_R
@@ -939,14 +948,34 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                             "vjp build level"))
                     return false;
                 p_ready = fuse;
+                if (fuse) partial[l] = 1;
             }
             J.parent.base = dcoef_of(p, l);
+            if (recompute && partial[l - 1]) {
+                set_err("adjoint checkpoint: a rebuilt level would read coefficients handed over as spectra");
+                return false;
+            }
             if (dspec_of(l)) {                                    // dP_X from both pairs' spectra
                 J.tchild = view(l - 1, true);
-                return hip_ok(icm::launch_build_large_t(J, p->d_tw, p->twShift, spec_of(l), dspec_of(l), p->d_bufB,
-                                                        s, &p->prof),
-                              "vjp tangent build level");
+                icm::LevelJob N{};
+                bool fuse = false;
+                if (!recompute && l + 1 <= L - 2 && spec_of(l + 1) && dspec_of(l + 1) && (p->fuse_columns & 2)) {
+                    N = pjob(l + 1);
+                    N.tchild = view(l, true);
+                    fuse = icm::build_fusable_t(J, N);
+                }
+                if (!hip_ok(icm::launch_build_large_t(J, p->d_tw, p->twShift, spec_of(l), dspec_of(l), p->d_bufB, s,
+                                                      &p->prof, d_ready && !recompute, fuse ? &N : nullptr,
+                                                      fuse ? dspec_of(l + 1) : nullptr),
+                            "vjp tangent build level"))
+                    return false;
+                if (!recompute) {
+                    d_ready = fuse;
+                    if (fuse) partial[l] = 1;
+                }
+                return true;
             }
+            if (!recompute) d_ready = false;
             J.child = view(l - 1, true);
             J.child_r = view(l - 1, false);
             if (!level(J, true)) return false;                    // dP_X = dP_L P_R

@@ -3022,7 +3022,10 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__
 // the two end terms of every parent are stored (the level above computes its
 // own end terms exactly from them); the rest are never stored: fused only
 // where every later reader takes the spectra.
-template <int F1>
+// TAN: the adjoint's tangent build instead (dP_X into J.parent, J.child the
+// children's P, J.tchild their dP, as StParentTangent stores it), handing the
+// level above its dP child-pair transforms (a missing right child 0).
+template <int F1, bool TAN = false>
 __global__ __launch_bounds__(kCT) void k_col_inv_fwd_build(const double2 *__restrict__ in, LevelJob J,
                                                            double2 *__restrict__ out, int nArr, int logF,
                                                            const double2 *tw, int twShift) {
@@ -3075,8 +3078,13 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd_build(const double2 *__rest
         const double2 z = lds[TL::at(c, r)];
         double x = 0.0, y = 0.0;
         if (n == 0) {
-            if (hx) x = child_prod_at(J, qi, p, 0);
-            y = hy ? child_prod_at(J, qi, p + 1, 0) : 1.0;   // a missing right child is the identity
+            if constexpr (TAN) {
+                if (hx) x = StParentTangent{J}.exact(qi, p, 0);
+                if (hy) y = StParentTangent{J}.exact(qi, p + 1, 0);   // a missing right child's dP is 0
+            } else {
+                if (hx) x = child_prod_at(J, qi, p, 0);
+                y = hy ? child_prod_at(J, qi, p + 1, 0) : 1.0;   // a missing right child is the identity
+            }
         } else if (n < pc) {
             if (hx) x = z.x * invF;
             if (hy) y = z.y * invF;
@@ -3091,8 +3099,13 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd_build(const double2 *__rest
     double2 top = make_double2(0.0, 0.0);       // coefficient F, at row F1 of column 0
     if (tile == 0 && pc > F) {
         const int tc = J.child.count - 1;
-        if (hx) top.x = child_prod_at(J, qi, p, tc);
-        if (hy) top.y = child_prod_at(J, qi, p + 1, tc);
+        if constexpr (TAN) {
+            if (hx) top.x = StParentTangent{J}.exact(qi, p, tc);
+            if (hy) top.y = StParentTangent{J}.exact(qi, p + 1, tc);
+        } else {
+            if (hx) top.x = child_prod_at(J, qi, p, tc);
+            if (hy) top.y = child_prod_at(J, qi, p + 1, tc);
+        }
         if (t == 0) {
             if (hx) pb[(long long)p * J.parent.stride + F] = top.x;
             if (hy) pb[(long long)(p + 1) * J.parent.stride + F] = top.y;
@@ -3912,9 +3925,13 @@ static hipError_t col_inv(int F1, const double2 *in, Storer st, int nArr, int nq
 }
 
 static hipError_t col_inv_fwd_build(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
-                                    int logF, const double2 *tw, int twShift, hipStream_t s) {
+                                    int logF, const double2 *tw, int twShift, hipStream_t s, bool tan = false) {
     dim3 grid(F1, nArr, nq);
-#define ICM_CIB(V) case V: k_col_inv_fwd_build<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift); break;
+#define ICM_CIB(V)                                                                                            \
+    case V:                                                                                                   \
+        if (tan) k_col_inv_fwd_build<V, true><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift);     \
+        else k_col_inv_fwd_build<V, false><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift);       \
+        break;
     switch (F1) {
         ICM_CIB(4) ICM_CIB(8) ICM_CIB(16) ICM_CIB(32) ICM_CIB(64) ICM_CIB(128) ICM_CIB(256) ICM_CIB(512) ICM_CIB(1024)
     default: return hipErrorInvalidValue;
@@ -3988,8 +4005,10 @@ hipError_t launch_build_large(const LevelJob &J, const double2 *tw, int twShift,
 }
 
 hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShift, const double2 *specP,
-                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf) {
+                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf, bool d_ready,
+                                const LevelJob *next, double2 *next_specD) {
     if (!specP || !specD || col_fused(J.F)) return hipErrorInvalidValue;
+    if (next && (!next_specD || !build_fusable_t(J, *next))) return hipErrorInvalidValue;
     const int F1 = J.F >> 12;
     const int nrp = J.nr_parent, npairs = (nrp + 1) / 2;
     const double F = J.F, nq = J.nq;
@@ -3998,20 +4017,38 @@ hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShif
     K.child = J.tchild;
     K.child_r = CoefView{};
     K.r_missing_zero = 1;
-    PF_BEGIN();
-    ICM_TRY(col_fwd(F1, ChildRows{K}, specD, nrp, J.nq, J.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,ChildRows>", F1), level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F,
-           nq * nrp * colFl);
+    if (!d_ready) {
+        PF_BEGIN();
+        ICM_TRY(col_fwd(F1, ChildRows{K}, specD, nrp, J.nq, J.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,ChildRows>", F1),
+               level_coef_bytes(J.tchild, J.nr_child, J.nq) + 16.0 * nq * nrp * F, nq * nrp * colFl);
+    }
     PF_BEGIN();
     k_rowpair_build_t<RowBuildT><<<dim3(rowpair_groups(F1), npairs, J.nq), RowBuildT::NTH, 0, s>>>(specP, specD, bufB, nrp, npairs,
                                                                                        J.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_build_t", 16.0 * nq * (2 * nrp + npairs) * F, nq * (2 * nrp + npairs) * rowFl + 40.0 * nq * nrp * F);
+    if (next) {   // the level above's dP child-pair transforms straight from this level's inverse columns
+        PF_BEGIN();
+        ICM_TRY(col_inv_fwd_build(F1, bufB, J, next_specD, npairs, J.nq, J.logF, tw, twShift, s, true));
+        PF_END(kname("k_col_inv_fwd_build_t<%d>", F1), 16.0 * nq * npairs * F + 32.0 * nq * npairs * F,
+               nq * npairs * kF2 * (fft_flops(F1) + fft_flops(2 * F1)));
+        return hipSuccess;
+    }
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StParentTangent{J}, npairs, J.nq, J.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StParentTangent>", F1), 16.0 * nq * npairs * F + 8.0 * nq * nrp * J.parent.count,
            nq * npairs * kF2 * fft_flops(F1));
     return hipSuccess;
+}
+
+// The tangent hand-over: as build_fusable for the P pairs (same counts and
+// transform sizes), with the tangent build's own composition (children's dP
+// from tchild, nothing accumulated).
+bool build_fusable_t(const LevelJob &J, const LevelJob &Jn) {
+    return !col_fused(J.F) && J.F >= 4 * kF2 && J.F <= 1024 * kF2 && Jn.F == 2 * J.F && !J.acc_out &&
+           !J.child_r.base && !J.r_missing_zero && J.tchild.base && Jn.nr_child == J.nr_parent &&
+           Jn.child.count == J.parent.count && J.parent.count <= J.F + 1 && Jn.nq == J.nq;
 }
 static hipError_t col_inv_fwd(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
                               int logF, const double2 *tw, int twShift, hipStream_t s, bool keep_g0) {

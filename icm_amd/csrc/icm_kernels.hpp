@@ -163,8 +163,17 @@ bool build_fusable(const LevelJob &J, const LevelJob &next);
 // level's P-pair row-stage inputs (the spectrum cache, written by the P
 // build); the dP-pair inputs are written to specD (nq * n_real[l] * F
 // complex) for the A * dR propagate of the same q-batch to reuse.
+// d_ready: specD already holds this level's dP-pair transforms (the level
+// below handed them over).  next / next_specD (optional): the tangent build
+// of the level above is fused in: this level's column inverse writes its dP
+// child-pair transforms into next_specD; of this level's dP only the end
+// coefficients (0, count - 1 and the top term) are stored, for the exact
+// terms that read them (only where build_fusable_t, and where every later
+// reader takes the spectra).
 hipError_t launch_build_large_t(const LevelJob &J, const double2 *tw, int twShift, const double2 *specP,
-                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf);
+                                double2 *specD, double2 *bufB, hipStream_t s, Prof *pf, bool d_ready = false,
+                                const LevelJob *next = nullptr, double2 *next_specD = nullptr);
+bool build_fusable_t(const LevelJob &J, const LevelJob &next);
 // g_ready: bufA already holds this J's g-pair column transforms (the previous
 // launch had the same gpar, nq and parents), so they are not computed again.
 // next (optional): the propagate of the level below (F / 2, also a four-step
