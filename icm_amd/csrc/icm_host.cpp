@@ -1002,7 +1002,9 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
         // were) when the level below takes the same form; that level's A
         // coefficients are then never stored except coefficient 0, which its
         // B propagate's wrapped term reads.  a_ready: bufA holds this level's.
-        bool a_ready = false;
+        // b_ready: likewise for B (the level above's B column inverse wrote
+        // this level's B g-pair transforms; B's coefficients 0 and F / 2 kept).
+        bool a_ready = false, b_ready = false;
         auto one_pass_b = [&](int l) {
             return l < L - 1 && g.F[l] > icm::kMaxLdsF && !icm::col_fused(g.F[l]) && dspec_of(l) != nullptr;
         };
@@ -1053,10 +1055,20 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 if (!a_ready && !hip_ok(icm::launch_g_pairs(J, p->d_tw, p->twShift, p->d_bufA, s, &p->prof),
                                         "vjp propagate level"))
                     return -1;
+                icm::LevelJob NB{};                               // the level below's B propagate (B_X * R)
+                bool fuse_b = false;
+                if (l - 1 >= 1 && one_pass_b(l - 1) && (p->fuse_columns & 1)) {
+                    NB.nr_child = g.n_real[l - 2]; NB.nr_parent = g.n_real[l - 1]; NB.np = g.n_real[l - 1];
+                    NB.F = g.F[l - 1]; NB.logF = ilog2i(g.F[l - 1]); NB.nq = nq;
+                    NB.child = view(l - 2, false);
+                    NB.gpar = icm::CoefView{p->d_bv[nbv], Bout.plane, Bout.stride, Bout.count};
+                    fuse_b = icm::prop_b_fusable(K, NB);
+                }
                 if (!hip_ok(icm::launch_prop_large_b(K, KB, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, spec_of(l),
-                                                     dspec_of(l), s, &p->prof),
+                                                     dspec_of(l), s, &p->prof, b_ready, fuse_b ? &NB : nullptr),
                             "vjp propagate level"))
                     return -1;
+                b_ready = fuse_b;
                 icm::LevelJob N{};                                // the level below's A propagate
                 bool fuse = false;
                 if (l - 1 >= 1 && one_pass_b(l - 1) && (p->fuse_columns & 1)) {
@@ -1078,7 +1090,7 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
                 cb = nbv;
                 continue;
             }
-            a_ready = false;
+            a_ready = b_ready = false;
             if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
             // same g pair as the pass above: its column transforms are still in bufA
             if (!level(K, false, dspec_of(l), true)) return -1;

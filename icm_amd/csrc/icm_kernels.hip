@@ -2909,8 +2909,11 @@ __global__ __launch_bounds__(kCT) void k_col_inv(const double2 *__restrict__ in,
 // coefficients are never stored.
 // keep_g0: coefficient 0 of every child's g is stored as well (the adjoint's
 // B propagate of the level below reads it for its wrapped term).
-template <int F1>
-__global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__ in, LevelJob J,
+// B2: the adjoint's one-pass B propagate instead (StGChildren2's values: J's
+// and J2's wrapped terms added at coefficient c - 1), storing each child's
+// coefficients 0 and F / 2, which the level below's wrapped terms read.
+template <int F1, bool B2 = false>
+__global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__ in, LevelJob J, LevelJob J2,
                                                      double2 *__restrict__ out, int nArr, int logF,
                                                      const double2 *tw, int twShift, int keep_g0) {
     static_assert(F1 >= 8, "the next level must be a four-step level with column kernels");
@@ -2954,15 +2957,37 @@ __global__ __launch_bounds__(kCT) void k_col_inv_fwd(const double2 *__restrict__
             }
         }
         fft_tile<F1, true, kCT, TL>(lds, tw, twShift);
+        // coefficient n of both children from inverse value v (scaled; B2:
+        // the wrapped terms at c - 1)
+        auto value = [&](int n, double2 v) {
+            double vx = v.x * invF, vy = v.y * invF;
+            if constexpr (B2) {
+                if (has_l && J.gfix && n == J.child.count - 1) {
+                    vx += prop_wrap_term(J, qi, p, 0) + prop_wrap_term(J2, qi, p, 0);
+                    vy += prop_wrap_term(J, qi, p, 1) + prop_wrap_term(J2, qi, p, 1);
+                }
+            }
+            return make_double2((has_l && n < gcount) ? vx : 0.0, (has_r && n < gcount) ? vy : 0.0);
+        };
+        if constexpr (B2) {
+            if (t < 2 && has_l) {
+                const int n = t == 0 ? 0 : F / 2;
+                if (n < gcount && ((n & (kF2 - 1)) >> LOGTC) == tile) {
+                    const int r = n >> 12, c = (n & (kF2 - 1)) - tile * TC;
+                    const double2 z = value(n, lds[TL::at(c, r)]);
+                    double *gl = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+                    gl[n] = z.x;
+                    if (has_r) gl[J.gchild.stride + n] = z.y;
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < KEEP; ++u) {
             const int e = t + u * kCT;            // e < kTile / 2: rows r < F1 / 2
             const int r = e >> LOGTC, c = e & (TC - 1);
             const int n = r * kF2 + tile * TC + c;
-            const double2 v = lds[TL::at(c, r)];
-            held[h][u] = make_double2((has_l && n < gcount) ? v.x * invF : 0.0,
-                                      (has_r && n < gcount) ? v.y * invF : 0.0);
-            if (keep_g0 && n == 0 && has_l) {
+            held[h][u] = value(n, lds[TL::at(c, r)]);
+            if (!B2 && keep_g0 && n == 0 && has_l) {
                 double *gl = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
                 gl[0] = held[h][u].x;
                 if (has_r) gl[J.gchild.stride] = held[h][u].y;
@@ -4051,9 +4076,14 @@ bool build_fusable_t(const LevelJob &J, const LevelJob &Jn) {
            Jn.child.count == J.parent.count && J.parent.count <= J.F + 1 && Jn.nq == J.nq;
 }
 static hipError_t col_inv_fwd(int F1, const double2 *in, const LevelJob &J, double2 *out, int nArr, int nq,
-                              int logF, const double2 *tw, int twShift, hipStream_t s, bool keep_g0) {
+                              int logF, const double2 *tw, int twShift, hipStream_t s, bool keep_g0,
+                              const LevelJob *J2 = nullptr) {
     dim3 grid(F1 / 2, nArr, nq);
-#define ICM_CIF(V) case V: k_col_inv_fwd<V><<<grid, kCT, 0, s>>>(in, J, out, nArr, logF, tw, twShift, keep_g0 ? 1 : 0); break;
+#define ICM_CIF(V)                                                                                            \
+    case V:                                                                                                   \
+        if (J2) k_col_inv_fwd<V, true><<<grid, kCT, 0, s>>>(in, J, *J2, out, nArr, logF, tw, twShift, 0);    \
+        else k_col_inv_fwd<V, false><<<grid, kCT, 0, s>>>(in, J, J, out, nArr, logF, tw, twShift, keep_g0 ? 1 : 0); \
+        break;
     switch (F1) {
         ICM_CIF(8) ICM_CIF(16) ICM_CIF(32) ICM_CIF(64) ICM_CIF(128) ICM_CIF(256) ICM_CIF(512) ICM_CIF(1024)
         ICM_CIF(2048) ICM_CIF(4096)
@@ -4146,25 +4176,41 @@ hipError_t launch_prop_large(const LevelJob &J, const double2 *tw, int twShift, 
     return hipSuccess;
 }
 
+bool prop_b_fusable(const LevelJob &JdR, const LevelJob &Jn) {
+    return !col_fused(JdR.F) && JdR.F >= 8 * kF2 && Jn.F * 2 == JdR.F && !col_fused(Jn.F) && !JdR.plist &&
+           !Jn.plist && !JdR.acc_out && Jn.np == JdR.nr_child && Jn.gpar.count == JdR.gchild.count &&
+           Jn.nq == JdR.nq && JdR.np == JdR.nr_parent;
+}
+
 hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const double2 *tw, int twShift,
                                double2 *bufA, double2 *bufB, const double2 *specP, const double2 *specD, hipStream_t s,
-                               Prof *pf) {
+                               Prof *pf, bool b_ready, const LevelJob *next) {
     if (!specP || !specD || JdR.plist || col_fused(JdR.F)) return hipErrorInvalidValue;
+    if (next && !prop_b_fusable(JdR, *next)) return hipErrorInvalidValue;
     const int F1 = JdR.F >> 12;
     const int nrp = JdR.np, npairs = (nrp + 1) / 2;
     const long long r1 = (long long)JdR.nq * nrp * JdR.F;
     const double F = JdR.F, nq = JdR.nq;
     const double colFl = kF2 * fft_flops(F1) + 6.0 * F, rowFl = F1 * fft_flops(kF2);
     const double gin = JR.gpar.plane == 0 ? 8.0 * JR.gpar.count : 8.0 * nq * nrp * JR.gpar.count;
-    PF_BEGIN();   // the B pairs into the child-pair region of bufA (unused with the caches)
-    ICM_TRY(col_fwd(F1, GRows{JR}, bufA, npairs, JdR.nq, JdR.logF, tw, twShift, s));
-    PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    if (!b_ready) {
+        PF_BEGIN();   // the B pairs into the child-pair region of bufA (unused with the caches)
+        ICM_TRY(col_fwd(F1, GRows{JR}, bufA, npairs, JdR.nq, JdR.logF, tw, twShift, s));
+        PF_END(kname("k_col_fwd<%d,GRows>", F1), gin + 16.0 * nq * npairs * F, nq * npairs * colFl);
+    }
     PF_BEGIN();
     k_rowpair_prop_b<RowPropB, RowSrc, RowSrc, RowSrc, RowSrc, RowDst><<<dim3(rowpair_groups(F1), npairs, JdR.nq), RowPropB::NTH, 0, s>>>(
         RowSrc{specP, nrp, JdR.logF}, RowSrc{specD, nrp, JdR.logF}, RowSrc{bufA + r1, npairs, JdR.logF},
         RowSrc{bufA, npairs, JdR.logF}, RowDst{bufB, nrp, JdR.logF}, nrp, npairs, JdR.logF, tw, twShift);
     ICM_TRY(hipGetLastError());
     PF_END("k_rowpair_prop_b", 16.0 * nq * (3 * nrp + 2 * npairs) * F, nq * (3 * nrp + 2 * npairs) * rowFl);
+    if (next) {   // the level below's B pairs straight from this level's inverse columns
+        PF_BEGIN();
+        ICM_TRY(col_inv_fwd(F1, bufB, JdR, bufA, nrp, JdR.nq, JdR.logF, tw, twShift, s, false, &JR));
+        PF_END(kname("k_col_inv_fwd_b<%d>", F1), 16.0 * nq * nrp * F + 8.0 * nq * nrp * F,
+               nq * nrp * kF2 * (fft_flops(F1) + fft_flops(F1 / 2)));
+        return hipSuccess;
+    }
     PF_BEGIN();
     ICM_TRY(col_inv(F1, bufB, StGChildren2{JdR, JR}, nrp, JdR.nq, JdR.logF, tw, twShift, s));
     PF_END(kname("k_col_inv<%d,StGChildren2>", F1),

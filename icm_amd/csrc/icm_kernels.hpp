@@ -197,9 +197,15 @@ bool prop_fusable(const LevelJob &J, const LevelJob &next);
 // (specP: the P pairs' column transforms, specD: the dP pairs') and bufA
 // holding JdR's g-pair column transforms (the A propagate just ran); B's
 // pairs go to bufA's child-pair region.  No parent lists.
+// b_ready: bufA's child-pair region already holds B's g-pair transforms (the
+// level above handed them over).  next (optional): the level below's B
+// propagate (its JR) is fused in: this level's column inverse writes its B
+// g-pair transforms into that region; of the B children only coefficients 0
+// and F / 2 are stored, for the wrapped terms (only where prop_b_fusable).
 hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const double2 *tw, int twShift,
                                double2 *bufA, double2 *bufB, const double2 *specP, const double2 *specD, hipStream_t s,
-                               Prof *pf);
+                               Prof *pf, bool b_ready = false, const LevelJob *next = nullptr);
+bool prop_b_fusable(const LevelJob &JdR, const LevelJob &next);
 
 // Ragged batches.  Leaves: global blocks [q][NB][B+1] for q-planes
 // q0 .. q0 + nq - 1; extract writes partial[chunk][Ntot].  Levels: one
