@@ -210,14 +210,15 @@ def test_vjp_spectrum_cache_matches_recompute(n, k, q_batch, Q):
     assert err < 1e-11
 
 
-@pytest.mark.parametrize("n,k,Q", [(262144, 262144, 16), (65536, 65536, 32)])
+@pytest.mark.parametrize("n,k,Q", [(262144, 262144, 16), (65536, 65536, 32), (300001, 300001, 8)])
 def test_vjp_column_fusion_matches_unfused(n, k, Q):
     """The adjoint's column fusions (up-sweep P and dP builds handing the level
     above its pair transforms, down-sweep B then A propagates handing the
     level below its g-pair transforms, with the end coefficients the wrapped
     and exact terms read) against the same plan with every fusion off and the
     spectrum caches on: at k = n the wrapped terms apply at every four-step
-    level; n = 2^18 has three fused level pairs each way.  The fused kernels
+    level; n = 2^18 has three fused level pairs each way, n = 300001 odd node
+    counts (19, 5, 3: parents without a right child) at fused levels.  The fused kernels
     compute the same sums as the separate column kernels, except the F = 8192
     build's hand-over (a different rounding, as in the forward): dS within
     1e-13 of max|dS|, equities within 1e-13 of max|E|."""
