@@ -1092,6 +1092,19 @@ int run_vjp(IcmHipPlan *p, int Q, const double *d_payout, const double *d_v_orig
             }
             a_ready = b_ready = false;
             if (!level(J, false, spec_of(l))) return -1;         // A_L = A_X * R
+            if (icm::col_fused(J.F) && !root) {                  // F = 8192: B in one pass from the coefficients
+                icm::LevelJob KB = J;
+                KB.gpar = Bpar;
+                KB.gchild = Bout;
+                KB.gfix = gfix;
+                if (!hip_ok(icm::launch_prop_large_b(K, KB, p->d_tw, p->twShift, p->d_bufA, p->d_bufB, nullptr,
+                                                     nullptr, s, &p->prof),
+                            "vjp propagate level"))
+                    return -1;
+                ca = na;
+                cb = nbv;
+                continue;
+            }
             // same g pair as the pass above: its column transforms are still in bufA
             if (!level(K, false, dspec_of(l), true)) return -1;
             if (!root) {

@@ -4185,7 +4185,25 @@ bool prop_b_fusable(const LevelJob &JdR, const LevelJob &Jn) {
 hipError_t launch_prop_large_b(const LevelJob &JdR, const LevelJob &JR, const double2 *tw, int twShift,
                                double2 *bufA, double2 *bufB, const double2 *specP, const double2 *specD, hipStream_t s,
                                Prof *pf, bool b_ready, const LevelJob *next) {
-    if (!specP || !specD || JdR.plist || col_fused(JdR.F)) return hipErrorInvalidValue;
+    if (col_fused(JdR.F)) {   // F = 8192: the column stage inside the row kernel, straight from the coefficients
+        if (JdR.plist || b_ready || next) return hipErrorInvalidValue;
+        const int nrp = JdR.np, npairs = (nrp + 1) / 2;
+        const double F = JdR.F, nq = JdR.nq;
+        const double colFl = kF2 * fft_flops(2) + 6.0 * F, rowFl = 2 * fft_flops(kF2);
+        PF_BEGIN();
+        k_rowpair_prop_b<RowPropB, Col2Src<ChildRows>, Col2Src<ChildRows>, Col2Src<GRows>, Col2Src<GRows>,
+                         Col2Dst<StGChildren2>><<<dim3(1, npairs, JdR.nq), RowPropB::NTH, 0, s>>>(
+            Col2Src<ChildRows>{ChildRows{JR}, twShift}, Col2Src<ChildRows>{ChildRows{JdR}, twShift},
+            Col2Src<GRows>{GRows{JdR}, twShift}, Col2Src<GRows>{GRows{JR}, twShift},
+            Col2Dst<StGChildren2>{StGChildren2{JdR, JR}, twShift}, nrp, npairs, JdR.logF, tw, twShift);
+        ICM_TRY(hipGetLastError());
+        PF_END("k_rowpair_prop_b<F=8192>",
+               8.0 * nq * ((double)nrp * (JdR.gpar.count + JR.gpar.count) + 2.0 * JdR.nr_child * JdR.child.count) +
+                   8.0 * nq * JdR.nr_child * JdR.gchild.count,
+               nq * (4 * nrp + 2 * npairs) * (rowFl + colFl));
+        return hipSuccess;
+    }
+    if (!specP || !specD || JdR.plist) return hipErrorInvalidValue;
     if (next && !prop_b_fusable(JdR, *next)) return hipErrorInvalidValue;
     const int F1 = JdR.F >> 12;
     const int nrp = JdR.np, npairs = (nrp + 1) / 2;
