@@ -22,6 +22,33 @@
 
 namespace icm {
 
+// 0 and 1 that a loader reads in place of a coefficient past a row's end,
+// of a missing row or of the identity polynomial.  Such loaders choose an
+// address and load unconditionally: with the loads under per-lane branches
+// and the loaded value merged with a constant afterwards, the compiler
+// waited for all earlier loads at every merge, one memory latency per point.
+// (Not const: a const table would sit in the constant address space and
+// turn the selected loads into flat loads.)
+__device__ double kZeroOne[2] = {0.0, 1.0};
+// Write-only sink for stores of lanes with nothing to store (never read).
+__device__ double kSink[16];
+
+// A pointer read from device memory (a BatchInst field) is generic to the
+// compiler, and loads through it are flat loads, which count in both the
+// vector-memory and the LDS counters; with one of them in flight the
+// compiler waited for every outstanding load at each later wait.  Loads
+// through the global address space keep the counters in order.
+using gdouble = const __attribute__((address_space(1))) double;
+__device__ __forceinline__ gdouble *as_global(const double *p) { return (gdouble *)p; }
+
+// Waits for every outstanding vector memory access (vmcnt 0; expcnt and
+// lgkmcnt left at their maxima: simm16 0x0F70 in the gfx9 encoding).  Placed
+// before a loop whose body issues loads under per-lane branches: without it
+// the compiler could not tell whether loads issued before the loop were
+// still in flight and waited for all loads, prefetches included, at their
+// first use in every iteration.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------
 // Leaf build: one lane owns one block and multiplies its B linear factors in
 // registers (triangular recurrence, src/cpu/icm.c:2196-2213).  Players past n
@@ -467,7 +494,7 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
         gP[i] = (e < NP && gfirst + e / (B + 1) < gend) ? gfirst + e / (B + 1) : NB;
         mP[i] = e % (B + 1);
     }
-    const double *gbase[PPL];
+    gdouble *gbase[PPL];
     long long gplane[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; ++i) {
@@ -478,13 +505,23 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
         if (gG < gend) {
             const BatchInst &I2 = inst[blk_inst[gG]];
             if (mG < I2.G0) {
-                gbase[i] = I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG;
+                gbase[i] = as_global(I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG);
                 gplane[i] = I2.g0.plane;
             }
         }
     }
-    double pr[NPT], gr[PPL];
+    // The block's node and weight of a q-plane are fetched with that plane's
+    // block products, ahead of them: read inside the loop, after the next
+    // plane's loads, they waited for those loads (in-order vector memory
+    // counter), one memory latency per q-plane.  Lanes without a block read
+    // the 0 of kZeroOne.
+    gdouble *nodes = as_global(I ? I->logv + q0 : kZeroOne);
+    gdouble *wts = as_global(I ? I->w + q0 : kZeroOne);
+    const int qmul = I ? 1 : 0;
+    double pr[NPT], gr[PPL], lvn = 0.0, wn = 0.0;
     auto fetch = [&](int q) {
+        lvn = nodes[q * qmul];
+        wn = wts[q * qmul];
 #pragma unroll
         for (int i = 0; i < NPT; ++i)
             pr[i] = (gP[i] < NB) ? blocks[q * bplane + (long long)gP[i] * (B + 1) + mP[i]] : 0.0;
@@ -498,10 +535,14 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
 #pragma unroll
         for (int i = 0; i < PPL; ++i) shG[buf][t + i * 256] = gr[i];
     };
+    double lv = 0.0, wq = 0.0;
     if (qa < qe) {
         fetch(qa);
         stash(0);
+        lv = lvn;
+        wq = wn;
     }
+    vm_drain();
     __syncthreads();
     double acc[PPL];
 #pragma unroll
@@ -511,12 +552,11 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
         const bool more = q + 1 < qe;
         if (more) fetch(q + 1);
         if (any) {
-            const double lv = I->logv[q0 + q];
             double a[PPL], eq[PPL];
 #pragma unroll
             for (int u = 0; u < PPL; ++u) a[u] = valid[u] ? a_of(Sj[u], lv) : 0.75;
             leaf_psi_n<B, PPL>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk, eq);
-            const double el = exp(-lv), wq = I->w[q0 + q];
+            const double el = exp(-lv);
 #pragma unroll
             for (int u = 0; u < PPL; ++u) {
                 double pw = wq * Sj[u] * a[u] * el;
@@ -524,7 +564,11 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
                 acc[u] += pw * eq[u];
             }
         }
-        if (more) stash(cur ^ 1);
+        if (more) {
+            stash(cur ^ 1);
+            lv = lvn;
+            wq = wn;
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -533,11 +577,11 @@ __device__ __forceinline__ void leaf_extract_batch_n_body(int t, int gfirst, int
 }
 
 template <int B, int PPL>
-__global__ __launch_bounds__(256) void k_leaf_extract_batch_n(const BatchInst *__restrict__ inst,
-                                                              const int *__restrict__ blk_inst,
-                                                              const double *__restrict__ S,
-                                                              const double *__restrict__ blocks, int NB, int nq,
-                                                              int q0, int q_per_chunk, double *partial, int Ntot) {
+__global__ __launch_bounds__(256, 4) void k_leaf_extract_batch_n(const BatchInst *__restrict__ inst,
+                                                                 const int *__restrict__ blk_inst,
+                                                                 const double *__restrict__ S,
+                                                                 const double *__restrict__ blocks, int NB, int nq,
+                                                                 int q0, int q_per_chunk, double *partial, int Ntot) {
     using X = LeafExN<B, PPL>;
     __shared__ double shP[2][X::NP];
     __shared__ double shG[2][X::NG];
@@ -577,17 +621,24 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__r
     const int gP0 = gfirst + t / (B + 1), mP0 = t % (B + 1);
     const int gP1 = has1 ? gfirst + e1 / (B + 1) : NB, mP1 = e1 % (B + 1);
     const int gG = gfirst + t / B, mG = t % B;
-    const double *gbase = nullptr;
+    gdouble *gbase = nullptr;
     long long gplane = 0;
     if (gG < NB) {
         const BatchInst &I2 = inst[blk_inst[gG]];
         if (mG < I2.G0) {
-            gbase = I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG;
+            gbase = as_global(I2.g0.base + (long long)(gG - I2.boff) * I2.g0.stride + mG);
             gplane = I2.g0.plane;
         }
     }
-    double p0 = 0.0, p1 = 0.0, g0 = 0.0;
+    // node and weight fetched with the plane's operands, ahead of them (as
+    // leaf_extract_batch_n_body)
+    gdouble *nodes = as_global(I ? I->logv + q0 : kZeroOne);
+    gdouble *wts = as_global(I ? I->w + q0 : kZeroOne);
+    const int qmul = I ? 1 : 0;
+    double p0 = 0.0, p1 = 0.0, g0 = 0.0, lvn = 0.0, wn = 0.0;
     auto fetch = [&](int q) {
+        lvn = nodes[q * qmul];
+        wn = wts[q * qmul];
         p0 = (gP0 < NB) ? blocks[q * bplane + (long long)gP0 * (B + 1) + mP0] : 0.0;
         p1 = (gP1 < NB) ? blocks[q * bplane + (long long)gP1 * (B + 1) + mP1] : 0.0;
         g0 = gbase ? gbase[q * gplane] : 0.0;
@@ -597,10 +648,14 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__r
         if (has1) shP[buf][e1] = p1;
         shG[buf][t] = g0;
     };
+    double lv = 0.0, wq = 0.0;
     if (qa < qe) {
         fetch(qa);
         stash(0);
+        lv = lvn;
+        wq = wn;
     }
+    vm_drain();
     __syncthreads();
     double acc = 0.0;
     for (int q = qa; q < qe; ++q) {
@@ -608,14 +663,17 @@ __global__ __launch_bounds__(256) void k_leaf_extract_batch(const BatchInst *__r
         const bool more = q + 1 < qe;
         if (more) fetch(q + 1);
         if (valid) {
-            const double lv = I->logv[q0 + q];
             const double a = a_of(Sj, lv);
             const double eq = leaf_psi<B>(shP[cur] + lb * (B + 1), shG[cur] + lb * B, a, bsize, pk);
-            double pw = I->w[q0 + q] * Sj * a * exp(-lv);
+            double pw = wq * Sj * a * exp(-lv);
             if (!isfinite(pw)) pw = 0.0;
             acc += pw * eq;
         }
-        if (more) stash(cur ^ 1);
+        if (more) {
+            stash(cur ^ 1);
+            lv = lvn;
+            wq = wn;
+        }
         __syncthreads();
     }
     if (valid) partial[(long long)chunk * Ntot + I->poff + j] = acc;
@@ -1427,15 +1485,6 @@ __device__ __forceinline__ int slot_parent(const LevelJob &J, int i) {
     return J.plist ? J.plist[i] : i;
 }
 
-// 0 and 1 that a loader reads in place of a coefficient past a row's end,
-// of a missing row or of the identity polynomial.  Such loaders choose an
-// address and load unconditionally: with the loads under per-lane branches
-// and the loaded value merged with a constant afterwards, the compiler
-// waited for all earlier loads at every merge, one memory latency per point.
-// (Not const: a const table would sit in the constant address space and
-// turn the selected loads into flat loads.)
-__device__ double kZeroOne[2] = {0.0, 1.0};
-
 // Child pair of parent p packed as L + i R (q-plane qi); a missing right
 // child is the identity polynomial (or 0 with r_missing_zero); p < 0 gives
 // zeros.
@@ -2038,6 +2087,7 @@ __device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ in
                                                 int gend, int qa, int qe, int q0, int gb0, double *sa) {
     constexpr int CPL = B / 8;
     static_assert(CPL * 8 == B && 2 * 64 * B <= 2 * kTileMaxSize, "a values of two q-planes must fit the LDS");
+    static_assert(CPL + 1 <= 16, "a lane's stores must fit kSink");
     const int t = body_tid<true>(), bl = t >> 3, r = t & 7, sw = bl & 7;
     const int gb = gb0 + bl;
     const bool has = gb < gend;
@@ -2049,18 +2099,24 @@ __device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ in
     for (int u = 0; u < CPL; ++u) {
         const int j = b * B + r * CPL + u;
         pv[u] = has && j < I->n;
-        Sv[u] = pv[u] ? S[I->poff + j] : 0.0;
+        Sv[u] = *as_global(pv[u] ? S + I->poff + j : kZeroOne);
     }
     const long long bplane = (long long)NB * (B + 1);
+    // the next plane's node is read before this plane's stores: read after
+    // them it waited for them (stores count in the same in-order counter)
+    gdouble *nodes = as_global(has ? I->logv + q0 : kZeroOne);
+    const int qmul = has ? 1 : 0;
+    double lv = (qa < qe) ? nodes[qa * qmul] : 0.0;
+    vm_drain();
     for (int qi = qa; qi < qe; ++qi) {
         double *sq = sa + (qi & 1) * (64 * B) + bl * B;
-        const double lv = has ? I->logv[q0 + qi] : 0.0;
 #pragma unroll
         for (int u = 0; u < CPL; ++u) {
             const int i = r * CPL + u;
             sq[2 * ((i >> 1) ^ sw) + (i & 1)] = pv[u] ? a_of(Sv[u], lv) : 1.0;
         }
         __syncthreads();
+        lv = nodes[min(qi + 1, qe - 1) * qmul];
         double P[CPL], top = 0.0;
 #pragma unroll
         for (int u = 0; u < CPL; ++u) P[u] = 0.0;
@@ -2079,12 +2135,14 @@ __device__ __forceinline__ void leaf_build_coop(const BatchInst *__restrict__ in
                 P[0] = (r == 0) ? P[0] * a : a * P[0] + bb * left;
             }
         }
-        if (has) {
-            double *dst = blocks + qi * bplane + (long long)gb * (B + 1) + r * CPL;
+        // every lane stores CPL + 1 values (lanes without a block, and the
+        // top coefficient of lanes other than the last, into kSink): with
+        // a lane-dependent store count the next node's read, issued before
+        // these stores, waited for them
+        double *dst = has ? blocks + qi * bplane + (long long)gb * (B + 1) + r * CPL : kSink;
 #pragma unroll
-            for (int u = 0; u < CPL; ++u) dst[u] = P[u];
-            if (r == 7) dst[CPL] = top;
-        }
+        for (int u = 0; u < CPL; ++u) dst[u] = P[u];
+        *((has && r == 7) ? dst + CPL : kSink + CPL) = top;
     }
 }
 
