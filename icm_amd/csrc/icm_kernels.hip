@@ -40,6 +40,8 @@ __device__ double kSink[16];
 // through the global address space keep the counters in order.
 using gdouble = const __attribute__((address_space(1))) double;
 __device__ __forceinline__ gdouble *as_global(const double *p) { return (gdouble *)p; }
+using gdouble_rw = __attribute__((address_space(1))) double;
+__device__ __forceinline__ gdouble_rw *as_global_rw(double *p) { return (gdouble_rw *)p; }
 
 // Waits for every outstanding vector memory access (vmcnt 0; expcnt and
 // lgkmcnt left at their maxima: simm16 0x0F70 in the gfx9 encoding).  Placed
@@ -1476,7 +1478,7 @@ __device__ __forceinline__ double2 pack_ai(double2 A, double2 Bv) {
 
 // Coefficient m of node `node` in q-plane qi.
 __device__ __forceinline__ double view_at(const CoefView &V, int qi, int node, int m) {
-    return V.base[(long long)qi * V.plane + (long long)node * V.stride + m];
+    return as_global(V.base)[(long long)qi * V.plane + (long long)node * V.stride + m];
 }
 
 // Parent handled by slot i of a propagate launch (-1: none).
@@ -1498,16 +1500,17 @@ __device__ __forceinline__ double2 load_child_pair(const LevelJob &J, int qi, in
     const bool one = ok && !has_r && m == 0 && !J.r_missing_zero;
     const double *px = in ? lx : kZeroOne;
     const double *py = (in && has_r) ? ry : kZeroOne + (one ? 1 : 0);
-    return make_double2(*px, *py);
+    return make_double2(*as_global(px), *as_global(py));
 }
 
 // Exact product term L[m] R[m] of parent p (m = 0 or m = c-1).
 __device__ __forceinline__ double child_prod_at(const LevelJob &J, int qi, int p, int m) {
-    const double l = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m];
+    const double l =
+        as_global(J.child.base)[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + m];
     double r;
     if ((2 * p + 1) < J.nr_child) {
         const CoefView &R = J.child_r.base ? J.child_r : J.child;
-        r = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m];
+        r = as_global(R.base)[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + m];
     } else {
         r = (m == 0 && !J.r_missing_zero) ? 1.0 : 0.0;
     }
@@ -1515,8 +1518,9 @@ __device__ __forceinline__ double child_prod_at(const LevelJob &J, int qi, int p
 }
 
 __device__ __forceinline__ void put(double *dst, double v, int acc) {
-    if (acc) *dst += v;
-    else *dst = v;
+    gdouble_rw *g = as_global_rw(dst);
+    if (acc) *g += v;
+    else *g = v;
 }
 
 // Wrapped term of a propagate output coefficient m = c-1 when F = 2c - 2:
@@ -1524,18 +1528,19 @@ __device__ __forceinline__ void put(double *dst, double v, int acc) {
 //   side 0 (left output, correlated with R), side 1 (right output, with L).
 __device__ __forceinline__ double prop_wrap_term(const LevelJob &J, int qi, int p, int side) {
     const int c = J.child.count;
-    const double *Y = J.gpar.base + (long long)qi * J.gpar.plane + (long long)p * J.gpar.stride;
+    gdouble *Y = as_global(J.gpar.base + (long long)qi * J.gpar.plane + (long long)p * J.gpar.stride);
     const double yF = (J.F < J.gpar.count) ? Y[J.F] : 0.0;
     double x;
     if (side == 0) {
         if ((2 * p + 1) < J.nr_child) {
             const CoefView &R = J.child_r.base ? J.child_r : J.child;
-            x = R.base[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + c - 1];
+            x = as_global(R.base)[(long long)qi * R.plane + (long long)(2 * p + 1) * R.stride + c - 1];
         } else {
             x = (c == 1 && !J.r_missing_zero) ? 1.0 : 0.0;
         }
     } else {
-        x = J.child.base[(long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + c - 1];
+        const long long o = (long long)qi * J.child.plane + (long long)(2 * p) * J.child.stride + c - 1;
+        x = as_global(J.child.base)[o];
     }
     return x * (yF - Y[0]);
 }
@@ -1717,7 +1722,7 @@ __device__ __forceinline__ void prop_lds_body(const LevelJob &J, double2 *lds, c
     auto ldg = [&](int item, int m) {
         int qi, pr;
         M.at(item, qi, pr);
-        const double *gb = J.gpar.base + (long long)qi * J.gpar.plane;
+        gdouble *gb = as_global(J.gpar.base + (long long)qi * J.gpar.plane);
         const int pa = parent(item, pr, 0), pb = parent(item, pr, 1);
         double2 z = make_double2(0.0, 0.0);
         if (m < gcnt) {
@@ -2414,7 +2419,7 @@ __device__ __forceinline__ double2 load_view_pair(const CoefView &V, int qi, int
     const double *b = V.base + (long long)qi * V.plane + (long long)(2 * p) * V.stride + m;
     const double *px = in ? b : kZeroOne;
     const double *py = (in && has_r) ? b + V.stride : kZeroOne + (one ? 1 : 0);
-    return make_double2(*px, *py);
+    return make_double2(*as_global(px), *as_global(py));
 }
 
 // Item maps as the forward kernels, with one parent per item ("pair" = parent).
@@ -2479,8 +2484,10 @@ __device__ __forceinline__ void build_lds_t_body(const LevelJob &J, double2 *lds
         int qi, p;
         M.at(item, qi, p);
         if (p >= J.nr_parent) return;
-        double *pp = J.parent.base + (long long)qi * J.parent.plane + (long long)p * J.parent.stride;
-        double *dp = J.tparent.base + (long long)qi * J.tparent.plane + (long long)p * J.tparent.stride;
+        gdouble_rw *pp =
+            as_global_rw(J.parent.base + (long long)qi * J.parent.plane + (long long)p * J.parent.stride);
+        gdouble_rw *dp =
+            as_global_rw(J.tparent.base + (long long)qi * J.tparent.plane + (long long)p * J.tparent.stride);
         const bool has_r = (2 * p + 1) < J.nr_child;
         auto exact = [&](int mm, double &pv, double &dv) {
             const double l = view_at(J.child, qi, 2 * p, mm), dl = view_at(J.tchild, qi, 2 * p, mm);
@@ -2591,7 +2598,8 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
         int qi, p;
         M.at(item, qi, p);
         if (p < J.nr_parent && m < J.gchild.count) {
-            double *g = J.gchild.base + (long long)qi * J.gchild.plane + (long long)(2 * p) * J.gchild.stride;
+            gdouble_rw *g = as_global_rw(J.gchild.base + (long long)qi * J.gchild.plane +
+                                         (long long)(2 * p) * J.gchild.stride);
             g[m] = v.x * invF;
             if (2 * p + 1 < J.nr_child) g[J.gchild.stride + m] = v.y * invF;
         }
@@ -2642,7 +2650,8 @@ __device__ __forceinline__ void prop_lds_t_body(const LevelJob &J, double2 *lds,
                 bl += pr * (bF - b0) + dr * (aF - a0);
                 br += pl * (bF - b0) + dl * (aF - a0);
             }
-            double *bb = J.bchild.base + (long long)qi * J.bchild.plane + (long long)(2 * p) * J.bchild.stride;
+            gdouble_rw *bb = as_global_rw(J.bchild.base + (long long)qi * J.bchild.plane +
+                                          (long long)(2 * p) * J.bchild.stride);
             bb[m] = bl;
             if (2 * p + 1 < J.nr_child) bb[J.bchild.stride + m] = br;
         }
