@@ -102,15 +102,31 @@ __global__ __launch_bounds__(64) void k_leaf_build(LeafJob J) {
     if (b < J.nb) {
         const double lv = J.logv[qi];
         const long long arow = (long long)qi * J.n;
+        // the players' stacks (or a values) are loaded CH at a time,
+        // unconditionally (past n: kZeroOne), so each chunk waits once:
+        // loaded under a per-player branch, every player waited for its
+        // own load, one memory latency per player
+        constexpr int CH = B < 16 ? B : 16;
+        const double *src = J.a_in ? J.a_in + arow : J.S;
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int j = b * B + i;
-            double a = 1.0;
-            if (j < J.n) a = J.a_in ? J.a_in[arow + j] : a_of(J.S[j], lv);
-            const double bb = 1.0 - a;
+        for (int i0 = 0; i0 < B; i0 += CH) {
+            double sv[CH];
 #pragma unroll
-            for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
-            P[0] *= a;
+            for (int u = 0; u < CH; ++u) {
+                const int j = b * B + i0 + u;
+                sv[u] = *as_global(j < J.n ? src + j : kZeroOne);
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int i = i0 + u;
+                const int j = b * B + i;
+                double a = 1.0;
+                if (j < J.n) a = J.a_in ? sv[u] : a_of(sv[u], lv);
+                const double bb = 1.0 - a;
+#pragma unroll
+                for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
+                P[0] *= a;
+            }
         }
     }
     store_block_products<B>(P, sh, J.blocks + (long long)qi * J.nb * (B + 1) + (long long)b0 * (B + 1),
@@ -704,28 +720,45 @@ __global__ __launch_bounds__(64) void k_leaf_build_vjp(VjpLeafJob V) {
         const double lv = J.logv[qi], wq = J.wq[qi];
         const double ivv = exp(-lv);
         const long long arow = (long long)qi * J.n;
+        // operands loaded CH players at a time, unconditionally (as
+        // k_leaf_build): stack, a value (with a_in) and c_in or v
+        constexpr int CH = 8;
+        static_assert(B % CH == 0, "whole chunks");
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int j = b * B + i;
-            double a = 1.0, c = 0.0;
-            if (j < J.n) {
-                a = J.a_in ? J.a_in[arow + j] : a_of(J.S[j], lv);
-                if (V.c_in) {
-                    c = V.c_in[arow + j];
-                } else {
-                    double pw = wq * J.S[j] * a * ivv;
-                    if (!isfinite(pw)) pw = 0.0;
-                    c = V.v[j] * pw;
+        for (int i0 = 0; i0 < B; i0 += CH) {
+            double sv[CH], av[CH], cv[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int j = b * B + i0 + u;
+                const bool ok = j < J.n;
+                sv[u] = *as_global(ok ? J.S + j : kZeroOne);
+                av[u] = J.a_in ? *as_global(ok ? J.a_in + arow + j : kZeroOne) : 0.0;
+                cv[u] = *as_global(ok ? (V.c_in ? V.c_in + arow + j : V.v + j) : kZeroOne);
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int i = i0 + u;
+                const int j = b * B + i;
+                double a = 1.0, c = 0.0;
+                if (j < J.n) {
+                    a = J.a_in ? av[u] : a_of(sv[u], lv);
+                    if (V.c_in) {
+                        c = cv[u];
+                    } else {
+                        double pw = wq * sv[u] * a * ivv;
+                        if (!isfinite(pw)) pw = 0.0;
+                        c = cv[u] * pw;
+                    }
                 }
-            }
-            const double bb = 1.0 - a;
+                const double bb = 1.0 - a;
 #pragma unroll
-            for (int m = i + 1; m >= 1; --m) {
-                D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
-                P[m] = a * P[m] + bb * P[m - 1];
+                for (int m = i + 1; m >= 1; --m) {
+                    D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
+                    P[m] = a * P[m] + bb * P[m - 1];
+                }
+                D[0] = a * D[0] + c * P[0];
+                P[0] *= a;
             }
-            D[0] = a * D[0] + c * P[0];
-            P[0] *= a;
         }
     }
     const int nblk = min(64, J.nb - b0);
