@@ -1192,25 +1192,40 @@ __global__ __launch_bounds__(64) void k_leaf_build_vjp_batch(const BatchInst *__
         const double lv = I.logv[q0 + qi], wq = I.w[q0 + qi];
         const double ivv = exp(-lv);
         const int b = gb - I.boff;
+        // stacks and v loaded eight players at a time, unconditionally (as
+        // k_leaf_build_vjp)
+        constexpr int CH = 8;
+        static_assert(B % CH == 0, "whole chunks");
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int j = b * B + i;
-            double a = 1.0, c = 0.0;
-            if (j < I.n) {
-                const double Sj = S[I.poff + j];
-                a = a_of(Sj, lv);
-                double pw = wq * Sj * a * ivv;
-                if (!isfinite(pw)) pw = 0.0;
-                c = v[I.poff + j] * pw;
-            }
-            const double bb = 1.0 - a;
+        for (int i0 = 0; i0 < B; i0 += CH) {
+            double sv[CH], vv[CH];
 #pragma unroll
-            for (int m = i + 1; m >= 1; --m) {
-                D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
-                P[m] = a * P[m] + bb * P[m - 1];
+            for (int u = 0; u < CH; ++u) {
+                const int j = b * B + i0 + u;
+                const bool ok = j < I.n;
+                sv[u] = *as_global(ok ? S + I.poff + j : kZeroOne);
+                vv[u] = *as_global(ok ? v + I.poff + j : kZeroOne);
             }
-            D[0] = a * D[0] + c * P[0];
-            P[0] *= a;
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int j = b * B + i0 + u;
+                double a = 1.0, c = 0.0;
+                if (j < I.n) {
+                    const double Sj = sv[u];
+                    a = a_of(Sj, lv);
+                    double pw = wq * Sj * a * ivv;
+                    if (!isfinite(pw)) pw = 0.0;
+                    c = vv[u] * pw;
+                }
+                const double bb = 1.0 - a;
+#pragma unroll
+                for (int m = i0 + u + 1; m >= 1; --m) {
+                    D[m] = a * D[m] + bb * D[m - 1] + c * P[m];
+                    P[m] = a * P[m] + bb * P[m - 1];
+                }
+                D[0] = a * D[0] + c * P[0];
+                P[0] *= a;
+            }
         }
     }
     const int nblk = min(64, NB - b0);
