@@ -445,14 +445,25 @@ __global__ __launch_bounds__(64) void k_leaf_build_batch(const BatchInst *__rest
         const BatchInst &I = inst[blk_inst[gb]];
         const double lv = I.logv[q0 + qi];
         const int b = gb - I.boff;
+        // stacks loaded CH players at a time, unconditionally (as k_leaf_build)
+        constexpr int CH = B < 16 ? B : 16;
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const int j = b * B + i;
-            const double a = (j < I.n) ? a_of(S[I.poff + j], lv) : 1.0;
-            const double bb = 1.0 - a;
+        for (int i0 = 0; i0 < B; i0 += CH) {
+            double sv[CH];
 #pragma unroll
-            for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
-            P[0] *= a;
+            for (int u = 0; u < CH; ++u) {
+                const int j = b * B + i0 + u;
+                sv[u] = *as_global(j < I.n ? S + I.poff + j : kZeroOne);
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int i = i0 + u;
+                const double a = (b * B + i < I.n) ? a_of(sv[u], lv) : 1.0;
+                const double bb = 1.0 - a;
+#pragma unroll
+                for (int m = i + 1; m >= 1; --m) P[m] = a * P[m] + bb * P[m - 1];
+                P[0] *= a;
+            }
         }
     }
     store_block_products<B>(P, sh, blocks + (long long)qi * NB * (B + 1) + (long long)b0 * (B + 1), min(64, NB - b0));
